@@ -1,0 +1,673 @@
+// tga_batch.hip -- thousands of independent group-by queries in ONE persistent
+// launch fed by a device work queue, captured in a hipGraph.
+//
+// The reference runs one query per call and blocks six times per tile
+// (T/src/group_then_partition.cu:97,173,221,386,409,418); a batch of small
+// queries would pay those round trips and per-kernel launch costs per query.
+// Here the host builds, once per batch, an ordered list of work items:
+//
+//   INIT     clear a slice of a query's global table
+//   COUNT    partition histogram of one tile           (large-cardinality queries)
+//   DIRECT   LDS aggregation of a row range -> table   (small-cardinality queries)
+//   SCAN     exclusive scan of a query's tile histogram
+//   SCATTER  LDS counting sort + coalesced runs of one tile
+//   PAGG     LDS aggregation of one partition -> table
+//   COMPACT  compaction of a slice of a query's table into its output
+//
+// Workgroups pull items from an atomic ticket.  An item may depend only on
+// items earlier in the list (INIT < COUNT < DIRECT < SCAN < SCATTER < PAGG <
+// COMPACT), so every item it waits for is finished or held by a running
+// workgroup: the spin waits always end.  Producers publish with an
+// agent-scope release before bumping their stage counter; consumers acquire
+// after observing it.
+#include <algorithm>
+#include <numeric>
+
+#include "tga_batch_internal.hpp"
+#include "tga_device.hpp"
+
+namespace tga {
+
+constexpr int kBatchThreads = 512;
+constexpr uint32_t kCompactSlice = 65536;
+
+__device__ __forceinline__ void wait_done(const BQuery &Q, uint32_t type) {
+  if (Q.need[type] == 0) return;
+  uint32_t *p = &Q.done[type];
+  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < Q.need[type]) __builtin_amdgcn_s_sleep(2);
+}
+
+template <typename K>
+__device__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
+  const GTable &g = Q.g;
+  const uint64_t n = g.S + 1;
+  K *keys = (K *)g.keys;
+  for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) keys[i] = KeyTraits<K>::kEmpty;
+  for (int c = 0; c < Q.s.nacc; ++c) {
+    const int kind = Q.s.acc[c].kind;
+    if (kind == ACC_SUM_F) {
+      for (int l = 0; l < kLimbs; ++l)
+        for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint64_t *)g.acc[c])[(uint64_t)l * n + i] = 0;
+    } else if (acc_global_bytes(kind) == 4) {
+      const uint32_t id = (uint32_t)acc_identity(kind);
+      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint32_t *)g.acc[c])[i] = id;
+    } else {
+      const uint64_t id = acc_identity(kind);
+      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint64_t *)g.acc[c])[i] = id;
+    }
+  }
+}
+
+template <typename K>
+__device__ void item_count(const BQuery &Q, uint32_t tile, uint32_t *hist) {
+  const uint32_t P = 1u << Q.pbits;
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const K *keys = (const K *)Q.keys;
+  const uint64_t b = (uint64_t)tile * Q.tile_rows, e = min(Q.n, b + Q.tile_rows);
+  for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
+    atomicAdd(&hist[hash_partition(key_hash((uint64_t)keys[r]), Q.pbits)], 1u);
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) Q.cnt[(uint64_t)p * Q.ntiles + tile] = hist[p];
+}
+
+// One workgroup scans the whole [P][ntiles] histogram of a query.
+__device__ void item_scan(const BQuery &Q, uint32_t *buf, uint32_t *scratch) {
+  const uint32_t P = 1u << Q.pbits;
+  const uint64_t cells = (uint64_t)P * Q.ntiles;
+  const uint32_t chunk = blockDim.x * 8;
+  uint32_t run = 0;
+  for (uint64_t base = 0; base < cells; base += chunk) {
+    const uint32_t m = (uint32_t)min<uint64_t>(chunk, cells - base);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = Q.cnt[base + i];
+    __syncthreads();
+    const uint32_t tot = block_exclusive_scan_lds(buf, m, scratch);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+      const uint64_t idx = base + i;
+      Q.offs[idx] = run + buf[i];
+      if (idx % Q.ntiles == 0) Q.pstart[idx / Q.ntiles] = run + buf[i];  // first tile of partition p
+    }
+    run += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) Q.pstart[P] = Q.n;
+}
+
+template <typename K>
+__device__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_t *scratch) {
+  constexpr int ITEMS = 16;
+  const uint32_t P = 1u << Q.pbits;
+  const uint32_t TR = Q.tile_rows;  // == blockDim.x * ITEMS
+  uint32_t *hist = (uint32_t *)smem;
+  uint32_t *goff = hist + P;
+  K *skey = (K *)(goff + P);
+  uint32_t *sval = (uint32_t *)(skey + TR);  // one 32-bit value column
+  uint16_t *spid = (uint16_t *)(sval + TR);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const K *keys = (const K *)Q.keys;
+  const uint32_t *vals = (const uint32_t *)Q.s.cols[0];
+  const uint64_t b = (uint64_t)tile * TR;
+  K kk[ITEMS];
+  uint32_t vv[ITEMS], pp[ITEMS], rk[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    const bool act = row < Q.n;
+    kk[i] = act ? keys[row] : KeyTraits<K>::kEmpty;
+    vv[i] = act ? vals[row] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    pp[i] = hash_partition(key_hash((uint64_t)kk[i]), Q.pbits);
+    rk[i] = row < Q.n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
+  }
+  __syncthreads();
+  block_exclusive_scan_lds(hist, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = Q.offs[(uint64_t)p * Q.ntiles + tile];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    if (row < Q.n) {
+      const uint32_t lp = hist[pp[i]] + rk[i];
+      skey[lp] = kk[i];
+      sval[lp] = vv[i];
+      spid[lp] = (uint16_t)pp[i];
+    }
+  }
+  __syncthreads();
+  const uint32_t rows = (uint32_t)min<uint64_t>(TR, Q.n - b);
+  for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+    const uint32_t p = spid[j];
+    const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
+    ((K *)Q.pkeys)[dst] = skey[j];
+    ((uint32_t *)Q.pcols[0])[dst] = sval[j];
+  }
+}
+
+template <typename K>
+__device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e, char *smem,
+                         uint32_t *sentinel) {
+  LdsTable<K> t{smem, Q.L, sentinel};
+  lds_init(t, s);
+  __syncthreads();
+  agg_rows<K>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
+  __syncthreads();
+  lds_flush<K>(t, s, Q.g);
+}
+
+// Compacts table slots [a, b) of one query; output rows are reserved with an
+// atomic on the query's counter (order inside a query is unspecified).
+template <typename K>
+__device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *scratch, uint64_t *s_base) {
+  const GTable &g = Q.g;
+  const uint64_t n = g.S + 1;
+  const K *keys = (const K *)g.keys;
+  for (uint64_t base = a; base < b; base += blockDim.x) {
+    const uint64_t slot = base + threadIdx.x;
+    bool occ = false;
+    if (slot < b) {
+      if (slot < g.S) occ = keys[slot] != KeyTraits<K>::kEmpty;
+      else occ = g.flags[FLAG_SENTINEL] != 0;
+    }
+    const uint64_t bal = __ballot(occ);
+    const uint32_t pre = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
+    const int wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) scratch[wave] = (uint32_t)__builtin_popcountll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (uint32_t w = 0; w < blockDim.x / kWave; ++w) { const uint32_t c = scratch[w]; scratch[w] = run; run += c; }
+      *s_base = run ? atomicAdd((unsigned long long *)Q.count_out, (unsigned long long)run) : 0;
+    }
+    __syncthreads();
+    if (occ) {
+      const uint64_t pos = *s_base + scratch[wave] + pre;
+      if (pos < Q.od.capacity) {
+        ((K *)Q.od.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
+        for (int c = 0; c < Q.s.nacc; ++c) {
+          const int kind = Q.s.acc[c].kind;
+          if (kind == ACC_COUNT || kind == ACC_SUM_I) ((uint64_t *)Q.od.cols[c])[pos] = ((const uint64_t *)g.acc[c])[slot];
+          else if (kind == ACC_MIN_I32 || kind == ACC_MAX_I32) ((uint32_t *)Q.od.cols[c])[pos] = ((const uint32_t *)g.acc[c])[slot];
+          else if (kind == ACC_SUM_F) {
+            int64_t Lm[kLimbs];
+            for (int l = 0; l < kLimbs; ++l) Lm[l] = ((const int64_t *)g.acc[c])[(uint64_t)l * n + slot];
+            ((double *)Q.od.cols[c])[pos] = fx_to_double(Lm);
+          } else {
+            const int64_t v = ((const int64_t *)g.acc[c])[slot];
+            const int dt = Q.s.col_dtype[Q.s.acc[c].col];
+            if (dt == DT_U64) ((uint64_t *)Q.od.cols[c])[pos] = (uint64_t)v ^ (1ULL << 63);
+            else if (dt == DT_F64) ((uint64_t *)Q.od.cols[c])[pos] = ordered_to_f64_bits(v);
+            else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)Q.od.cols[c])[pos] = (uint32_t)v;
+            else ((int64_t *)Q.od.cols[c])[pos] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename K>
+__device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
+                         uint64_t *s_base) {
+  switch (it.type) {
+    case IT_INIT: item_init<K>(Q, it.a, it.b); break;
+    case IT_COUNT: item_count<K>(Q, (uint32_t)it.a, (uint32_t *)smem); break;
+    case IT_DIRECT: item_agg<K>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel); break;
+    case IT_SCAN: item_scan(Q, (uint32_t *)smem, scratch); break;
+    case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch); break;
+    case IT_PAGG: {
+      Schema sp = Q.s;
+      for (int c = 0; c < sp.ncols; ++c) sp.cols[c] = Q.pcols[c];
+      item_agg<K>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel);
+      break;
+    }
+    case IT_COMPACT: item_compact<K>(Q, it.a, it.b, scratch, s_base); break;
+    default: break;
+  }
+}
+
+__global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
+                                                         uint32_t nitems, uint32_t *ticket) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t s_item, s_sentinel;
+  __shared__ uint32_t scratch[kBatchThreads / kWave + 1];
+  __shared__ uint64_t s_base;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t i = s_item;
+    if (i >= nitems) break;
+    const BItem it = items[i];
+    const BQuery &Q = qs[it.q];
+    if (threadIdx.x == 0) {
+      switch (it.type) {
+        case IT_DIRECT: wait_done(Q, IT_INIT); break;
+        case IT_SCAN: wait_done(Q, IT_COUNT); break;
+        case IT_SCATTER: wait_done(Q, IT_SCAN); break;
+        case IT_PAGG: wait_done(Q, IT_SCATTER); wait_done(Q, IT_INIT); break;
+        case IT_COMPACT: wait_done(Q, IT_DIRECT); wait_done(Q, IT_PAGG); wait_done(Q, IT_INIT); break;
+        default: break;
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (Q.key_bytes == 4) run_item<uint32_t>(Q, it, smem, scratch, &s_sentinel, &s_base);
+    else run_item<uint64_t>(Q, it, smem, scratch, &s_sentinel, &s_base);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(&Q.done[it.type], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_batch_reset(uint32_t *done, uint64_t ndone, uint64_t *counts, uint64_t nq, uint32_t *ticket) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ndone; i += (uint64_t)gridDim.x * blockDim.x) done[i] = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) counts[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0;
+}
+
+}  // namespace tga
+
+
+namespace tga {
+
+namespace {
+uint64_t np2(uint64_t x) { uint64_t p = 1; while (p < x) p <<= 1; return p; }
+int lg2(uint64_t x) { int b = 0; while ((1ULL << b) < x) ++b; return b; }
+
+// Per-query shape decisions of the batched path (kept in one place so the
+// admission estimate and the build agree).
+struct QShape {
+  bool part = false;
+  uint32_t T = 256;
+  uint64_t S = 256;
+  uint64_t cap = 1;
+  int pbits = 0;
+  uint32_t tile_rows = 0, ntiles = 0;
+};
+
+QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est) {
+  QShape z;
+  const size_t lds_max = ctx->lds_max - 2048;
+  const uint64_t G = std::max<uint64_t>(1, std::min<uint64_t>(est, std::max<uint64_t>(n, 1)));
+  const uint32_t sb = lds_slot_bytes(q.schema, q.key_bytes);
+  uint32_t tmax = 256;
+  while ((uint64_t)(tmax * 2 + 1) * sb + 256 <= lds_max && tmax < 8192) tmax *= 2;
+  z.part = q.schema.ncols == 1 && dtype_bytes(q.schema.col_dtype[0]) == 4 && 2 * G > tmax && n > 65536;
+  z.T = z.part ? tmax : (uint32_t)std::min<uint64_t>(tmax, std::max<uint64_t>(256, np2(2 * G)));
+  z.S = std::min<uint64_t>(np2(std::max<uint64_t>(256, 2 * G)), np2(std::max<uint64_t>(256, 2 * n)));
+  z.cap = std::min<uint64_t>(z.S + 1, n + 1);
+  if (z.part) {
+    z.pbits = std::max(1, std::min(12, lg2(np2((2 * G + z.T - 1) / z.T))));
+    z.tile_rows = kBatchThreads * 16;
+    z.ntiles = (uint32_t)((n + z.tile_rows - 1) / z.tile_rows);
+  }
+  return z;
+}
+}  // namespace
+
+BatchExec::~BatchExec() {
+  if (h_counts) hipHostFree(h_counts);
+  if (h_flags) hipHostFree(h_flags);
+}
+
+uint64_t batch_query_bytes(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est) {
+  const QShape z = shape_of(ctx, q, n, est);
+  uint64_t b = (z.S + 1) * q.key_bytes;
+  for (int a = 0; a < q.schema.nacc; ++a) b += (z.S + 1) * acc_global_bytes(q.schema.acc[a].kind);
+  b += z.cap * q.key_bytes;
+  for (int a = 0; a < q.schema.nacc; ++a) b += z.cap * dtype_bytes(q.out_dtype[a]);
+  if (z.part) {
+    const uint64_t cells = (uint64_t)(1u << z.pbits) * z.ntiles;
+    b += cells * 8 + ((1ULL << z.pbits) + 1) * 8 + n * (q.key_bytes + 4);
+  }
+  return b + 64 * 1024;
+}
+
+tga_status batch_prepare(tga_context *ctx, const tga_query *queries, int32_t n, BatchExec &b) {
+  b.ctx = ctx;
+  b.nq = n;
+  b.spec.resize(n);
+  b.rows.resize(n);
+  b.keys.resize(n);
+  for (int32_t i = 0; i < n; ++i) {
+    const tga_query &q = queries[i];
+    std::string e = build_spec(&q.key, q.values, q.n_values, q.aggs, q.n_aggs, b.spec[i]);
+    if (e.empty() && q.n_rows < 0) e = "negative row count";
+    if (e.empty() && q.key.location != TGA_DEVICE) e = "batched queries need device-resident columns";
+    for (int c = 0; e.empty() && c < q.n_values; ++c)
+      if (q.values[c].location != TGA_DEVICE) e = "batched queries need device-resident columns";
+    if (e.empty() && q.n_rows > 0 && !q.key.data) e = "null key column";
+    if (!e.empty()) return fail(ctx, TGA_ERR_INVALID_ARG, "query %d: %s", i, e.c_str());
+    b.rows[i] = (uint64_t)q.n_rows;
+    b.keys[i] = q.key.data;
+  }
+  // estimates: the caller's hint, else up to 1024 strided keys per query
+  // copied in one transfer and fed to the GEE estimator
+  b.est.assign(n, 0);
+  std::vector<int32_t> todo;
+  for (int32_t i = 0; i < n; ++i) {
+    if (queries[i].est_groups > 0) b.est[i] = (uint64_t)queries[i].est_groups;
+    else if (queries[i].n_rows > 0) todo.push_back(i);
+    else b.est[i] = 1;
+  }
+  if (todo.empty()) return TGA_OK;
+  const uint64_t M = 1024;
+  DBuf buf;
+  if (!buf.alloc(ctx, todo.size() * M * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch sample");
+  std::vector<uint64_t> ms(todo.size());
+  for (size_t j = 0; j < todo.size(); ++j) {
+    const tga_query &q = queries[todo[j]];
+    ms[j] = std::min<uint64_t>(M, (uint64_t)q.n_rows);
+    TGA_HIP(ctx, launch_sample_keys(dtype_bytes(q.key.dtype), q.key.data, (uint64_t)q.n_rows, ms[j],
+                                    buf.as<uint64_t>() + j * M, ctx->stream));
+  }
+  std::vector<uint64_t> h(todo.size() * M);
+  TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (size_t j = 0; j < todo.size(); ++j) {
+    uint64_t *p = h.data() + j * M;
+    const uint64_t m = ms[j];
+    std::sort(p, p + m);
+    uint64_t f1 = 0, rest = 0;
+    for (uint64_t i = 0; i < m;) {
+      uint64_t k = i;
+      while (k < m && p[k] == p[i]) ++k;
+      if (k - i == 1) ++f1; else ++rest;
+      i = k;
+    }
+    const uint64_t rows = (uint64_t)queries[todo[j]].n_rows;
+    const double d = std::sqrt((double)rows / (double)m) * (double)f1 + (double)rest;
+    b.est[todo[j]] = (uint64_t)std::min<double>((double)rows, std::max(1.0, d));
+  }
+  return TGA_OK;
+}
+
+tga_status batch_build(BatchExec &b, const tga_query *qs) {
+  tga_context *ctx = b.ctx;
+  const int32_t n = b.nq;
+  b.hq.assign(n, BQuery{});
+  b.cap.assign(n, 0);
+  b.okeys.clear();
+  b.okeys.resize(n);
+  b.ocols.clear();
+  b.ocols.resize(n);
+  std::vector<std::vector<BItem>> lists(kItemTypes);
+  size_t lds_need = 0;
+  const uint64_t total_done = (uint64_t)n * kItemTypes;
+  if (!b.ddone.alloc(ctx, total_done * 4) || !b.dcounts.alloc(ctx, (uint64_t)n * 8) || !b.dticket.alloc(ctx, 64) ||
+      !b.dflags.alloc(ctx, (uint64_t)n * kFlagWords * 4))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch counters");
+  if (hipHostMalloc((void **)&b.h_counts, std::max(1, n) * 8) != hipSuccess ||
+      hipHostMalloc((void **)&b.h_flags, std::max(1, n) * kFlagWords * 4) != hipSuccess)
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch status");
+  for (int32_t i = 0; i < n; ++i) {
+    const QuerySpec &q = b.spec[i];
+    BQuery &Q = b.hq[i];
+    Q.s = q.schema;
+    Q.keys = qs[i].key.data;
+    Q.n = b.rows[i];
+    Q.key_bytes = q.key_bytes;
+    Q.done = b.ddone.as<uint32_t>() + (uint64_t)i * kItemTypes;
+    Q.count_out = b.dcounts.as<uint64_t>() + i;
+    const QShape z = shape_of(ctx, q, Q.n, b.est[i]);
+    Q.path = z.part ? TGA_PATH_PARTITIONED : TGA_PATH_DIRECT;
+    Q.L = make_layout(q.schema, q.key_bytes, z.T);
+    Q.nbits = lg2((uint64_t)z.T + 1);
+    Q.max_probe = (int)std::min<uint32_t>(z.T, 256);
+    lds_need = std::max<size_t>(lds_need, Q.L.bytes);
+    // global table; its status words live in the batch's flag block
+    {
+      const uint64_t S = z.S;
+      DBuf k;
+      if (!k.alloc(ctx, (S + 1) * q.key_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch tables");
+      Q.g.keys = k.p;
+      b.bufs.push_back(std::move(k));
+      for (int a = 0; a < q.schema.nacc; ++a) {
+        DBuf c;
+        if (!c.alloc(ctx, (S + 1) * acc_global_bytes(q.schema.acc[a].kind)))
+          return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch tables");
+        Q.g.acc[a] = c.p;
+        b.bufs.push_back(std::move(c));
+      }
+      Q.g.S = S;
+      Q.g.flags = b.dflags.as<uint32_t>() + (uint64_t)i * kFlagWords;
+    }
+    b.cap[i] = z.cap;
+    Q.od.capacity = z.cap;
+    if (!b.okeys[i].alloc(ctx, z.cap * q.key_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch outputs");
+    Q.od.keys = b.okeys[i].p;
+    for (int a = 0; a < q.schema.nacc; ++a) {
+      if (!b.ocols[i][a].alloc(ctx, z.cap * dtype_bytes(q.out_dtype[a])))
+        return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch outputs");
+      Q.od.cols[a] = b.ocols[i][a].p;
+    }
+    for (uint64_t a = 0; a < z.S + 1; a += kCompactSlice)
+      lists[IT_INIT].push_back({(uint32_t)i, IT_INIT, a, std::min<uint64_t>(z.S + 1, a + kCompactSlice)});
+    if (Q.n > 0 && !z.part) {
+      // enough rows per item to amortise the table flush
+      const uint64_t per = std::max<uint64_t>(16384, std::min<uint64_t>(262144, 16 * (uint64_t)z.T));
+      for (uint64_t a = 0; a < Q.n; a += per)
+        lists[IT_DIRECT].push_back({(uint32_t)i, IT_DIRECT, a, std::min<uint64_t>(Q.n, a + per)});
+    } else if (Q.n > 0) {
+      Q.pbits = z.pbits;
+      const uint32_t PP = 1u << z.pbits;
+      Q.tile_rows = z.tile_rows;
+      Q.ntiles = z.ntiles;
+      const uint64_t cells = (uint64_t)PP * Q.ntiles;
+      DBuf c1, c2, c3, c4, c5;
+      if (!c1.alloc(ctx, cells * 4) || !c2.alloc(ctx, cells * 4) || !c3.alloc(ctx, (PP + 1) * 8) ||
+          !c4.alloc(ctx, Q.n * q.key_bytes) || !c5.alloc(ctx, Q.n * 4))
+        return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch partition buffers");
+      Q.cnt = c1.as<uint32_t>();
+      Q.offs = c2.as<uint32_t>();
+      Q.pstart = c3.as<uint64_t>();
+      Q.pkeys = c4.p;
+      Q.pcols[0] = c5.p;
+      b.bufs.push_back(std::move(c1));
+      b.bufs.push_back(std::move(c2));
+      b.bufs.push_back(std::move(c3));
+      b.bufs.push_back(std::move(c4));
+      b.bufs.push_back(std::move(c5));
+      lds_need = std::max(lds_need, (size_t)8 * PP + (size_t)Q.tile_rows * (q.key_bytes + 4 + 2));
+      lds_need = std::max<size_t>(lds_need, (size_t)4 * std::min<uint64_t>(cells, kBatchThreads * 8));
+      for (uint32_t t = 0; t < Q.ntiles; ++t) lists[IT_COUNT].push_back({(uint32_t)i, IT_COUNT, t, 0});
+      lists[IT_SCAN].push_back({(uint32_t)i, IT_SCAN, 0, 0});
+      for (uint32_t t = 0; t < Q.ntiles; ++t) lists[IT_SCATTER].push_back({(uint32_t)i, IT_SCATTER, t, 0});
+      for (uint32_t p = 0; p < PP; ++p) lists[IT_PAGG].push_back({(uint32_t)i, IT_PAGG, p, 0});
+    }
+    for (uint64_t a = 0; a < z.S + 1; a += kCompactSlice)
+      lists[IT_COMPACT].push_back({(uint32_t)i, IT_COMPACT, a, std::min<uint64_t>(z.S + 1, a + kCompactSlice)});
+  }
+  for (auto &l : lists)
+    for (auto &it : l) b.hq[it.q].need[it.type] += 1;
+  if (lds_need > ctx->lds_max) return fail(ctx, TGA_ERR_UNSUPPORTED, "batch needs %zu B of LDS", lds_need);
+  // stage by stage; the largest queries first inside a stage
+  std::vector<BItem> items;
+  for (int t = 0; t < kItemTypes; ++t) {
+    auto &l = lists[t];
+    std::stable_sort(l.begin(), l.end(), [&](const BItem &x, const BItem &y) { return b.hq[x.q].n > b.hq[y.q].n; });
+    items.insert(items.end(), l.begin(), l.end());
+  }
+  b.nitems = (uint32_t)items.size();
+  if (!b.dq.alloc(ctx, std::max(1, n) * sizeof(BQuery)) || !b.ditems.alloc(ctx, std::max<size_t>(1, items.size()) * sizeof(BItem)))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch queue");
+  if (n) TGA_HIP(ctx, hipMemcpyAsync(b.dq.p, b.hq.data(), (uint64_t)n * sizeof(BQuery), hipMemcpyHostToDevice, ctx->stream));
+  if (!items.empty())
+    TGA_HIP(ctx, hipMemcpyAsync(b.ditems.p, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  b.lds = std::max<size_t>(lds_need, 1024);
+  if (b.lds > 65536)
+    TGA_HIP(ctx, hipFuncSetAttribute((const void *)&k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+  const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / b.lds));
+  b.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->cus * per_cu, b.nitems));
+  return TGA_OK;
+}
+
+tga_status batch_enqueue(BatchExec &b, hipStream_t st) {
+  tga_context *ctx = b.ctx;
+  const uint64_t n = (uint64_t)b.nq;
+  hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, b.ddone.as<uint32_t>(), n * kItemTypes,
+                     b.dcounts.as<uint64_t>(), n, b.dticket.as<uint32_t>());
+  TGA_HIP(ctx, hipGetLastError());
+  if (b.nitems) {
+    hipLaunchKernelGGL(k_batch, dim3(b.grid), dim3(kBatchThreads), b.lds, st, b.dq.as<BQuery>(), b.ditems.as<BItem>(),
+                       b.nitems, b.dticket.as<uint32_t>());
+    TGA_HIP(ctx, hipGetLastError());
+  }
+  if (n) {
+    TGA_HIP(ctx, hipMemcpyAsync(b.h_counts, b.dcounts.p, n * 8, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipMemcpyAsync(b.h_flags, b.dflags.p, n * kFlagWords * 4, hipMemcpyDeviceToHost, st));
+  }
+  return TGA_OK;
+}
+
+tga_status batch_finalize(BatchExec &b, std::vector<uint64_t> &groups) {
+  tga_context *ctx = b.ctx;
+  groups.assign(b.nq, 0);
+  for (int32_t i = 0; i < b.nq; ++i) {
+    const uint32_t *fl = b.h_flags + (uint64_t)i * kFlagWords;
+    if (fl[FLAG_RANGE]) return fail(ctx, TGA_ERR_RANGE, "query %d: fp64 SUM input outside the exact window", i);
+    groups[i] = b.h_counts[i];
+    if (fl[FLAG_OVERFLOW] || b.h_counts[i] > b.cap[i]) {
+      // the estimate was far too low: this query alone, with a larger table
+      const uint64_t est = std::max<uint64_t>(b.h_counts[i], b.hq[i].g.S) * 4;
+      Plan p = autotune(ctx, b.spec[i], b.rows[i], est, nullptr);
+      PlainOut po;
+      uint64_t G = 0;
+      tga_status st = run_plain(ctx, b.spec[i], b.keys[i], b.rows[i], p, &po, nullptr, nullptr, 0, &G);
+      if (st) return st;
+      b.okeys[i] = std::move(po.keys);
+      for (int a = 0; a < b.spec[i].schema.nacc; ++a) b.ocols[i][a] = std::move(po.cols[a]);
+      b.cap[i] = po.capacity;
+      groups[i] = G;
+    }
+  }
+  return TGA_OK;
+}
+
+}  // namespace tga
+
+using namespace tga;
+
+struct tga_batch {
+  tga_context *ctx = nullptr;
+  int mode = 0;
+  BatchExec ex;
+  std::vector<uint64_t> groups;
+  std::vector<Plan> plan;  // per-query mode
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+#define TGA_API extern "C" __attribute__((visibility("default")))
+
+TGA_API tga_status tga_batch_create(tga_context *ctx, const tga_query *queries, int32_t n, int32_t mode,
+                                    tga_batch **out) {
+  if (!ctx || !out || n < 0 || (n > 0 && !queries) || (mode != TGA_BATCH_PERSISTENT && mode != TGA_BATCH_PER_QUERY))
+    return TGA_ERR_INVALID_ARG;
+  *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  auto *b = new tga_batch();
+  b->ctx = ctx;
+  b->mode = mode;
+  b->groups.assign(n, 0);
+  tga_status st = batch_prepare(ctx, queries, n, b->ex);
+  if (!st && mode == TGA_BATCH_PERSISTENT) {
+    st = batch_build(b->ex, queries);
+    if (!st) {
+      // one graph: reset kernel + persistent kernel + status copies
+      hipError_t be = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+      if (be != hipSuccess) st = fail(ctx, TGA_ERR_HIP, "begin capture: %s", hipGetErrorString(be));
+      else {
+        tga_status est = batch_enqueue(b->ex, ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (est || ce != hipSuccess) {
+          if (g) hipGraphDestroy(g);
+          st = est ? est : fail(ctx, TGA_ERR_HIP, "end capture: %s", hipGetErrorString(ce));
+        } else {
+          b->graph = g;
+          hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+          if (ie != hipSuccess) st = fail(ctx, TGA_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+        }
+      }
+    }
+  } else if (!st) {
+    b->plan.resize(n);
+    for (int32_t i = 0; i < n; ++i) b->plan[i] = autotune(ctx, b->ex.spec[i], b->ex.rows[i], b->ex.est[i], nullptr);
+    b->ex.okeys.resize(n);
+    b->ex.ocols.resize(n);
+    b->ex.cap.assign(n, 0);
+  }
+  if (st) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_batch_run(tga_batch *b) {
+  if (!b) return TGA_ERR_INVALID_ARG;
+  tga_context *ctx = b->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  BatchExec &ex = b->ex;
+  if (b->mode == TGA_BATCH_PER_QUERY) {
+    for (int32_t i = 0; i < ex.nq; ++i) {
+      PlainOut po;
+      uint64_t G = 0;
+      tga_status st = run_plain(ctx, ex.spec[i], ex.keys[i], ex.rows[i], b->plan[i], &po, nullptr, nullptr, 0, &G);
+      if (st) return st;
+      ex.okeys[i] = std::move(po.keys);
+      for (int a = 0; a < ex.spec[i].schema.nacc; ++a) ex.ocols[i][a] = std::move(po.cols[a]);
+      ex.cap[i] = po.capacity;
+      b->groups[i] = G;
+    }
+    return TGA_OK;
+  }
+  TGA_HIP(ctx, timed(ctx, "batch_persistent", [&] { return hipGraphLaunch(b->exec, ctx->stream); }));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->prof.on) ctx->prof.resolve();
+  return batch_finalize(ex, b->groups);
+}
+
+TGA_API tga_status tga_batch_num_groups(const tga_batch *b, int32_t q, int64_t *n) {
+  if (!b || !n || q < 0 || q >= b->ex.nq) return TGA_ERR_INVALID_ARG;
+  *n = (int64_t)b->groups[q];
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_batch_copy_to_host(const tga_batch *b, int32_t q, int32_t index, void *dst, int64_t cap) {
+  if (!b || !dst || q < 0 || q >= b->ex.nq || cap < 0) return TGA_ERR_INVALID_ARG;
+  const QuerySpec &s = b->ex.spec[q];
+  if (index < -1 || index >= s.schema.nacc) return TGA_ERR_INVALID_ARG;
+  tga_context *ctx = b->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  const uint64_t rows = std::min<uint64_t>((uint64_t)cap, b->groups[q]);
+  if (!rows) return TGA_OK;
+  const int dt = index < 0 ? s.key_dtype : s.out_dtype[index];
+  const void *src = index < 0 ? b->ex.okeys[q].p : b->ex.ocols[q][index].p;
+  TGA_HIP(ctx, hipMemcpyAsync(dst, src, rows * dtype_bytes(dt), hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_batch_destroy(tga_batch *b) {
+  if (!b) return TGA_ERR_INVALID_ARG;
+  tga_context *ctx = b->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  delete b;
+  return TGA_OK;
+}

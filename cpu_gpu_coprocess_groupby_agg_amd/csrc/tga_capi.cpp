@@ -1,0 +1,837 @@
+// tga_capi.cpp -- C ABI of the plain operator: context, arena, autotuner,
+// execution of the direct and partitioned paths, results and utilities.
+//
+// Reference host orchestration replaced (T/ = tiled_groupby_agg_partition/):
+//   groupby_agg_partition / _thread   T/src/group_then_partition.cu:308-528
+//   groupby_agg_intra_partition       T/src/groupby_agg_intra_partition.cu:159-411
+//   pre_device_alloc, pre_alloc_device_memory  T/src/util.cu:4-40, intra:51-86
+// Columns stay resident in HBM; one host thread drives one HIP stream with no
+// host round trip between kernels (the reference blocks six times per tile,
+// group_then_partition.cu:97,173,221,386,409,418).
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+
+#include "tga_host.hpp"
+
+namespace tga {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+tga_status fail(tga_context *ctx, tga_status st, const char *fmt, ...) {
+  if (ctx) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// arena
+// ---------------------------------------------------------------------------
+uint64_t Arena::round(uint64_t bytes) {
+  if (bytes <= 4096) return 4096;
+  if (bytes <= (1ULL << 21)) {  // powers of two up to 2 MiB
+    uint64_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  // above: multiples of 2 MiB, rounded up by at most 1/8
+  uint64_t step = 1ULL << 21;
+  while (step * 8 < bytes) step <<= 1;
+  return (bytes + step - 1) / step * step;
+}
+
+void *Arena::alloc(uint64_t bytes) {
+  const uint64_t c = round(bytes);
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = free_.find(c);
+  if (it != free_.end() && !it->second.empty()) {
+    void *p = it->second.back();
+    it->second.pop_back();
+    cached_ -= c;
+    in_use_ += c;
+    live_[p] = c;
+    peak_ = std::max(peak_, in_use_);
+    return p;
+  }
+  if (in_use_ + c > budget_) return nullptr;
+  if (in_use_ + cached_ + c > budget_) {
+    // drop cached blocks until the new one fits
+    for (auto &kv : free_) {
+      for (void *q : kv.second) { hipFree(q); cached_ -= kv.first; }
+      kv.second.clear();
+      if (in_use_ + cached_ + c <= budget_) break;
+    }
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, c) != hipSuccess) {
+    (void)hipGetLastError();
+    for (auto &kv : free_) { for (void *q : kv.second) hipFree(q); kv.second.clear(); }
+    cached_ = 0;
+    if (hipMalloc(&p, c) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  }
+  in_use_ += c;
+  live_[p] = c;
+  peak_ = std::max(peak_, in_use_);
+  return p;
+}
+
+void Arena::free(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  const uint64_t c = it->second;
+  live_.erase(it);
+  in_use_ -= c;
+  cached_ += c;
+  free_[c].push_back(p);
+}
+
+void Arena::release_cached() {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto &kv : free_) { for (void *q : kv.second) hipFree(q); kv.second.clear(); }
+  cached_ = 0;
+}
+
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+hipEvent_t Profiler::get() {
+  if (!pool_.empty()) { hipEvent_t e = pool_.back(); pool_.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+void Profiler::begin(const char *name, hipStream_t st) {
+  Pending p{name, get(), get()};
+  hipEventRecord(p.a, st);
+  pending_.push_back(p);
+}
+void Profiler::end(hipStream_t st) {
+  if (!pending_.empty()) hipEventRecord(pending_.back().b, st);
+}
+void Profiler::resolve() {
+  for (auto &p : pending_) {
+    float ms = 0;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto it = acc_.find(p.name);
+      if (it == acc_.end()) { order_.push_back(p.name); acc_[p.name] = {1, ms}; }
+      else { it->second.first += 1; it->second.second += ms; }
+    }
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+void Profiler::reset() { resolve(); acc_.clear(); order_.clear(); }
+void Profiler::destroy() {
+  resolve();
+  for (hipEvent_t e : pool_) hipEventDestroy(e);
+  pool_.clear();
+}
+std::vector<std::pair<std::string, std::pair<int64_t, double>>> Profiler::entries() const {
+  std::vector<std::pair<std::string, std::pair<int64_t, double>>> v;
+  for (auto &n : order_) v.push_back({n, acc_.at(n)});
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// schema
+// ---------------------------------------------------------------------------
+static bool valid_dtype(int d) { return d >= DT_U32 && d <= DT_F64; }
+
+std::string build_spec(const tga_column *key, const tga_column *values, int32_t n_values, const tga_agg *aggs,
+                       int32_t n_aggs, QuerySpec &q) {
+  if (!key) return "key column is null";
+  if (!(key->dtype == DT_U32 || key->dtype == DT_I32 || key->dtype == DT_U64 || key->dtype == DT_I64))
+    return "key dtype must be u32, i32, u64 or i64";
+  if (n_values < 0 || n_values > kMaxCols) return "at most 4 value columns";
+  if (n_values > 0 && !values) return "value columns are null";
+  if (n_aggs < 1 || n_aggs > kMaxAccs) return "between 1 and 8 aggregates";
+  if (!aggs) return "aggregate list is null";
+  q = QuerySpec();
+  q.key_dtype = key->dtype;
+  q.key_bytes = dtype_bytes(key->dtype);
+  Schema &s = q.schema;
+  s.ncols = n_values;
+  for (int c = 0; c < n_values; ++c) {
+    if (!valid_dtype(values[c].dtype)) return "bad value dtype";
+    s.cols[c] = values[c].data;
+    s.col_dtype[c] = values[c].dtype;
+  }
+  s.nacc = n_aggs;
+  int nfx = 0;
+  for (int a = 0; a < n_aggs; ++a) {
+    const int op = aggs[a].op;
+    int kind = -1;
+    int col = aggs[a].column;
+    if (op == OP_COUNT) { kind = ACC_COUNT; col = 0; q.out_dtype[a] = DT_I64; }
+    else {
+      if (col < 0 || col >= n_values) return "aggregate column out of range";
+      const int dt = s.col_dtype[col];
+      if (op == OP_SUM) {
+        if (dt == DT_F64) {
+          if (nfx >= kMaxFx) return "at most 2 fp64 SUM aggregates per query";
+          kind = ACC_SUM_F;
+          s.fx_slot[a] = nfx++;
+          q.out_dtype[a] = DT_F64;
+        } else { kind = ACC_SUM_I; q.out_dtype[a] = DT_I64; }
+      } else if (op == OP_MIN || op == OP_MAX) {
+        if (dt == DT_I32) kind = op == OP_MIN ? ACC_MIN_I32 : ACC_MAX_I32;
+        else kind = op == OP_MIN ? ACC_MIN_I64 : ACC_MAX_I64;
+        q.out_dtype[a] = dt;
+      } else {
+        return "unknown aggregate op";
+      }
+    }
+    s.acc[a].kind = kind;
+    s.acc[a].col = col;
+  }
+  q.nfx = nfx;
+  return "";
+}
+
+uint32_t lds_slot_bytes(const Schema &s, int key_bytes) {
+  uint32_t b = (uint32_t)key_bytes;
+  for (int a = 0; a < s.nacc; ++a) b += (uint32_t)acc_lds_bytes(s.acc[a].kind);
+  return b;
+}
+
+Layout make_layout(const Schema &s, int key_bytes, uint32_t slots) {
+  Layout L{};
+  L.slots = slots;
+  const uint64_t n = (uint64_t)slots + 1;
+  uint64_t off = 0;
+  auto place = [&](uint64_t bytes) {
+    const uint64_t o = off;
+    off += (bytes + 15) & ~15ULL;
+    return (uint32_t)o;
+  };
+  // 8-byte arrays first
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 0 && key_bytes == 8) L.key_off = place(n * 8);
+    if (pass == 1 && key_bytes == 4) L.key_off = place(n * 4);
+    for (int a = 0; a < s.nacc; ++a) {
+      const int b = acc_lds_bytes(s.acc[a].kind);
+      if ((pass == 0) == (b >= 8)) L.acc_off[a] = place(n * (uint64_t)b);
+    }
+  }
+  L.bytes = (uint32_t)off;
+  return L;
+}
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+static int ilog2(uint64_t x) {
+  int b = 0;
+  while ((1ULL << b) < x) ++b;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+//
+// Picks the path, the LDS table size, the partition fanout, tile and grid
+// from (rows, estimated groups) and the device; every explicit option wins.
+// ---------------------------------------------------------------------------
+Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est, const tga_options *opt) {
+  tga_options o{};
+  if (opt) o = *opt;
+  Plan p;
+  const uint32_t sb = lds_slot_bytes(q.schema, q.key_bytes);
+  const size_t lds_max = ctx->lds_max - 1024;
+  if (est < 1) est = 1;
+  if (est > n && n > 0) est = n;
+  // largest power-of-two table fitting one workgroup's LDS
+  uint32_t tmax = 64;
+  while ((uint64_t)(tmax * 2 + 1) * sb + 256 <= lds_max) tmax *= 2;
+
+  int path = o.path;
+  if (path == TGA_PATH_AUTO) path = (est * 2 <= tmax) ? TGA_PATH_DIRECT : TGA_PATH_PARTITIONED;
+  p.path = path;
+
+  if (path == TGA_PATH_DIRECT) {
+    uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : (uint32_t)std::min<uint64_t>(tmax, std::max<uint64_t>(256, next_pow2(est * 2)));
+    p.table_slots = T;
+    p.block = o.block_threads > 0 ? (uint32_t)o.block_threads : 256;
+    const uint64_t lds = make_layout(q.schema, q.key_bytes, T).bytes;
+    uint32_t per_cu = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (ctx->lds_max) / std::max<uint64_t>(lds, 1)));
+    uint32_t W = o.workgroups > 0 ? (uint32_t)o.workgroups : (uint32_t)ctx->cus * per_cu;
+    // at least ~16K rows per workgroup so each flush is amortised
+    const uint64_t min_rows = 16384;
+    W = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(W, (n + min_rows - 1) / min_rows));
+    p.workgroups = W;
+    p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(1, (n + W - 1) / W);
+  } else {
+    uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : tmax;
+    p.table_slots = T;
+    int pbits = o.partition_bits;
+    if (pbits <= 0) {
+      // groups per partition at most half a table
+      const uint64_t P = next_pow2((est * 2 + T - 1) / T);
+      pbits = std::max(1, ilog2(P));
+    }
+    pbits = std::min(pbits, 12);
+    p.pbits = pbits;
+    uint32_t tile = o.tile_rows > 0 ? (uint32_t)o.tile_rows : 8192;
+    // the scatter tile must fit the LDS with the 2^pbits histogram
+    while (tile > 1024 && scatter_lds_bytes(q.schema, q.key_bytes, pbits, tile) > lds_max) tile /= 2;
+    p.tile_rows = tile;
+    p.block = o.block_threads > 0 ? (uint32_t)o.block_threads : 1024;
+    const uint64_t lds = make_layout(q.schema, q.key_bytes, T).bytes;
+    const uint32_t per_cu = (uint32_t)std::max<uint64_t>(1, ctx->lds_max / std::max<uint64_t>(lds, 1));
+    p.workgroups = o.workgroups > 0 ? (uint32_t)o.workgroups : (uint32_t)ctx->cus * std::min<uint32_t>(per_cu, 2);
+    const uint64_t P = 1ULL << pbits;
+    p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(65536, 2 * ((n + P - 1) / P));
+  }
+  p.layout = make_layout(q.schema, q.key_bytes, p.table_slots);
+  p.nbits = ilog2((uint64_t)p.table_slots + 1);
+  p.max_probe = (int)std::min<uint32_t>(p.table_slots, 256);
+  // global table at load <= 1/2 of the estimate, at most 2 n slots
+  uint64_t gs = next_pow2(std::max<uint64_t>(1024, est * 2));
+  gs = std::min<uint64_t>(gs, next_pow2(std::max<uint64_t>(1024, 2 * n)));
+  p.gslots = gs;
+  return p;
+}
+
+static void plan_to_options(const Plan &p, tga_options &o) {
+  o = tga_options{};
+  o.path = p.path;
+  o.partition_bits = p.pbits;
+  o.table_slots = (int32_t)p.table_slots;
+  o.tile_rows = (int32_t)p.tile_rows;
+  o.workgroups = (int32_t)p.workgroups;
+  o.block_threads = (int32_t)p.block;
+  o.chunk_rows = (int64_t)p.chunk_rows;
+  o.est_groups = (int64_t)(p.gslots / 2);
+}
+
+// ---------------------------------------------------------------------------
+// cardinality estimate
+// ---------------------------------------------------------------------------
+tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est) {
+  if (n == 0) { *est = 1; return TGA_OK; }
+  const uint64_t m = std::min<uint64_t>(n, 16384);
+  DBuf buf;
+  if (!buf.alloc(ctx, m * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
+  TGA_HIP(ctx, launch_sample_keys(key_bytes, d_keys, n, m, buf.as<uint64_t>(), ctx->stream));
+  std::vector<uint64_t> h(m);
+  TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::sort(h.begin(), h.end());
+  uint64_t f1 = 0, rest = 0;
+  for (uint64_t i = 0; i < m;) {
+    uint64_t j = i;
+    while (j < m && h[j] == h[i]) ++j;
+    if (j - i == 1) ++f1; else ++rest;
+    i = j;
+  }
+  const double d = std::sqrt((double)n / (double)m) * (double)f1 + (double)rest;
+  *est = (uint64_t)std::min<double>((double)n, std::max<double>(1.0, d));
+  return TGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plain operator execution
+// ---------------------------------------------------------------------------
+hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable &g, std::vector<DBuf> &bufs,
+                               DBuf &flags) {
+  bufs.clear();
+  bufs.resize(q.schema.nacc + 1);
+  const uint64_t n = S + 1;
+  if (!bufs[0].alloc(ctx, n * q.key_bytes)) return hipErrorOutOfMemory;
+  g.keys = bufs[0].p;
+  for (int a = 0; a < q.schema.nacc; ++a) {
+    if (!bufs[a + 1].alloc(ctx, n * (uint64_t)acc_global_bytes(q.schema.acc[a].kind))) return hipErrorOutOfMemory;
+    g.acc[a] = bufs[a + 1].p;
+  }
+  if (!flags.alloc(ctx, 64)) return hipErrorOutOfMemory;
+  g.flags = flags.as<uint32_t>();
+  g.S = S;
+  return hipSuccess;
+}
+
+static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
+
+// Aggregates rows [0, n) of device-resident columns into the global table g
+// (direct or partitioned path).  Temporaries are appended to `keep` and must
+// outlive the work queued on the context stream.
+tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, const Plan &plan,
+                          const GTable &g, std::vector<DBuf> &keep) {
+  const Schema &s = q.schema;
+  hipStream_t st = ctx->stream;
+  DBuf planbuf;  // pstart[P+1] u64 | cstart[P+1] u32 | ticket
+  if (n > 0 && plan.path == TGA_PATH_DIRECT) {
+    if (!planbuf.alloc(ctx, 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
+    AggPlan ap{planbuf.as<uint64_t>(), (uint32_t *)(planbuf.as<char>() + 16), 1, (uint32_t *)(planbuf.as<char>() + 32)};
+    const uint64_t nch = (n + plan.chunk_rows - 1) / plan.chunk_rows;
+    TGA_HIP(ctx, launch_plan_direct(ap, n, (uint32_t)nch, st));
+    const unsigned grid = (unsigned)std::min<uint64_t>(plan.workgroups, nch);
+    TGA_HIP(ctx, timed(ctx, "agg_direct", [&] {
+      return launch_agg(q.key_bytes, d_keys, s, ap, plan.layout, g, plan.nbits, plan.max_probe, grid, plan.block, st);
+    }));
+  } else if (n > 0) {
+    // partitioned path, in super-chunks of < 2^31 rows (u32 offsets)
+    const uint64_t kSuper = 1ULL << 31;
+    const uint32_t P = 1u << plan.pbits;
+    for (uint64_t r0 = 0; r0 < n; r0 += kSuper) {
+      const uint64_t m = std::min<uint64_t>(kSuper, n - r0);
+      const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
+      const uint64_t cells = (uint64_t)P * ntiles;
+      DBuf cnt, offs, desc, pkeys;
+      DBuf pcols[kMaxCols];
+      if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
+          !pkeys.alloc(ctx, m * q.key_bytes))
+        return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+      Schema sc = s;
+      PartOut po{};
+      const char *kbase = (const char *)d_keys + r0 * q.key_bytes;
+      for (int c = 0; c < s.ncols; ++c) {
+        const uint64_t cb = (uint64_t)dtype_bytes(s.col_dtype[c]);
+        if (!pcols[c].alloc(ctx, m * cb)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+        po.cols[c] = pcols[c].p;
+        sc.cols[c] = (const char *)s.cols[c] + r0 * cb;
+      }
+      const uint64_t dwords = scan_tiles(cells) + 1;
+      uint32_t *ticket = (uint32_t *)(desc.as<uint64_t>() + dwords);
+      TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
+      TGA_HIP(ctx, timed(ctx, "part_count", [&] {
+        return launch_part_count(q.key_bytes, kbase, m, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
+      }));
+      TGA_HIP(ctx, timed(ctx, "scan", [&] {
+        return launch_scan_u32(cnt.as<uint32_t>(), offs.as<uint32_t>(), cells, desc.as<uint64_t>(), ticket, st);
+      }));
+      TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+        return launch_part_scatter(q.key_bytes, kbase, sc, m, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
+                                   pkeys.p, po, st);
+      }));
+      if (!planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
+      AggPlan ap{planbuf.as<uint64_t>(), (uint32_t *)(planbuf.as<uint64_t>() + (P + 1)), P,
+                 (uint32_t *)(planbuf.as<uint64_t>() + (P + 1)) + (P + 1)};
+      TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
+        return launch_plan_chunks(offs.as<uint32_t>(), P, ntiles, m, plan.chunk_rows, ap, st);
+      }));
+      Schema sp = s;
+      for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
+      TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
+        return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
+                          plan.block, st);
+      }));
+      keep.push_back(std::move(planbuf));
+      keep.push_back(std::move(cnt));
+      keep.push_back(std::move(offs));
+      keep.push_back(std::move(desc));
+      keep.push_back(std::move(pkeys));
+      for (int c = 0; c < s.ncols; ++c) keep.push_back(std::move(pcols[c]));
+      if (r0 + kSuper < n) {  // reuse the buffers of this super-chunk
+        TGA_HIP(ctx, hipStreamSynchronize(st));
+        keep.clear();
+      }
+    }
+  }
+
+  if (planbuf.p) keep.push_back(std::move(planbuf));
+  return TGA_OK;
+}
+
+// Compacts the occupied slots of g into od (slot order), then waits for the
+// stream and returns the status words and the group count.
+tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, const OutDesc &od, uint32_t *hflags,
+                        uint64_t *hcount) {
+  hipStream_t st = ctx->stream;
+  const uint64_t ctiles = compact_tiles(g.S + 1);
+  DBuf cdesc;
+  if (!cdesc.alloc(ctx, (ctiles + 2) * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "compaction");
+  uint32_t *cticket = (uint32_t *)(cdesc.as<uint64_t>() + ctiles);
+  uint64_t *count = cdesc.as<uint64_t>() + ctiles + 1;
+  TGA_HIP(ctx, launch_fill_u64(cdesc.as<uint64_t>(), ctiles + 2, 0, st));
+  TGA_HIP(ctx, timed(ctx, "compact", [&] {
+    return launch_compact(q.key_bytes, g, q.schema, od, cdesc.as<uint64_t>(), cticket, count, st);
+  }));
+  TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof(uint32_t) * kFlagWords, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipMemcpyAsync(hcount, count, 8, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipStreamSynchronize(st));
+  return TGA_OK;
+}
+
+tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, const Plan &plan_in,
+                     PlainOut *out, void *ext_keys, void *const *ext_cols, uint64_t ext_cap, uint64_t *groups) {
+  Plan plan = plan_in;
+  hipStream_t st = ctx->stream;
+  const Schema &s = q.schema;
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    GTable g{};
+    std::vector<DBuf> gbufs;
+    DBuf flags;
+    if (gtable_alloc(ctx, q, plan.gslots, g, gbufs, flags) != hipSuccess)
+      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table of %llu slots", (unsigned long long)plan.gslots);
+    TGA_HIP(ctx, timed(ctx, "gtable_init", [&] { return launch_gtable_init(q.key_bytes, g, s, st); }));
+
+    std::vector<DBuf> keep;  // temporaries alive until the stream is synchronised
+    {
+      tga_status a = aggregate_into(ctx, q, d_keys, n, plan, g, keep);
+      if (a) return a;
+    }
+
+    // (d) compaction
+    const uint64_t cap = ext_keys ? ext_cap : plan.gslots + 1;
+    PlainOut tmp;
+    OutDesc od{};
+    od.capacity = cap;
+    if (ext_keys) {
+      od.keys = ext_keys;
+      for (int a = 0; a < s.nacc; ++a) od.cols[a] = ext_cols[a];
+    } else {
+      if (!tmp.keys.alloc(ctx, std::max<uint64_t>(cap, 1) * q.key_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "output");
+      od.keys = tmp.keys.p;
+      for (int a = 0; a < s.nacc; ++a) {
+        if (!tmp.cols[a].alloc(ctx, std::max<uint64_t>(cap, 1) * out_elem_bytes(q.out_dtype[a])))
+          return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "output");
+        od.cols[a] = tmp.cols[a].p;
+      }
+    }
+    uint32_t hflags[kFlagWords];
+    uint64_t hcount = 0;
+    {
+      tga_status c = compact_into(ctx, q, g, od, hflags, &hcount);
+      if (c) return c;
+    }
+    if (ctx->prof.on) ctx->prof.resolve();
+    if (hflags[FLAG_OVERFLOW]) {
+      // the estimate was too low: grow the global table and run again
+      plan.gslots *= 4;
+      continue;
+    }
+    if (hflags[FLAG_RANGE]) return fail(ctx, TGA_ERR_RANGE, "fp64 SUM input outside [2^-96, 2^64) or not finite");
+    *groups = hcount;
+    if (out) {
+      out->keys = std::move(tmp.keys);
+      for (int a = 0; a < s.nacc; ++a) out->cols[a] = std::move(tmp.cols[a]);
+      out->groups = hcount;
+      out->capacity = cap;
+    }
+    ctx->last_plan = tga_options{};
+    plan_to_options(plan, ctx->last_plan);
+    if (ext_keys && hcount > ext_cap) return fail(ctx, TGA_ERR_CAPACITY, "%llu groups > capacity %llu",
+                                                 (unsigned long long)hcount, (unsigned long long)ext_cap);
+    return TGA_OK;
+  }
+  return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table kept overflowing");
+}
+
+}  // namespace tga
+
+using namespace tga;
+
+// ---------------------------------------------------------------------------
+// staging of host columns
+// ---------------------------------------------------------------------------
+namespace {
+struct Staged {
+  tga_column key{};
+  tga_column vals[kMaxCols]{};
+  DBuf bufs[kMaxCols + 1];
+};
+
+tga_status stage(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t nv, int64_t n, Staged &o) {
+  o.key = *key;
+  for (int c = 0; c < nv; ++c) o.vals[c] = values[c];
+  auto one = [&](tga_column &col, DBuf &b) -> tga_status {
+    if (col.location != TGA_HOST || n == 0) return TGA_OK;
+    const uint64_t bytes = (uint64_t)n * dtype_bytes(col.dtype);
+    if (!b.alloc(ctx, bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "staging");
+    TGA_HIP(ctx, hipMemcpyAsync(b.p, col.data, bytes, hipMemcpyHostToDevice, ctx->stream));
+    col.data = b.p;
+    col.location = TGA_DEVICE;
+    return TGA_OK;
+  };
+  tga_status st = one(o.key, o.bufs[0]);
+  if (st) return st;
+  for (int c = 0; c < nv; ++c)
+    if ((st = one(o.vals[c], o.bufs[c + 1]))) return st;
+  return TGA_OK;
+}
+
+tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values, int64_t n_rows,
+                   const tga_agg *aggs, int32_t n_aggs, const tga_options *opt, Staged &sg, QuerySpec &q, Plan &plan) {
+  if (n_rows < 0) return fail(ctx, TGA_ERR_INVALID_ARG, "negative row count");
+  std::string e = build_spec(key, values, n_values, aggs, n_aggs, q);
+  if (!e.empty()) return fail(ctx, TGA_ERR_INVALID_ARG, "%s", e.c_str());
+  if (n_rows > 0 && !key->data) return fail(ctx, TGA_ERR_INVALID_ARG, "key data is null");
+  for (int c = 0; c < n_values; ++c)
+    if (n_rows > 0 && !values[c].data) return fail(ctx, TGA_ERR_INVALID_ARG, "value column %d is null", c);
+  tga_status st = stage(ctx, key, values, n_values, n_rows, sg);
+  if (st) return st;
+  for (int c = 0; c < n_values; ++c) q.schema.cols[c] = sg.vals[c].data;
+  uint64_t est = opt && opt->est_groups > 0 ? (uint64_t)opt->est_groups : 0;
+  if (!est) {
+    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est);
+    if (st) return st;
+  }
+  plan = autotune(ctx, q, (uint64_t)n_rows, est, opt);
+  return TGA_OK;
+}
+}  // namespace
+
+#define TGA_API extern "C" __attribute__((visibility("default")))
+
+TGA_API tga_status tga_options_default(tga_options *opt) {
+  if (!opt) return TGA_ERR_INVALID_ARG;
+  *opt = tga_options{};
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_context_create(int32_t device, uint64_t mem_budget_bytes, tga_context **out) {
+  if (!out) return TGA_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TGA_ERR_HIP;
+  if (device < 0 || device >= n) return TGA_ERR_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return TGA_ERR_HIP;
+  auto *c = new tga_context();
+  c->device = device;
+  if (hipGetDeviceProperties(&c->props, device) != hipSuccess) { delete c; return TGA_ERR_HIP; }
+  c->cus = c->props.multiProcessorCount;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_max = (size_t)lds;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TGA_ERR_HIP; }
+  uint64_t budget = mem_budget_bytes;
+  if (!budget) {
+    size_t fr = 0, tot = 0;
+    hipMemGetInfo(&fr, &tot);
+    budget = (uint64_t)(fr * 0.9);
+  }
+  c->arena = new Arena(budget);
+  *out = c;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_context_destroy(tga_context *ctx) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    ctx->prof.destroy();
+    delete ctx->arena;
+    ctx->arena = nullptr;
+    hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return TGA_OK;
+}
+
+TGA_API const char *tga_last_error(const tga_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+TGA_API tga_status tga_device_info(tga_context *ctx, char *name, size_t name_len, int32_t *cus, int64_t *hbm) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  if (name && name_len) { std::snprintf(name, name_len, "%s", ctx->props.gcnArchName); }
+  if (cus) *cus = ctx->cus;
+  if (hbm) *hbm = (int64_t)ctx->props.totalGlobalMem;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_groupby(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values,
+                               int64_t n_rows, const tga_agg *aggs, int32_t n_aggs, const tga_options *opt,
+                               tga_result **out) {
+  if (!ctx || !out) return TGA_ERR_INVALID_ARG;
+  *out = nullptr;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  Staged sg;
+  QuerySpec q;
+  Plan plan;
+  tga_status st = prepare(ctx, key, values, n_values, n_rows, aggs, n_aggs, opt, sg, q, plan);
+  if (st) return st;
+  auto *r = new tga_result();
+  r->ctx = ctx;
+  r->key_dtype = q.key_dtype;
+  r->naggs = n_aggs;
+  for (int a = 0; a < n_aggs; ++a) r->dtype[a] = q.out_dtype[a];
+  uint64_t G = 0;
+  st = run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, &r->out, nullptr, nullptr, 0, &G);
+  if (st) { delete r; return st; }
+  *out = r;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_groupby_into(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values,
+                                    int64_t n_rows, const tga_agg *aggs, int32_t n_aggs, const tga_options *opt,
+                                    void *out_keys, void *const *out_aggs, int64_t capacity, int64_t *n_groups) {
+  if (!ctx || !out_keys || !out_aggs || capacity < 0 || !n_groups) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  for (int a = 0; a < n_aggs && a < kMaxAccs; ++a)
+    if (!out_aggs[a]) return fail(ctx, TGA_ERR_INVALID_ARG, "output column %d is null", a);
+  Staged sg;
+  QuerySpec q;
+  Plan plan;
+  tga_status st = prepare(ctx, key, values, n_values, n_rows, aggs, n_aggs, opt, sg, q, plan);
+  if (st) return st;
+  uint64_t G = 0;
+  st = run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, nullptr, out_keys, out_aggs, (uint64_t)capacity, &G);
+  *n_groups = (int64_t)G;
+  return st;
+}
+
+TGA_API tga_status tga_last_plan(tga_context *ctx, tga_options *plan) {
+  if (!ctx || !plan) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  *plan = ctx->last_plan;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_result_num_groups(const tga_result *r, int64_t *n) {
+  if (!r || !n) return TGA_ERR_INVALID_ARG;
+  *n = (int64_t)r->out.groups;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_result_column(const tga_result *r, int32_t index, const void **ptr, int32_t *dtype) {
+  if (!r || index < -1 || index >= r->naggs) return TGA_ERR_INVALID_ARG;
+  if (index < 0) {
+    if (ptr) *ptr = r->out.keys.p;
+    if (dtype) *dtype = r->key_dtype;
+  } else {
+    if (ptr) *ptr = r->out.cols[index].p;
+    if (dtype) *dtype = r->dtype[index];
+  }
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_result_copy_to_host(const tga_result *r, int32_t index, void *dst, int64_t cap) {
+  if (!r || !dst || index < -1 || index >= r->naggs || cap < 0) return TGA_ERR_INVALID_ARG;
+  const int dt = index < 0 ? r->key_dtype : r->dtype[index];
+  const void *src = index < 0 ? r->out.keys.p : r->out.cols[index].p;
+  const uint64_t rows = std::min<uint64_t>((uint64_t)cap, r->out.groups);
+  if (rows == 0) return TGA_OK;
+  std::lock_guard<std::recursive_mutex> g(r->ctx->mu);
+  hipSetDevice(r->ctx->device);
+  TGA_HIP(r->ctx, hipMemcpyAsync(dst, src, rows * dtype_bytes(dt), hipMemcpyDeviceToHost, r->ctx->stream));
+  TGA_HIP(r->ctx, hipStreamSynchronize(r->ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_result_free(tga_result *r) {
+  if (!r) return TGA_ERR_INVALID_ARG;
+  tga_context *ctx = r->ctx;
+  {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    delete r;
+  }
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_scan_exclusive_u32(tga_context *ctx, const uint32_t *d_in, uint32_t *d_out, int64_t n) {
+  if (!ctx || n < 0 || (n > 0 && (!d_in || !d_out))) return TGA_ERR_INVALID_ARG;
+  if (n == 0) return TGA_OK;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  const uint64_t tiles = scan_tiles((uint64_t)n);
+  DBuf desc;
+  if (!desc.alloc(ctx, (tiles + 1) * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "scan descriptors");
+  TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), tiles + 1, 0, ctx->stream));
+  TGA_HIP(ctx, timed(ctx, "scan", [&] {
+    return launch_scan_u32(d_in, d_out, (uint64_t)n, desc.as<uint64_t>(), (uint32_t *)(desc.as<uint64_t>() + tiles),
+                           ctx->stream);
+  }));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->prof.on) ctx->prof.resolve();
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int64_t n, uint64_t a, uint64_t b,
+                                double x, double y, uint64_t seed, uint64_t salt) {
+  if (!ctx || n < 0 || (n > 0 && !d_out)) return TGA_ERR_INVALID_ARG;
+  if ((kind == TGA_GEN_KEYS_U32 || kind == TGA_GEN_KEYS_U64 || kind == TGA_GEN_ZIPF_U64) && a == 0)
+    return fail(ctx, TGA_ERR_INVALID_ARG, "distinct count must be positive");
+  if (kind == TGA_GEN_VALS_I32 && b == 0) return fail(ctx, TGA_ERR_INVALID_ARG, "empty value range");
+  if (n == 0) return TGA_OK;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  TGA_HIP(ctx, launch_gen(kind, d_out, (uint64_t)n, a, b, x, y, seed, salt, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_device_alloc(tga_context *ctx, uint64_t bytes, void **out) {
+  if (!ctx || !out) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  void *p = nullptr;
+  TGA_HIP(ctx, hipMalloc(&p, bytes ? bytes : 1));
+  *out = p;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_device_free(tga_context *ctx, void *p) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  TGA_HIP(ctx, hipFree(p));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_memcpy_h2d(tga_context *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  TGA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_memcpy_d2h(tga_context *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  TGA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_synchronize(tga_context *ctx) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_profile_enable(tga_context *ctx, int32_t on) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  ctx->prof.on = on != 0;
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_profile_reset(tga_context *ctx) {
+  if (!ctx) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  ctx->prof.reset();
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_profile_entry(tga_context *ctx, int32_t i, char *name, size_t name_len, int64_t *launches,
+                                     double *total_ms) {
+  if (!ctx || i < 0) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  ctx->prof.resolve();
+  auto v = ctx->prof.entries();
+  if ((size_t)i >= v.size()) return TGA_ERR_INVALID_ARG;
+  if (name && name_len) std::snprintf(name, name_len, "%s", v[(size_t)i].first.c_str());
+  if (launches) *launches = v[(size_t)i].second.first;
+  if (total_ms) *total_ms = v[(size_t)i].second.second;
+  return TGA_OK;
+}

@@ -1,0 +1,214 @@
+// tga_common.hpp -- definitions shared by host and device code of the
+// MI355X (gfx950) tiled hash group-by.
+//
+// Keys are 32- or 64-bit bit patterns compared for equality (the reference
+// keys are u32, T/include/util.cuh:6).  Values come from up to kMaxCols
+// columns; each aggregate becomes one accumulator ("acc") whose state lives in
+// a structure-of-arrays hash table, first in LDS per workgroup, then in a
+// global table in HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TGA_HD __host__ __device__ __forceinline__
+
+namespace tga {
+
+constexpr int kMaxCols = 4;
+constexpr int kMaxAccs = 8;
+constexpr int kMaxFx = 2;  // exact fp64 SUM accumulators per query
+
+// Exact fp64 SUM: a double is split into 32-bit chunks of the fixed-point
+// integer round(x * 2^kFxShift) and each chunk is added into an int64 limb.
+// Limb i carries weight 2^(32 i - kFxShift).  Five limbs cover [2^-96, 2^64);
+// bits below 2^-96 are truncated, |x| >= 2^64, Inf and NaN raise a range flag.
+// Each add is < 2^32 in magnitude, so a limb is exact for < 2^31 rows.
+constexpr int kLimbs = 5;
+constexpr int kFxShift = 96;
+
+enum Dtype : int32_t { DT_U32 = 1, DT_I32 = 2, DT_U64 = 3, DT_I64 = 4, DT_F64 = 5 };
+enum AggOp : int32_t { OP_COUNT = 0, OP_SUM = 1, OP_MIN = 2, OP_MAX = 3 };
+enum AccKind : int32_t {
+  ACC_COUNT = 0,    // LDS u32, global u64
+  ACC_SUM_I = 1,    // int64 (wraps mod 2^64)
+  ACC_MIN_I32 = 2,  // int32 column only
+  ACC_MAX_I32 = 3,
+  ACC_MIN_I64 = 4,  // every other column through an order-preserving map to int64
+  ACC_MAX_I64 = 5,
+  ACC_SUM_F = 6,    // exact fixed-point limbs
+};
+
+// global-table status words
+enum : int { FLAG_SENTINEL = 0, FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_GROUPS = 3, kFlagWords = 4 };
+
+struct Acc {
+  int32_t kind;
+  int32_t col;
+};
+
+struct Schema {
+  const void *cols[kMaxCols];
+  int32_t col_dtype[kMaxCols];
+  int32_t ncols;
+  int32_t nacc;
+  Acc acc[kMaxAccs];
+  int32_t fx_slot[kMaxAccs];  // ACC_SUM_F accumulators: index among them
+};
+
+// Global table in HBM: the LDS table's SoA shape with S regular slots.
+struct GTable {
+  void *keys;               // [S+1]
+  void *acc[kMaxAccs];      // [S+1] each (ACC_SUM_F: [kLimbs][S+1])
+  uint64_t S;
+  uint32_t *flags;          // kFlagWords status words
+};
+
+// Byte offsets of the SoA arrays of one table with `slots` entries plus one
+// extra entry (index `slots`) reserved for the key whose bits equal the empty
+// marker, so every key value is supported.
+struct Layout {
+  uint32_t slots;
+  uint32_t key_off;
+  uint32_t acc_off[kMaxAccs];
+  uint32_t bytes;
+};
+
+TGA_HD int dtype_bytes(int dt) { return (dt == DT_U32 || dt == DT_I32) ? 4 : 8; }
+
+TGA_HD int acc_lds_bytes(int kind) {
+  switch (kind) {
+    case ACC_COUNT: case ACC_MIN_I32: case ACC_MAX_I32: return 4;
+    case ACC_SUM_F: return 8 * kLimbs;
+    default: return 8;
+  }
+}
+TGA_HD int acc_global_bytes(int kind) {
+  switch (kind) {
+    case ACC_MIN_I32: case ACC_MAX_I32: return 4;
+    case ACC_SUM_F: return 8 * kLimbs;
+    default: return 8;
+  }
+}
+
+// SplitMix64 finalizer (Steele, Lea & Flood 2014): a bijection on 64 bits.
+TGA_HD uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+// MurmurHash3 fmix32: a bijection on 32 bits.
+TGA_HD uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+  return h;
+}
+
+// Row hash.  Independent bit fields feed the three placements, so keys of
+// one partition still spread over all slots of the tables (the reference uses
+// one hash modulo both P and the capacity, T/src/kernel.cu:225,327):
+//   partition  = top `pbits` bits
+//   LDS slot   = bits 0..31 (multiply-shift onto [0, slots))
+//   global slot= bits 16..47
+TGA_HD uint64_t key_hash(uint64_t k) { return mix64(k ^ 0x5851f42d4c957f2dULL); }
+TGA_HD uint32_t hash_partition(uint64_t h, int pbits) { return pbits ? (uint32_t)(h >> (64 - pbits)) : 0u; }
+TGA_HD uint32_t hash_lds_slot(uint64_t h, uint32_t slots) {
+  return (uint32_t)(((uint64_t)(uint32_t)h * slots) >> 32);
+}
+TGA_HD uint64_t hash_global_slot(uint64_t h, uint64_t slots) {
+  return ((h >> 16) & 0xffffffffULL) * slots >> 32;
+}
+
+// Order-preserving maps onto int64 for MIN/MAX.
+TGA_HD int64_t f64_bits_to_ordered(uint64_t b) { return (int64_t)(b ^ ((uint64_t)((int64_t)b >> 63) & 0x7fffffffffffffffULL)); }
+TGA_HD uint64_t ordered_to_f64_bits(int64_t o) { return (uint64_t)o ^ ((uint64_t)(o >> 63) & 0x7fffffffffffffffULL); }
+
+// Splits a double into at most three signed 32-bit chunks of its fixed-point
+// image; returns the first limb index, or -1 when nothing is added.  Sets
+// *range when x cannot be represented (|x| >= 2^64, Inf, NaN).
+TGA_HD int fx_split(uint64_t b, int64_t c[3], int *range) {
+  uint32_t e = (uint32_t)(b >> 52) & 0x7ffu;
+  uint64_t m = b & ((1ULL << 52) - 1);
+  if (e == 0x7ffu) { *range = 1; return -1; }
+  int ex;
+  if (e == 0) { if (m == 0) return -1; ex = -1074; }
+  else { m |= 1ULL << 52; ex = (int)e - 1075; }
+  int sh = ex + kFxShift;  // bit position of m's bit 0 in the fixed-point integer
+  if (sh < 0) {
+    if (sh <= -53) return -1;  // entirely below the window
+    m >>= -sh;
+    sh = 0;
+    if (m == 0) return -1;
+  }
+  if (sh + 53 > 32 * kLimbs) { *range = 1; return -1; }
+  int i0 = sh >> 5, lo = sh & 31;
+  uint32_t c0, c1, c2;
+  if (lo == 0) { c0 = (uint32_t)m; c1 = (uint32_t)(m >> 32); c2 = 0; }
+  else {
+    c0 = (uint32_t)(m << lo);
+    c1 = (uint32_t)(m >> (32 - lo));
+    c2 = lo > 11 ? (uint32_t)(m >> (64 - lo)) : 0u;
+  }
+  bool neg = (b >> 63) != 0;
+  c[0] = neg ? -(int64_t)c0 : (int64_t)c0;
+  c[1] = neg ? -(int64_t)c1 : (int64_t)c1;
+  c[2] = neg ? -(int64_t)c2 : (int64_t)c2;
+  return i0;
+}
+
+// Exact value of sum_i L[i] * 2^(32 i - kFxShift), rounded to nearest even.
+TGA_HD double fx_to_double(const int64_t *L) {
+  constexpr int W = kLimbs + 2;
+  uint32_t w[W];
+  __int128 carry = 0;
+  for (int i = 0; i < kLimbs; ++i) {
+    __int128 t = (__int128)L[i] + carry;
+    w[i] = (uint32_t)(t & 0xffffffff);
+    carry = t >> 32;
+  }
+  w[kLimbs] = (uint32_t)(carry & 0xffffffff);
+  w[kLimbs + 1] = (uint32_t)((carry >> 32) & 0xffffffff);
+  bool neg = carry < 0;
+  if (neg) {  // two's-complement negate of the W-word integer
+    uint64_t cy = 1;
+    for (int i = 0; i < W; ++i) {
+      uint64_t t = (uint64_t)(uint32_t)~w[i] + cy;
+      w[i] = (uint32_t)t;
+      cy = t >> 32;
+    }
+  }
+  int top = -1;
+  for (int i = W - 1; i >= 0; --i)
+    if (w[i]) { top = i; break; }
+  if (top < 0) return 0.0;
+  int hb = 32 * top + 31 - __builtin_clz(w[top]);  // highest set bit
+  // top 64 bits of the magnitude, plus a sticky bit for everything below
+  uint64_t mant = 0;
+  bool sticky = false;
+  for (int bit = 0; bit < 64; ++bit) {
+    int pos = hb - bit;
+    uint64_t v = 0;
+    if (pos >= 0) v = (w[pos >> 5] >> (pos & 31)) & 1u;
+    mant = (mant << 1) | v;
+  }
+  for (int pos = hb - 64; pos >= 0; --pos)
+    if ((w[pos >> 5] >> (pos & 31)) & 1u) { sticky = true; break; }
+  uint64_t q = mant >> 11;
+  uint64_t rem = mant & 0x7ff;
+  if (rem > 0x400 || (rem == 0x400 && (sticky || (q & 1)))) {
+    ++q;
+    if (q == (1ULL << 53)) { q >>= 1; ++hb; }
+  }
+  // value = q * 2^(hb - 52 - kFxShift), always a normal double here
+  int ex = hb - 52 - kFxShift;
+  double r = (double)q;
+  // scale by 2^ex exactly
+  while (ex > 0) { int s = ex > 60 ? 60 : ex; r *= (double)(1ULL << s); ex -= s; }
+  while (ex < 0) { int s = -ex > 60 ? 60 : -ex; r /= (double)(1ULL << s); ex += s; }
+  return neg ? -r : r;
+}
+
+template <typename K> struct KeyTraits;
+template <> struct KeyTraits<uint32_t> { static constexpr uint32_t kEmpty = 0xffffffffu; };
+template <> struct KeyTraits<uint64_t> { static constexpr uint64_t kEmpty = ~0ULL; };
+
+}  // namespace tga

@@ -1,0 +1,460 @@
+// tga_device.hpp -- device building blocks of the gfx950 group-by kernels:
+// wave64 match folding, the LDS open-addressing table, the global table in
+// HBM, and the decoupled look-back prefix used by every scan/compaction.
+#pragma once
+
+#include "tga_common.hpp"
+
+namespace tga {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = (int)__lane_id();
+  return l == 0 ? 0ULL : (~0ULL >> (64 - l));
+}
+
+// Mask of the active lanes whose `v` equals this lane's `v`, from one ballot
+// per bit (v < 2^nbits).  HIP's __match_any is a serial loop over lanes.
+__device__ __forceinline__ uint64_t match_mask(uint32_t v, int nbits, bool active) {
+  uint64_t m = __ballot(active);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (v >> b) & 1u;
+    const uint64_t bal = __ballot(active && bit);
+    m &= bit ? bal : ~bal;
+  }
+  return active ? m : 0ULL;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    if constexpr (sizeof(T) == 8) v += (T)shfl64((uint64_t)v, (int)(__lane_id() ^ o));
+    else v += (T)__shfl((int)v, (int)(__lane_id() ^ o), kWave);
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// column access
+// ---------------------------------------------------------------------------
+
+// Raw 64-bit image of one cell: 32-bit columns are sign/zero extended.
+__device__ __forceinline__ uint64_t load_cell(const Schema &s, int c, int64_t row) {
+  const void *p = s.cols[c];
+  switch (s.col_dtype[c]) {
+    case DT_U32: return (uint64_t)((const uint32_t *)p)[row];
+    case DT_I32: return (uint64_t)(int64_t)((const int32_t *)p)[row];
+    default: return ((const uint64_t *)p)[row];
+  }
+}
+
+// Operand an accumulator folds in for one row.
+__device__ __forceinline__ uint64_t acc_operand(const Schema &s, int a, const uint64_t *cell) {
+  const Acc ac = s.acc[a];
+  if (ac.kind == ACC_COUNT) return 1;
+  const uint64_t raw = cell[ac.col];
+  if (ac.kind == ACC_MIN_I64 || ac.kind == ACC_MAX_I64) {
+    const int dt = s.col_dtype[ac.col];
+    if (dt == DT_U64) return raw ^ (1ULL << 63);
+    if (dt == DT_F64) return (uint64_t)f64_bits_to_ordered(raw);
+  }
+  return raw;
+}
+
+// ---------------------------------------------------------------------------
+// identity values and combine rules
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t acc_identity(int kind) {
+  switch (kind) {
+    case ACC_MIN_I32: return (uint64_t)(uint32_t)INT32_MAX;
+    case ACC_MAX_I32: return (uint64_t)(uint32_t)INT32_MIN;
+    case ACC_MIN_I64: return (uint64_t)INT64_MAX;
+    case ACC_MAX_I64: return (uint64_t)INT64_MIN;
+    default: return 0;
+  }
+}
+
+// Fold operand b into a (registers), for every kind except ACC_SUM_F.
+__device__ __forceinline__ uint64_t acc_fold(int kind, uint64_t a, uint64_t b) {
+  switch (kind) {
+    case ACC_COUNT: case ACC_SUM_I: return a + b;
+    case ACC_MIN_I32: return (uint64_t)(int64_t)min((int32_t)a, (int32_t)b);
+    case ACC_MAX_I32: return (uint64_t)(int64_t)max((int32_t)a, (int32_t)b);
+    case ACC_MIN_I64: return (uint64_t)min((int64_t)a, (int64_t)b);
+    case ACC_MAX_I64: return (uint64_t)max((int64_t)a, (int64_t)b);
+    default: return a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS table: keys[slots+1] plus one SoA array per accumulator; index `slots`
+// holds the key equal to the empty marker.
+// ---------------------------------------------------------------------------
+
+template <typename K>
+struct LdsTable {
+  char *base;
+  Layout L;
+  uint32_t *sentinel;  // nonzero once the empty-marker key was inserted
+
+  __device__ __forceinline__ K *keys() const { return (K *)(base + L.key_off); }
+  template <typename T>
+  __device__ __forceinline__ T *arr(int a) const { return (T *)(base + L.acc_off[a]); }
+};
+
+template <typename K>
+__device__ void lds_init(const LdsTable<K> &t, const Schema &s) {
+  const uint32_t n = t.L.slots + 1;
+  K *k = t.keys();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) k[i] = KeyTraits<K>::kEmpty;
+  for (int a = 0; a < s.nacc; ++a) {
+    const int kind = s.acc[a].kind;
+    if (kind == ACC_SUM_F) {
+      int64_t *p = t.template arr<int64_t>(a);
+      for (uint32_t i = threadIdx.x; i < n * kLimbs; i += blockDim.x) p[i] = 0;
+    } else if (acc_lds_bytes(kind) == 4) {
+      uint32_t *p = t.template arr<uint32_t>(a);
+      const uint32_t id = (uint32_t)acc_identity(kind);
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = id;
+    } else {
+      uint64_t *p = t.template arr<uint64_t>(a);
+      const uint64_t id = acc_identity(kind);
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = id;
+    }
+  }
+  if (threadIdx.x == 0) *t.sentinel = 0;
+}
+
+// Slot of `key`, inserting it if absent; -1 when `max_probe` slots were full.
+template <typename K>
+__device__ __forceinline__ int lds_find_or_insert(const LdsTable<K> &t, K key, uint64_t h, int max_probe) {
+  const uint32_t S = t.L.slots;
+  if (key == KeyTraits<K>::kEmpty) {
+    *t.sentinel = 1;
+    return (int)S;
+  }
+  K *keys = t.keys();
+  uint32_t s = hash_lds_slot(h, S);
+  for (int i = 0; i < max_probe; ++i) {
+    K cur = keys[s];
+    if (cur == key) return (int)s;
+    if (cur == KeyTraits<K>::kEmpty) {
+      const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
+      if (prev == KeyTraits<K>::kEmpty || prev == key) return (int)s;
+    }
+    s = (s + 1 == S) ? 0 : s + 1;
+  }
+  return -1;
+}
+
+// Apply one folded operand to LDS slot `s`.
+template <typename K>
+__device__ __forceinline__ void lds_apply(const LdsTable<K> &t, int a, int kind, int s, uint64_t v) {
+  switch (kind) {
+    case ACC_COUNT: atomicAdd(t.template arr<uint32_t>(a) + s, (uint32_t)v); break;
+    case ACC_SUM_I: atomicAdd(t.template arr<unsigned long long>(a) + s, (unsigned long long)v); break;
+    case ACC_MIN_I32: atomicMin(t.template arr<int>(a) + s, (int)(int32_t)v); break;
+    case ACC_MAX_I32: atomicMax(t.template arr<int>(a) + s, (int)(int32_t)v); break;
+    case ACC_MIN_I64: atomicMin(t.template arr<long long>(a) + s, (long long)v); break;
+    case ACC_MAX_I64: atomicMax(t.template arr<long long>(a) + s, (long long)v); break;
+    default: break;
+  }
+}
+
+template <typename K>
+__device__ __forceinline__ void lds_apply_limbs(const LdsTable<K> &t, int a, int s, const int64_t *lv) {
+  unsigned long long *p = t.template arr<unsigned long long>(a);
+  const uint32_t n = t.L.slots + 1;
+#pragma unroll
+  for (int l = 0; l < kLimbs; ++l)
+    if (lv[l]) atomicAdd(p + (size_t)l * n + s, (unsigned long long)lv[l]);
+}
+
+// ---------------------------------------------------------------------------
+// global table in HBM: the same SoA shape with `S` regular slots
+// ---------------------------------------------------------------------------
+
+template <typename K>
+__device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint64_t h) {
+  // flags[FLAG_GROUPS] counts the groups inserted so far
+  if (key == KeyTraits<K>::kEmpty) {
+    if (atomicOr(&g.flags[FLAG_SENTINEL], 1u) == 0u) atomicAdd(&g.flags[FLAG_GROUPS], 1u);
+    return (int64_t)g.S;
+  }
+  K *keys = (K *)g.keys;
+  const uint64_t S = g.S;
+  uint64_t s = hash_global_slot(h, S);
+  for (uint64_t i = 0; i < S; ++i) {
+    const K cur = keys[s];
+    if (cur == key) return (int64_t)s;
+    if (cur == KeyTraits<K>::kEmpty) {
+      const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
+      if (prev == KeyTraits<K>::kEmpty) {
+        atomicAdd(&g.flags[FLAG_GROUPS], 1u);
+        return (int64_t)s;
+      }
+      if (prev == key) return (int64_t)s;
+    }
+    s = (s + 1 == S) ? 0 : s + 1;
+  }
+  atomicOr(&g.flags[FLAG_OVERFLOW], 1u);
+  return -1;
+}
+
+__device__ __forceinline__ void g_apply(const GTable &g, int a, int kind, int64_t s, uint64_t v) {
+  switch (kind) {
+    case ACC_COUNT: case ACC_SUM_I: atomicAdd((unsigned long long *)g.acc[a] + s, (unsigned long long)v); break;
+    case ACC_MIN_I32: atomicMin((int *)g.acc[a] + s, (int)(int32_t)v); break;
+    case ACC_MAX_I32: atomicMax((int *)g.acc[a] + s, (int)(int32_t)v); break;
+    case ACC_MIN_I64: atomicMin((long long *)g.acc[a] + s, (long long)v); break;
+    case ACC_MAX_I64: atomicMax((long long *)g.acc[a] + s, (long long)v); break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ void g_apply_limbs(const GTable &g, int a, int64_t s, const int64_t *lv) {
+  unsigned long long *p = (unsigned long long *)g.acc[a];
+  const uint64_t n = g.S + 1;
+#pragma unroll
+  for (int l = 0; l < kLimbs; ++l)
+    if (lv[l]) atomicAdd(p + (uint64_t)l * n + s, (unsigned long long)lv[l]);
+}
+
+// Expand a double into a limb vector (zeros except at most three limbs).
+__device__ __forceinline__ void fx_expand(uint64_t bits, int64_t *lv, uint32_t *flags) {
+#pragma unroll
+  for (int l = 0; l < kLimbs; ++l) lv[l] = 0;
+  int64_t c[3];
+  int range = 0;
+  const int i0 = fx_split(bits, c, &range);
+  if (range) atomicOr(&flags[FLAG_RANGE], 1u);
+  if (i0 < 0) return;
+#pragma unroll
+  for (int l = 0; l < kLimbs; ++l) {
+    const int d = l - i0;
+    if (d == 0) lv[l] = c[0];
+    else if (d == 1) lv[l] = c[1];
+    else if (d == 2) lv[l] = c[2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decoupled look-back (Merrill & Garland 2016)
+//
+// desc[t] = flag (bits 63..62: 0 nothing yet, 1 tile aggregate, 2 inclusive
+// prefix) | value (bits 61..0).  Tile ids come from an atomic ticket, so every
+// lower tile belongs to a workgroup that started earlier and is running; the
+// spin below therefore always ends.
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t kLbAgg = 1ULL << 62;
+constexpr uint64_t kLbInc = 2ULL << 62;
+constexpr uint64_t kLbVal = (1ULL << 62) - 1;
+
+__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lb_load(uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of one wave.  Returns the exclusive prefix of tile
+// `tile` and publishes its inclusive prefix.
+__device__ uint64_t lookback_exclusive(uint64_t *desc, uint64_t tile, uint64_t agg) {
+  const int lane = (int)__lane_id();
+  if (tile == 0) {
+    if (lane == 0) lb_store(&desc[0], kLbInc | (agg & kLbVal));
+    return 0;
+  }
+  if (lane == 0) lb_store(&desc[tile], kLbAgg | (agg & kLbVal));
+  uint64_t excl = 0;
+  int64_t end = (int64_t)tile;  // looking at tiles [end-64, end)
+  for (;;) {
+    const int64_t idx = end - 1 - lane;
+    uint64_t d = idx >= 0 ? lb_load(&desc[idx]) : kLbInc;
+    while (__ballot((d >> 62) == 0) != 0) {
+      if ((d >> 62) == 0) {
+        __builtin_amdgcn_s_sleep(1);
+        d = lb_load(&desc[idx]);
+      }
+    }
+    const uint64_t incmask = __ballot((d >> 62) == 2);
+    if (incmask) {
+      const int k = __builtin_ctzll(incmask);  // nearest tile with an inclusive prefix
+      const uint64_t v = lane <= k ? (d & kLbVal) : 0;
+      excl += wave_sum<uint64_t>(v);
+      break;
+    }
+    excl += wave_sum<uint64_t>(d & kLbVal);
+    end -= kWave;
+  }
+  if (lane == 0) lb_store(&desc[tile], kLbInc | ((excl + agg) & kLbVal));
+  return excl;
+}
+
+// ---------------------------------------------------------------------------
+// block-wide helpers and the row -> LDS table -> global table pipeline
+// ---------------------------------------------------------------------------
+
+// Exclusive scan of n u32 values held in LDS, in place; returns the total.
+// scratch needs blockDim/64 + 1 words.  All threads must call it.
+__device__ inline uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t per = (n + nt - 1) / nt;
+  const uint32_t b = tid * per, e = min(n, b + per);
+  uint32_t local = 0;
+  for (uint32_t i = b; i < e; ++i) local += arr[i];
+  // inclusive wave scan of the per-thread sums
+  const int lane = (int)__lane_id(), wave = (int)(tid >> 6);
+  uint32_t x = local;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) scratch[wave] = x;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    const uint32_t nw = nt >> 6;
+    for (uint32_t w = 0; w < nw; ++w) { const uint32_t v = scratch[w]; scratch[w] = run; run += v; }
+    scratch[nw] = run;
+  }
+  __syncthreads();
+  uint32_t run = scratch[wave] + x - local;
+  for (uint32_t i = b; i < e; ++i) { const uint32_t v = arr[i]; arr[i] = run; run += v; }
+  const uint32_t total = scratch[nt >> 6];
+  __syncthreads();
+  return total;
+}
+
+// Aggregates rows [b, e) into the LDS table.  Rows whose key finds no free
+// slot within max_probe go straight to the global table.
+template <typename K>
+__device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b, uint64_t e,
+                         const LdsTable<K> &t, const GTable &g, int nbits, int max_probe) {
+  constexpr int U = 4;
+  const int lane = (int)__lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  const int nw = (int)(blockDim.x >> 6);
+  const int ncol = s.ncols, nacc = s.nacc;
+  for (uint64_t base = b + (uint64_t)wave * kWave * U; base < e; base += (uint64_t)nw * kWave * U) {
+    K kk[U];
+    uint64_t cell[U][kMaxCols];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t row = base + (uint64_t)u * kWave + lane;
+      const bool act = row < e;
+      kk[u] = act ? keys[row] : KeyTraits<K>::kEmpty;
+#pragma unroll
+      for (int c = 0; c < kMaxCols; ++c) cell[u][c] = (act && c < ncol) ? load_cell(s, c, row) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t row = base + (uint64_t)u * kWave + lane;
+      const bool act = row < e;
+      if (!__ballot(act)) break;
+      const K key = kk[u];
+      const uint64_t h = key_hash((uint64_t)key);
+      const int slot = act ? lds_find_or_insert(t, key, h, max_probe) : -1;
+      uint64_t ops[kMaxAccs];
+      int64_t lv[kMaxFx][kLimbs];
+#pragma unroll
+      for (int a = 0; a < kMaxAccs; ++a) {
+        ops[a] = 0;
+        if (a < nacc && act) {
+          ops[a] = acc_operand(s, a, cell[u]);
+          if (s.acc[a].kind == ACC_SUM_F) fx_expand(ops[a], lv[s.fx_slot[a]], g.flags);
+        }
+      }
+      const bool spill = act && slot < 0;
+      if (__ballot(spill)) {
+        if (spill) {
+          const int64_t gs = g_find_or_insert<K>(g, key, h);
+          if (gs >= 0) {
+#pragma unroll
+            for (int a = 0; a < kMaxAccs; ++a) {
+              if (a >= nacc) break;
+              const int kind = s.acc[a].kind;
+              if (kind == ACC_SUM_F) g_apply_limbs(g, a, gs, lv[s.fx_slot[a]]);
+              else g_apply(g, a, kind, gs, ops[a]);
+            }
+          }
+        }
+      }
+      const bool on = act && slot >= 0;
+      const uint64_t m = match_mask((uint32_t)slot, nbits, on);
+      const bool leader = on && ((m & lanemask_lt()) == 0);
+      uint64_t rest = leader ? (m & (m - 1)) : 0ULL;
+      // fold duplicate keys of this wave into their leader lane
+      while (__ballot(rest != 0)) {
+        const int src = rest ? (int)__builtin_ctzll(rest) : lane;
+#pragma unroll
+        for (int a = 0; a < kMaxAccs; ++a) {
+          if (a >= nacc) break;
+          const int kind = s.acc[a].kind;
+          if (kind == ACC_SUM_F) {
+            const int f = s.fx_slot[a];
+#pragma unroll
+            for (int l = 0; l < kLimbs; ++l) {
+              const int64_t o = (int64_t)shfl64((uint64_t)lv[f][l], src);
+              if (rest) lv[f][l] += o;
+            }
+          } else {
+            const uint64_t o = shfl64(ops[a], src);
+            if (rest) ops[a] = acc_fold(kind, ops[a], o);
+          }
+        }
+        if (rest) rest &= rest - 1;
+      }
+      if (leader) {
+#pragma unroll
+        for (int a = 0; a < kMaxAccs; ++a) {
+          if (a >= nacc) break;
+          const int kind = s.acc[a].kind;
+          if (kind == ACC_SUM_F) lds_apply_limbs(t, a, slot, lv[s.fx_slot[a]]);
+          else lds_apply(t, a, kind, slot, ops[a]);
+        }
+      }
+    }
+  }
+}
+
+template <typename K>
+__device__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g) {
+  const uint32_t S = t.L.slots, n = S + 1;
+  const K *keys = t.keys();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    K key;
+    bool occ;
+    if (i < S) { key = keys[i]; occ = key != KeyTraits<K>::kEmpty; }
+    else { key = KeyTraits<K>::kEmpty; occ = *t.sentinel != 0; }
+    if (!occ) continue;
+    const int64_t gs = g_find_or_insert<K>(g, key, key_hash((uint64_t)key));
+    if (gs < 0) continue;
+    for (int a = 0; a < s.nacc; ++a) {
+      const int kind = s.acc[a].kind;
+      if (kind == ACC_SUM_F) {
+        int64_t lv[kLimbs];
+        const int64_t *p = t.template arr<int64_t>(a);
+#pragma unroll
+        for (int l = 0; l < kLimbs; ++l) lv[l] = p[(size_t)l * n + i];
+        g_apply_limbs(g, a, gs, lv);
+      } else if (acc_lds_bytes(kind) == 4) {
+        const uint32_t v = t.template arr<uint32_t>(a)[i];
+        g_apply(g, a, kind, gs, kind == ACC_COUNT ? (uint64_t)v : (uint64_t)(int64_t)(int32_t)v);
+      } else {
+        g_apply(g, a, kind, gs, t.template arr<uint64_t>(a)[i]);
+      }
+    }
+  }
+}
+
+}  // namespace tga

@@ -1,0 +1,196 @@
+// tga_host.hpp -- host-side runtime shared by the C ABI translation units:
+// the context (device, stream, lock), the HBM caching arena, kernel timing,
+// schema construction and the autotuner.
+#pragma once
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tga_groupby.h"
+#include "tga_internal.hpp"
+
+namespace tga {
+
+// ---------------------------------------------------------------------------
+// caching arena over hipMalloc with a byte budget
+// ---------------------------------------------------------------------------
+class Arena {
+ public:
+  explicit Arena(uint64_t budget) : budget_(budget) {}
+  ~Arena() { release_cached(); }
+  // Returns nullptr when the budget or the device is exhausted.
+  void *alloc(uint64_t bytes);
+  void free(void *p);
+  void release_cached();
+  uint64_t in_use() const { return in_use_; }
+  uint64_t peak() const { return peak_; }
+  uint64_t budget() const { return budget_; }
+  static uint64_t round(uint64_t bytes);
+
+ private:
+  std::mutex mu_;
+  uint64_t budget_;
+  uint64_t in_use_ = 0, cached_ = 0, peak_ = 0;
+  std::map<uint64_t, std::vector<void *>> free_;  // size class -> blocks
+  std::map<void *, uint64_t> live_;
+};
+
+// ---------------------------------------------------------------------------
+// per-kernel HIP-event timing on the context stream
+// ---------------------------------------------------------------------------
+class Profiler {
+ public:
+  bool on = false;
+  void begin(const char *name, hipStream_t st);
+  void end(hipStream_t st);
+  void resolve();  // after the stream was synchronised
+  void reset();
+  void destroy();
+  std::vector<std::pair<std::string, std::pair<int64_t, double>>> entries() const;
+
+ private:
+  struct Pending { std::string name; hipEvent_t a, b; };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::map<std::string, std::pair<int64_t, double>> acc_;
+  std::vector<std::string> order_;
+  hipEvent_t get();
+};
+
+}  // namespace tga
+
+struct tga_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t props{};
+  int cus = 0;
+  size_t lds_max = 65536;  // dynamic LDS bytes one workgroup may use
+  std::recursive_mutex mu;
+  tga::Arena *arena = nullptr;
+  tga::Profiler prof;
+  std::string err;
+  tga_options last_plan{};
+};
+
+namespace tga {
+
+// Sets the context's error text (printf-style) and returns `st`.
+tga_status fail(tga_context *ctx, tga_status st, const char *fmt, ...);
+
+#define TGA_HIP(ctx, call)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return ::tga::fail((ctx), TGA_ERR_HIP, "%s: %s (%s:%d)", #call,       \
+                                             hipGetErrorString(e_), __FILE__, __LINE__);        \
+  } while (0)
+
+// RAII device buffer from the context arena.
+struct DBuf {
+  tga_context *ctx = nullptr;
+  void *p = nullptr;
+  uint64_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept { *this = std::move(o); }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) { reset(); ctx = o.ctx; p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() { if (p && ctx) ctx->arena->free(p); p = nullptr; bytes = 0; }
+  bool alloc(tga_context *c, uint64_t n) {
+    reset();
+    ctx = c;
+    bytes = n ? n : 1;
+    p = c->arena->alloc(bytes);
+    return p != nullptr;
+  }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+// A validated query: schema + key info + output dtypes.
+struct QuerySpec {
+  Schema schema{};
+  int key_bytes = 8;
+  int key_dtype = DT_U64;
+  int32_t out_dtype[kMaxAccs]{};
+  int nfx = 0;
+};
+
+// Builds the accumulator schema from the aggregate list; returns an error
+// text (empty on success).
+std::string build_spec(const tga_column *key, const tga_column *values, int32_t n_values, const tga_agg *aggs,
+                       int32_t n_aggs, QuerySpec &q);
+
+// LDS layout of a table with `slots` slots for this schema.
+Layout make_layout(const Schema &s, int key_bytes, uint32_t slots);
+uint32_t lds_slot_bytes(const Schema &s, int key_bytes);
+
+struct Plan {
+  int path = TGA_PATH_DIRECT;
+  int pbits = 0;
+  uint32_t table_slots = 0;
+  uint32_t tile_rows = 8192;
+  uint32_t workgroups = 0;
+  uint32_t block = 256;
+  uint64_t chunk_rows = 0;
+  uint64_t gslots = 0;  // global table slots
+  Layout layout{};
+  int nbits = 0;
+  int max_probe = 0;
+};
+
+// Autotuner: plan from (rows, estimated groups, schema, device), honouring
+// any explicit field of `opt`.
+Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint64_t est_groups,
+              const tga_options *opt);
+
+// Cardinality estimate from a strided key sample (GEE estimator,
+// Charikar et al. 2000).  Needs the context lock.
+tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est);
+
+// Output of one plain group-by execution (arena buffers owned by caller).
+struct PlainOut {
+  DBuf keys;
+  DBuf cols[kMaxAccs];
+  uint64_t groups = 0;
+  uint64_t capacity = 0;
+};
+
+// Runs the plain operator on device-resident columns.  If `ext` is non-null
+// the results go to those caller buffers with capacity ext_cap.
+tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n_rows, const Plan &plan_in,
+                     PlainOut *out, void *ext_keys, void *const *ext_cols, uint64_t ext_cap, uint64_t *groups);
+
+hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable &g, std::vector<DBuf> &bufs,
+                        DBuf &flags);
+tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, const Plan &plan,
+                          const GTable &g, std::vector<DBuf> &keep);
+tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, const OutDesc &od, uint32_t *hflags,
+                        uint64_t *hcount);
+
+// Timed launch helper.
+template <typename F>
+hipError_t timed(tga_context *ctx, const char *name, F &&f) {
+  if (ctx->prof.on) ctx->prof.begin(name, ctx->stream);
+  hipError_t e = f();
+  if (ctx->prof.on) ctx->prof.end(ctx->stream);
+  return e;
+}
+
+}  // namespace tga
+
+struct tga_result {
+  tga_context *ctx = nullptr;
+  tga::PlainOut out;
+  int32_t key_dtype = 0;
+  int32_t naggs = 0;
+  int32_t dtype[tga::kMaxAccs]{};
+};

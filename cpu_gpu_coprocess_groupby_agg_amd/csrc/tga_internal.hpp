@@ -1,0 +1,57 @@
+// tga_internal.hpp -- host-side launch interface between the C ABI layer and
+// the kernels (each kernel is launched from the translation unit defining it).
+#pragma once
+
+#include <cmath>
+
+#include "tga_common.hpp"
+
+namespace tga {
+
+// Work plan of k_agg: segment p covers rows [pstart[p], pstart[p+1]) and is
+// cut into cstart[p+1]-cstart[p] slices; ticket hands out slice ids.
+struct AggPlan {
+  uint64_t *pstart;  // [P+1]
+  uint32_t *cstart;  // [P+1]
+  uint32_t P;
+  uint32_t *ticket;
+};
+
+struct PartOut {
+  void *cols[kMaxCols];
+};
+
+struct OutDesc {
+  void *keys;
+  void *cols[kMaxAccs];
+  uint64_t capacity;
+};
+
+enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_ZIPF_U64 = 4, GEN_NORMAL_F64 = 5 };
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st);
+hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
+                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st);
+hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
+hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
+                              const AggPlan &plan, hipStream_t st);
+hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
+                             uint32_t ntiles, uint32_t *cnt, hipStream_t st);
+uint64_t scan_tiles(uint64_t n);
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
+                           hipStream_t st);
+size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows);
+hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
+                               const PartOut &o, hipStream_t st);
+uint64_t compact_tiles(uint64_t slots_plus_one);
+hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
+                          uint32_t *ticket, uint64_t *count_out, hipStream_t st);
+hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
+                              hipStream_t st);
+hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,
+                      uint64_t salt, hipStream_t st);
+
+}  // namespace tga

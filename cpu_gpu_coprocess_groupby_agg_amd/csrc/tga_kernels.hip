@@ -1,0 +1,609 @@
+// tga_kernels.hip -- gfx950 kernels of the plain group-by operator.
+//
+//   (a) k_agg          LDS open-addressing pre-aggregation with wave64 match
+//                      folding; flushes each workgroup table into the global
+//                      table.  Used directly on the input when the groups fit
+//                      one LDS table, and per partition after (b).
+//   (b) k_part_count   per-tile partition histogram
+//       k_scan_u32     decoupled-look-back exclusive scan of the counts
+//       k_part_scatter LDS counting sort of a tile, then coalesced runs
+//   (c) k_plan_chunks  partition -> work-item plan for k_agg
+//   (d) k_compact      one-pass compaction of the global table (own look-back)
+//
+// Reference counterparts (T/ = tiled_groupby_agg_partition/):
+//   K1-K5 pre-aggregation      T/src/kernel.cu:9-186     -> k_agg
+//   K6/K7 + CUB scan            T/src/kernel.cu:189-291, T/src/util.cu:68-88
+//                                                        -> k_part_count/k_scan_u32/k_part_scatter
+//   K8/K9 intra-partition agg   T/src/kernel.cu:294-378  -> k_agg over partitions
+//   S5 + K10 compaction         T/src/kernel.cu:380-406  -> k_compact
+#include "tga_device.hpp"
+#include "tga_internal.hpp"
+
+namespace tga {
+
+// ---------------------------------------------------------------------------
+// fills
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename K>
+__global__ void k_gtable_init(GTable g, Schema s) {
+  const uint64_t n = g.S + 1;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  K *keys = (K *)g.keys;
+  for (uint64_t i = t0; i < n; i += stride) keys[i] = KeyTraits<K>::kEmpty;
+  for (int a = 0; a < s.nacc; ++a) {
+    const int kind = s.acc[a].kind;
+    if (kind == ACC_SUM_F) {
+      uint64_t *p = (uint64_t *)g.acc[a];
+      for (uint64_t i = t0; i < n * kLimbs; i += stride) p[i] = 0;
+    } else if (acc_global_bytes(kind) == 4) {
+      uint32_t *p = (uint32_t *)g.acc[a];
+      const uint32_t id = (uint32_t)acc_identity(kind);
+      for (uint64_t i = t0; i < n; i += stride) p[i] = id;
+    } else {
+      uint64_t *p = (uint64_t *)g.acc[a];
+      const uint64_t id = acc_identity(kind);
+      for (uint64_t i = t0; i < n; i += stride) p[i] = id;
+    }
+  }
+  if (t0 < kFlagWords) g.flags[t0] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// (a)/(c) LDS aggregation
+// ---------------------------------------------------------------------------
+
+// Persistent: workgroups pull work items from plan.ticket until none remain.
+// Item -> (segment p, slice j of nch) through plan.cstart; rows of segment p
+// are [pstart[p], pstart[p+1]).
+template <typename K>
+__global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
+                                              GTable g, int nbits, int max_probe) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t s_item, s_sentinel;
+  LdsTable<K> t{smem, L, &s_sentinel};
+  const uint32_t total = plan.cstart[plan.P];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(plan.ticket, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    if (item >= total) break;
+    uint32_t lo = 0, hi = plan.P;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (plan.cstart[mid] <= item) lo = mid; else hi = mid;
+    }
+    const uint32_t p = lo;
+    const uint32_t j = item - plan.cstart[p], nch = plan.cstart[p + 1] - plan.cstart[p];
+    const uint64_t sb = plan.pstart[p], se = plan.pstart[p + 1], len = se - sb;
+    const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
+    lds_init(t, s);
+    __syncthreads();
+    agg_rows<K>(keys, s, b, e, t, g, nbits, max_probe);
+    __syncthreads();
+    lds_flush<K>(t, s, g);
+    __syncthreads();
+  }
+}
+
+// Single-segment plan over rows [0, n) in nchunks slices.
+__global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
+  if (threadIdx.x == 0) {
+    plan.pstart[0] = 0;
+    plan.pstart[1] = n;
+    plan.cstart[0] = 0;
+    plan.cstart[1] = nchunks;
+    *plan.ticket = 0;
+  }
+}
+
+// (c) partition p has rows [offs[p*ntiles], offs[(p+1)*ntiles]) after the
+// scan; it becomes ceil(rows/chmax) work items.
+__global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n,
+                                                      uint64_t chmax, AggPlan plan) {
+  __shared__ uint32_t scratch[17];
+  extern __shared__ uint32_t cnt[];  // P + 1
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    const uint64_t b = offs[(uint64_t)p * ntiles];
+    const uint64_t e = p + 1 < P ? offs[(uint64_t)(p + 1) * ntiles] : n;
+    plan.pstart[p] = b;
+    const uint64_t len = e - b;
+    cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
+  }
+  if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
+  __syncthreads();
+  const uint32_t total = block_exclusive_scan_lds(cnt, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) plan.cstart[p] = cnt[p];
+  if (threadIdx.x == 0) plan.cstart[P] = total;
+}
+
+// ---------------------------------------------------------------------------
+// (b) partitioning
+// ---------------------------------------------------------------------------
+
+template <typename K>
+__global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys, uint64_t n, int pbits,
+                                                     uint32_t tile_rows, uint32_t ntiles, uint32_t *cnt) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t P = 1u << pbits;
+  const uint32_t tile = blockIdx.x;
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const uint64_t b = (uint64_t)tile * tile_rows;
+  const uint64_t e = min(n, b + tile_rows);
+#pragma unroll 8
+  for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
+    atomicAdd(&hist[hash_partition(key_hash((uint64_t)keys[r]), pbits)], 1u);
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[(uint64_t)p * ntiles + tile] = hist[p];
+}
+
+// Exclusive scan of n u32 (total < 2^32), 4096 values per tile, tiles in
+// ticket order, offsets from the decoupled look-back.
+constexpr int kScanThreads = 256, kScanItems = 16;
+__global__ void __launch_bounds__(kScanThreads) k_scan_u32(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+                                                           uint64_t n, uint64_t *desc, uint32_t *ticket) {
+  __shared__ uint32_t s_tile, s_excl;
+  __shared__ uint32_t scratch[kScanThreads / kWave + 1];
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t base = tile * (kScanThreads * kScanItems) + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < n ? in[base + i] : 0u;
+    local += v[i];
+  }
+  // block exclusive scan of the thread sums
+  const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+  uint32_t x = local;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) scratch[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < kScanThreads / kWave; ++w) { const uint32_t t = scratch[w]; scratch[w] = run; run += t; }
+    scratch[kScanThreads / kWave] = run;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const uint64_t ex = lookback_exclusive(desc, tile, scratch[kScanThreads / kWave]);
+    if (lane == 0) s_excl = (uint32_t)ex;
+  }
+  __syncthreads();
+  uint32_t run = s_excl + scratch[wave] + x - local;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+}
+
+// Tile of blockDim*ITEMS rows: rank rows by partition in LDS, then write each
+// partition's run contiguously at its scanned offset.
+template <typename K, int ITEMS>
+__global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
+                                                       uint32_t ntiles, const uint32_t *__restrict__ offs, K *okeys,
+                                                       PartOut o) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t scratch[17];
+  const uint32_t P = 1u << pbits;
+  const uint32_t TR = blockDim.x * ITEMS;
+  uint32_t *hist = (uint32_t *)smem;
+  uint32_t *goff = hist + P;
+  char *stage = (char *)(goff + P);
+  // stage layout: K keys[TR] | per column [TR] cells | u16 pid[TR]
+  K *skey = (K *)stage;
+  char *scol = stage + (size_t)TR * sizeof(K);
+  const int ncol = s.ncols;
+  uint32_t colb[kMaxCols];
+  size_t coff[kMaxCols];
+  size_t acc_off = 0;
+#pragma unroll
+  for (int c = 0; c < kMaxCols; ++c) {
+    colb[c] = c < ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
+    coff[c] = acc_off;
+    acc_off += (size_t)TR * colb[c];
+  }
+  uint16_t *spid = (uint16_t *)(scol + acc_off);
+
+  const uint32_t tile = blockIdx.x;
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const uint64_t b = (uint64_t)tile * TR;
+  K kk[ITEMS];
+  uint32_t pp[ITEMS], rk[ITEMS];
+  uint64_t cell[ITEMS][kMaxCols];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    const bool act = row < n;
+    kk[i] = act ? keys[row] : KeyTraits<K>::kEmpty;
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) cell[i][c] = (act && c < ncol) ? load_cell(s, c, row) : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    pp[i] = hash_partition(key_hash((uint64_t)kk[i]), pbits);
+    rk[i] = row < n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
+  }
+  __syncthreads();
+  block_exclusive_scan_lds(hist, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = offs[(uint64_t)p * ntiles + tile];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+    if (row < n) {
+      const uint32_t lp = hist[pp[i]] + rk[i];
+      skey[lp] = kk[i];
+#pragma unroll
+      for (int c = 0; c < kMaxCols; ++c) {
+        if (c >= ncol) break;
+        if (colb[c] == 4) ((uint32_t *)(scol + coff[c]))[lp] = (uint32_t)cell[i][c];
+        else ((uint64_t *)(scol + coff[c]))[lp] = cell[i][c];
+      }
+      spid[lp] = (uint16_t)pp[i];
+    }
+  }
+  __syncthreads();
+  const uint32_t rows = (uint32_t)min((uint64_t)TR, n - b);
+  for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+    const uint32_t p = spid[j];
+    const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
+    okeys[dst] = skey[j];
+#pragma unroll
+    for (int c = 0; c < kMaxCols; ++c) {
+      if (c >= ncol) break;
+      if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[j];
+      else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (d) compaction: occupied slots of the global table -> dense output, in slot
+// order, one pass with the look-back for the output offsets.
+// ---------------------------------------------------------------------------
+
+constexpr int kCompactThreads = 256, kCompactItems = 8;
+
+template <typename K>
+__global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s, OutDesc o, uint64_t *desc,
+                                                             uint32_t *ticket, uint64_t *count_out) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t wcnt[kCompactItems][kCompactThreads / kWave];
+  __shared__ uint32_t s_total;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t n = g.S + 1;
+  const uint64_t TS = (uint64_t)kCompactThreads * kCompactItems;
+  const uint64_t base = tile * TS;
+  const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+  const K *keys = (const K *)g.keys;
+  bool occ[kCompactItems];
+  uint32_t wpre[kCompactItems];
+#pragma unroll
+  for (int i = 0; i < kCompactItems; ++i) {
+    const uint64_t slot = base + (uint64_t)i * kCompactThreads + threadIdx.x;
+    bool oc = false;
+    if (slot < g.S) oc = keys[slot] != KeyTraits<K>::kEmpty;
+    else if (slot == g.S) oc = g.flags[FLAG_SENTINEL] != 0;
+    occ[i] = oc;
+    const uint64_t bal = __ballot(oc);
+    wpre[i] = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
+    if (lane == 0) wcnt[i][wave] = (uint32_t)__builtin_popcountll(bal);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < kCompactItems; ++i)
+      for (int w = 0; w < kCompactThreads / kWave; ++w) { const uint32_t c = wcnt[i][w]; wcnt[i][w] = run; run += c; }
+    s_total = run;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const uint64_t ex = lookback_exclusive(desc, tile, s_total);
+    if (lane == 0) {
+      s_excl = ex;
+      if (base + TS >= n) *count_out = ex + s_total;  // last tile
+    }
+  }
+  __syncthreads();
+  const uint64_t excl = s_excl;
+#pragma unroll
+  for (int i = 0; i < kCompactItems; ++i) {
+    if (!occ[i]) continue;
+    const uint64_t slot = base + (uint64_t)i * kCompactThreads + threadIdx.x;
+    const uint64_t pos = excl + wcnt[i][wave] + wpre[i];
+    if (pos >= o.capacity) continue;
+    ((K *)o.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
+    for (int a = 0; a < s.nacc; ++a) {
+      const int kind = s.acc[a].kind;
+      switch (kind) {
+        case ACC_COUNT: case ACC_SUM_I:
+          ((uint64_t *)o.cols[a])[pos] = ((const uint64_t *)g.acc[a])[slot];
+          break;
+        case ACC_MIN_I32: case ACC_MAX_I32:
+          ((uint32_t *)o.cols[a])[pos] = ((const uint32_t *)g.acc[a])[slot];
+          break;
+        case ACC_SUM_F: {
+          int64_t L[kLimbs];
+#pragma unroll
+          for (int l = 0; l < kLimbs; ++l) L[l] = ((const int64_t *)g.acc[a])[(uint64_t)l * n + slot];
+          ((double *)o.cols[a])[pos] = fx_to_double(L);
+          break;
+        }
+        default: {  // ACC_MIN_I64 / ACC_MAX_I64 back to the column dtype
+          const int64_t v = ((const int64_t *)g.acc[a])[slot];
+          const int dt = s.col_dtype[s.acc[a].col];
+          if (dt == DT_U64) ((uint64_t *)o.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
+          else if (dt == DT_F64) ((uint64_t *)o.cols[a])[pos] = ordered_to_f64_bits(v);
+          else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)o.cols[a])[pos] = (uint32_t)v;
+          else ((int64_t *)o.cols[a])[pos] = v;
+          break;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// synthetic data (counter-based; integer streams match the host generators)
+// ---------------------------------------------------------------------------
+
+TGA_HD uint64_t splitmix_at(uint64_t seed, uint64_t i) { return mix64(seed + (i + 1) * 0x9e3779b97f4a7c15ULL); }
+TGA_HD uint64_t fastrange32(uint32_t u, uint64_t d) { return (uint64_t)(((unsigned __int128)u * d) >> 32); }
+
+__global__ void k_gen_keys_u32(uint32_t *out, uint64_t n, uint64_t distinct, uint64_t seed, uint64_t salt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = (uint32_t)splitmix_at(seed, i);
+    out[i] = fmix32((uint32_t)fastrange32(u, distinct) ^ (uint32_t)salt);
+  }
+}
+__global__ void k_gen_keys_u64(uint64_t *out, uint64_t n, uint64_t distinct, uint64_t seed, uint64_t salt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = (uint32_t)splitmix_at(seed, i);
+    out[i] = mix64(fastrange32(u, distinct) ^ salt);
+  }
+}
+__global__ void k_gen_vals_i32(int32_t *out, uint64_t n, int64_t lo, uint64_t span, uint64_t seed) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = (uint32_t)(splitmix_at(seed, i) >> 32);
+    out[i] = (int32_t)(lo + (int64_t)fastrange32(u, span));
+  }
+}
+
+// Zipf(s) ranks on [1, D] by rejection-inversion (Hoermann & Derflinger 1996),
+// mapped to distinct 64-bit keys mix64((rank-1) ^ salt).
+struct ZipfConst { double s, hx1, hn, sh; uint64_t D; };
+__device__ __forceinline__ double zh1(double x) { return fabs(x) > 1e-8 ? log1p(x) / x : 1.0 - x * (0.5 - x * (1.0 / 3.0)); }
+__device__ __forceinline__ double zh2(double x) { return fabs(x) > 1e-8 ? expm1(x) / x : 1.0 + x * 0.5 * (1.0 + x * (1.0 / 3.0)); }
+__device__ __forceinline__ double zH(double x, double s) { const double lx = log(x); return zh2((1.0 - s) * lx) * lx; }
+__device__ __forceinline__ double zh(double x, double s) { return exp(-s * log(x)); }
+__device__ __forceinline__ double zHinv(double x, double s) {
+  double t = x * (1.0 - s);
+  if (t < -1.0) t = -1.0;
+  return exp(zh1(t) * x);
+}
+__global__ void k_gen_zipf_u64(uint64_t *out, uint64_t n, ZipfConst z, uint64_t seed, uint64_t salt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = 1;
+    for (int it = 0; it < 64; ++it) {
+      const double u01 = (double)(splitmix_at(seed, i * 64 + it) >> 11) * 0x1.0p-53;
+      const double u = z.hn + u01 * (z.hx1 - z.hn);
+      const double x = zHinv(u, z.s);
+      double kx = floor(x + 0.5);
+      if (kx < 1.0) kx = 1.0;
+      if (kx > (double)z.D) kx = (double)z.D;
+      k = (uint64_t)kx;
+      if (kx - x <= z.sh || u >= zH(kx + 0.5, z.s) - zh(kx, z.s)) break;
+    }
+    out[i] = mix64((k - 1) ^ salt);
+  }
+}
+__global__ void k_gen_normal_f64(double *out, uint64_t n, double mean, double sigma, uint64_t seed) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const double u1 = ((double)(splitmix_at(seed, 2 * i) >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+    const double u2 = (double)(splitmix_at(seed, 2 * i + 1) >> 11) * 0x1.0p-53;
+    out[i] = mean + sigma * sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+  }
+}
+
+// strided key sample for the cardinality estimate
+template <typename K>
+__global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (uint64_t)keys[(uint64_t)(((unsigned __int128)i * n) / m)];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 16384) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st) {
+  const uint64_t n = (g.S + 1) * kLimbs;
+  if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, g, s);
+  else hipLaunchKernelGGL(k_gtable_init<uint64_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, g, s);
+  return hipGetLastError();
+}
+
+template <typename K>
+static hipError_t set_lds_limit(const void *fn, size_t bytes) {
+  (void)sizeof(K);
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
+                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st) {
+  const size_t lds = L.bytes;
+  hipError_t e;
+  if (key_bytes == 4) {
+    e = set_lds_limit<uint32_t>((const void *)&k_agg<uint32_t>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_agg<uint32_t>, dim3(grid), dim3(block), lds, st, (const uint32_t *)keys, s, plan, L, g,
+                       nbits, max_probe);
+  } else {
+    e = set_lds_limit<uint64_t>((const void *)&k_agg<uint64_t>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_agg<uint64_t>, dim3(grid), dim3(block), lds, st, (const uint64_t *)keys, s, plan, L, g,
+                       nbits, max_probe);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan_direct, dim3(1), dim3(64), 0, st, plan, n, nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
+                              const AggPlan &plan, hipStream_t st) {
+  const size_t lds = (size_t)(P + 1) * 4;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)&k_plan_chunks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, offs, P, ntiles, n, chmax, plan);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
+                             uint32_t ntiles, uint32_t *cnt, hipStream_t st) {
+  const size_t lds = (size_t)4 << pbits;
+  if (key_bytes == 4)
+    hipLaunchKernelGGL(k_part_count<uint32_t>, dim3(ntiles), dim3(512), lds, st, (const uint32_t *)keys, n, pbits,
+                       tile_rows, ntiles, cnt);
+  else
+    hipLaunchKernelGGL(k_part_count<uint64_t>, dim3(ntiles), dim3(512), lds, st, (const uint64_t *)keys, n, pbits,
+                       tile_rows, ntiles, cnt);
+  return hipGetLastError();
+}
+
+uint64_t scan_tiles(uint64_t n) { return (n + kScanThreads * kScanItems - 1) / (kScanThreads * kScanItems); }
+
+hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
+                           hipStream_t st) {
+  const uint64_t tiles = scan_tiles(n);
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, in, out, n, desc, ticket);
+  return hipGetLastError();
+}
+
+size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows) {
+  size_t b = (size_t)8 << pbits;  // hist + goff
+  size_t row = (size_t)key_bytes + 2;
+  for (int c = 0; c < s.ncols; ++c) row += (size_t)dtype_bytes(s.col_dtype[c]);
+  return b + row * tile_rows;
+}
+
+hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
+                               const PartOut &o, hipStream_t st) {
+  constexpr int ITEMS = 8;
+  const unsigned block = tile_rows / ITEMS;
+  const size_t lds = scatter_lds_bytes(s, key_bytes, pbits, tile_rows);
+  hipError_t e;
+  if (key_bytes == 4) {
+    e = set_lds_limit<uint32_t>((const void *)&k_part_scatter<uint32_t, ITEMS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_part_scatter<uint32_t, ITEMS>), dim3(ntiles), dim3(block), lds, st,
+                       (const uint32_t *)keys, s, n, pbits, ntiles, offs, (uint32_t *)okeys, o);
+  } else {
+    e = set_lds_limit<uint64_t>((const void *)&k_part_scatter<uint64_t, ITEMS>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_part_scatter<uint64_t, ITEMS>), dim3(ntiles), dim3(block), lds, st,
+                       (const uint64_t *)keys, s, n, pbits, ntiles, offs, (uint64_t *)okeys, o);
+  }
+  return hipGetLastError();
+}
+
+uint64_t compact_tiles(uint64_t slots_plus_one) {
+  const uint64_t TS = (uint64_t)kCompactThreads * kCompactItems;
+  return (slots_plus_one + TS - 1) / TS;
+}
+
+hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
+                          uint32_t *ticket, uint64_t *count_out, hipStream_t st) {
+  const uint64_t tiles = compact_tiles(g.S + 1);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL(k_compact<uint32_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
+                       count_out);
+  else
+    hipLaunchKernelGGL(k_compact<uint64_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
+                       count_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
+                              hipStream_t st) {
+  if (key_bytes == 4)
+    hipLaunchKernelGGL(k_sample_keys<uint32_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint32_t *)keys, n, m, out);
+  else
+    hipLaunchKernelGGL(k_sample_keys<uint64_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint64_t *)keys, n, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,
+                      uint64_t salt, hipStream_t st) {
+  const unsigned grid = grid_for(n, 256, 8192);
+  switch (kind) {
+    case GEN_KEYS_U32: hipLaunchKernelGGL(k_gen_keys_u32, dim3(grid), dim3(256), 0, st, (uint32_t *)out, n, a, seed, salt); break;
+    case GEN_KEYS_U64: hipLaunchKernelGGL(k_gen_keys_u64, dim3(grid), dim3(256), 0, st, (uint64_t *)out, n, a, seed, salt); break;
+    case GEN_VALS_I32: hipLaunchKernelGGL(k_gen_vals_i32, dim3(grid), dim3(256), 0, st, (int32_t *)out, n, (int64_t)a, b, seed); break;
+    case GEN_ZIPF_U64: {
+      ZipfConst z;
+      z.s = x; z.D = a;
+      // host-side constants of the rejection-inversion sampler
+      auto hh1 = [](double v) { return std::fabs(v) > 1e-8 ? std::log1p(v) / v : 1.0 - v * (0.5 - v * (1.0 / 3.0)); };
+      auto hh2 = [](double v) { return std::fabs(v) > 1e-8 ? std::expm1(v) / v : 1.0 + v * 0.5 * (1.0 + v * (1.0 / 3.0)); };
+      auto H = [&](double v) { const double lv = std::log(v); return hh2((1.0 - x) * lv) * lv; };
+      auto hfn = [&](double v) { return std::exp(-x * std::log(v)); };
+      auto Hinv = [&](double v) { double t = v * (1.0 - x); if (t < -1.0) t = -1.0; return std::exp(hh1(t) * v); };
+      z.hx1 = H(1.5) - 1.0;
+      z.hn = H((double)a + 0.5);
+      z.sh = 2.0 - Hinv(H(2.5) - hfn(2.0));
+      hipLaunchKernelGGL(k_gen_zipf_u64, dim3(grid), dim3(256), 0, st, (uint64_t *)out, n, z, seed, salt);
+      break;
+    }
+    case GEN_NORMAL_F64: hipLaunchKernelGGL(k_gen_normal_f64, dim3(grid), dim3(256), 0, st, (double *)out, n, x, y, seed); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tga

@@ -1,0 +1,163 @@
+"""GPU parity of the plain operator against the CPU oracle (bit-exact for
+integers; fp64 SUM within 1e-12 relative of the compensated oracle and equal
+to math.fsum).  Every call goes through the C ABI (libtga.so)."""
+import math
+
+import numpy as np
+import pytest
+
+import cpu_gpu_coprocess_groupby_agg_amd as tga
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+AGGS4 = ["count", ("sum", 0), ("min", 0), ("max", 0)]
+
+
+def check_equal(got, ref):
+    assert got["n_groups"] == ref["n_groups"]
+    np.testing.assert_array_equal(got["keys"], ref["keys"])
+    for g, r in zip(got["aggs"], ref["aggs"]):
+        np.testing.assert_array_equal(g, r)
+
+
+def test_generators_match_oracle(ctx):
+    n = 100_003
+    k32 = ctx.generate(tga.GEN_KEYS_U32, n, a=1 << 10, seed=11, salt=5).to_numpy()
+    k64 = ctx.generate(tga.GEN_KEYS_U64, n, a=1 << 20, seed=12, salt=6).to_numpy()
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=13).to_numpy()
+    np.testing.assert_array_equal(k32, O.gen_keys_u32(n, 1 << 10, 11, 5))
+    np.testing.assert_array_equal(k64, O.gen_keys_u64(n, 1 << 20, 12, 6))
+    np.testing.assert_array_equal(v, O.gen_vals_i32(n, -(1 << 30), 1 << 30, 13))
+
+
+@pytest.mark.parametrize("path", [tga.PATH_AUTO, tga.PATH_DIRECT, tga.PATH_PARTITIONED])
+def test_u32_keys_count_sum(ctx, path):
+    n, d = 1 << 20, 1 << 10
+    k = ctx.generate(tga.GEN_KEYS_U32, n, a=d, seed=1, salt=3)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 20)) & (2**64 - 1), b=1 << 21, seed=2)
+    got = ctx.groupby(k, [v], ["count", ("sum", 0)], path=path)
+    ref = O.groupby(k.to_numpy(), [v.to_numpy()], ["count", ("sum", 0)])
+    check_equal(got, ref)
+    assert got["n_groups"] == d
+
+
+@pytest.mark.parametrize("path", [tga.PATH_AUTO, tga.PATH_PARTITIONED, tga.PATH_DIRECT])
+def test_i64_keys_four_aggs(ctx, path):
+    n, d = 1 << 22, 1 << 16
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=5, salt=9)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=6)
+    kh = k.to_numpy().view(np.int64)
+    vh = v.to_numpy()
+    got = ctx.groupby(k, [v], AGGS4, path=path)
+    ref = O.groupby(kh.view(np.uint64), [vh], AGGS4)
+    check_equal(got, ref)
+
+
+def test_partitioned_small_tables_many_partitions(ctx):
+    n, d = 1 << 21, 200_000
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=15, salt=1)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=16)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, partition_bits=10, table_slots=1024)
+    ref = O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4)
+    check_equal(got, ref)
+
+
+def test_edge_empty_and_single(ctx):
+    k = np.zeros(0, np.uint32)
+    v = np.zeros(0, np.int32)
+    got = ctx.groupby(k, [v], ["count", ("sum", 0)])
+    assert got["n_groups"] == 0
+    got = ctx.groupby(np.array([7], np.uint32), [np.array([-3], np.int32)], AGGS4)
+    assert got["n_groups"] == 1 and got["aggs"][0][0] == 1 and got["aggs"][1][0] == -3
+    assert got["aggs"][2][0] == -3 and got["aggs"][3][0] == -3
+
+
+@pytest.mark.parametrize("path", [tga.PATH_DIRECT, tga.PATH_PARTITIONED])
+def test_single_key_max_contention(ctx, path):
+    n = 1 << 20
+    k = np.full(n, 42, np.uint64)
+    v = O.gen_vals_i32(n, -(1 << 31), (1 << 31) - 1, 3)
+    got = ctx.groupby(k, [v], AGGS4, path=path)
+    ref = O.groupby(k, [v], AGGS4)
+    check_equal(got, ref)
+
+
+def test_all_distinct_and_sentinel_keys(ctx):
+    n = 300_000
+    k = np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    k[:5] = np.uint64(2**64 - 1)  # the empty-slot marker as a real key
+    k[5] = 0
+    v = O.gen_vals_i32(n, -(1 << 31), (1 << 31) - 1, 4)
+    v[:3] = np.iinfo(np.int32).min
+    v[3] = np.iinfo(np.int32).max
+    for path in (tga.PATH_DIRECT, tga.PATH_PARTITIONED):
+        got = ctx.groupby(k, [v], AGGS4, path=path)
+        ref = O.groupby(k, [v], AGGS4)
+        check_equal(got, ref)
+    k32 = (np.arange(n, dtype=np.uint64) % 1000).astype(np.uint32)
+    k32[::7] = 0xFFFFFFFF
+    got = ctx.groupby(k32, [v], AGGS4)
+    check_equal(got, O.groupby(k32, [v], AGGS4))
+
+
+def test_global_table_overflow_retry(ctx):
+    n, d = 1 << 20, 100_000
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=21, salt=2)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1000, seed=22)
+    got = ctx.groupby(k, [v], AGGS4, est_groups=100)  # far too low: table must grow
+    check_equal(got, O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4))
+
+
+def test_lds_spill_path(ctx):
+    # more groups than the forced LDS table holds: rows spill to the global table
+    n, d = 1 << 20, 20_000
+    k = ctx.generate(tga.GEN_KEYS_U32, n, a=d, seed=31, salt=2)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=32)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_DIRECT, table_slots=256, est_groups=d)
+    check_equal(got, O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4))
+
+
+def test_multi_column_dtypes(ctx):
+    n = 200_000
+    k = O.gen_keys_u32(n, 5000, 41, 0)
+    a = O.gen_vals_i32(n, -(1 << 31), (1 << 31) - 1, 42)
+    b = O.gen_keys_u64(n, 1 << 40, 43, 7)            # u64 column
+    c = b.view(np.int64).copy()                        # i64 column
+    d = a.view(np.uint32).copy()                       # u32 column
+    aggs = [("sum", 0), ("min", 1), ("max", 1), ("min", 2), ("max", 2), ("sum", 3), ("max", 3), "count"]
+    got = ctx.groupby(k, [a, b, c, d], aggs)
+    check_equal(got, O.groupby(k, [a, b, c, d], aggs))
+
+
+def test_fp64_sum_exact(ctx):
+    n, d = 1 << 20, 3000
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=51, salt=3)
+    x = ctx.generate(tga.GEN_NORMAL_F64, n, x=0.0, y=1e3, seed=52)
+    kh, xh = k.to_numpy(), x.to_numpy()
+    got = ctx.groupby(k, [x], ["count", ("sum", 0), ("min", 0), ("max", 0)])
+    ref = O.groupby(kh, [xh], ["count", ("sum", 0), ("min", 0), ("max", 0)])
+    np.testing.assert_array_equal(got["keys"], ref["keys"])
+    np.testing.assert_array_equal(got["aggs"][0], ref["aggs"][0])
+    np.testing.assert_array_equal(got["aggs"][2], ref["aggs"][2])
+    np.testing.assert_array_equal(got["aggs"][3], ref["aggs"][3])
+    s, r = got["aggs"][1], ref["aggs"][1]
+    # tolerance of the compensated oracle: 1e-12 relative
+    assert np.all(np.abs(s - r) <= 1e-12 * np.maximum(np.abs(r), 1e-300))
+    # the GPU sum is the exact sum rounded once: equal to math.fsum
+    order = np.argsort(kh, kind="stable")
+    ks = kh[order]
+    xs = xh[order]
+    starts = np.r_[0, np.flatnonzero(np.diff(ks)) + 1, ks.size]
+    for i in range(0, len(starts) - 1, 97):
+        assert s[i] == math.fsum(xs[starts[i]:starts[i + 1]])
+
+
+def test_scan_matches_prefix_sum(ctx):
+    for n in (1, 4095, 4096, 4097, 1_000_003):
+        x = (np.arange(n, dtype=np.uint64) * 2654435761 % 1000).astype(np.uint32)
+        d_in = ctx.upload(x)
+        d_out = ctx.empty(n, tga.U32)
+        ctx.scan_exclusive_u32(d_in, d_out)
+        exp = (O.scan(x.astype(np.uint64)) & 0xFFFFFFFF).astype(np.uint32)
+        np.testing.assert_array_equal(d_out.to_numpy(), exp)
