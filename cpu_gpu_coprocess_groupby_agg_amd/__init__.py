@@ -130,6 +130,17 @@ def lib():
             "tga_server_wait": [vp, i64, P(vp)],
             "tga_server_stats_get": [vp, P(ServerStats)],
             "tga_server_destroy": [vp],
+            "tga_server_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double)],
+            "tga_sequential_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double)],
+            "tga_compat_partition": [vp, vp, vp, sz, sz, sz, sz, P(vp)],
+            "tga_partset_count": [vp, P(sz)],
+            "tga_partset_info": [vp, sz, P(sz), P(u64)],
+            "tga_partset_segment": [vp, sz, sz, P(vp), P(vp), P(ctypes.c_uint32)],
+            "tga_partset_create": [vp, sz, P(vp)],
+            "tga_partset_push": [vp, sz, vp, vp, ctypes.c_uint32],
+            "tga_partset_destroy": [vp],
+            "tga_compat_intra_partition": [vp, vp, sz, sz, sz, sz, P(vp), P(vp), P(sz), P(sz), P(sz)],
+            "tga_host_free": [vp],
         }
         for name, argt in list(sigs.items()) + list(optional.items()):
             fn = getattr(L, name, None)

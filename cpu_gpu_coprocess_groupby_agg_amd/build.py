@@ -60,7 +60,26 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER = os.path.join(HERE, "tga_compat_driver")
+
+
+def build_driver():
+    """The reference driver's call sequence against include/tga_compat.hpp
+    (tests/compat_driver.cpp), linked to libtga.so; used by the GPU tests."""
+    src = os.path.join(ROOT, "tests", "compat_driver.cpp")
+    if not os.path.exists(src):
+        return None
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", DRIVER + ".tmp",
+           "-L", HERE, "-ltga", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compat driver build failed\n" + r.stdout + r.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -1,0 +1,152 @@
+"""GPU tests of the reference's two phase entry points (u32 keys and values,
+u32 wrap-around SUM) through the C ABI and through the C++ compat header."""
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cpu_gpu_coprocess_groupby_agg_amd as tga
+import oracle as O
+from cpu_gpu_coprocess_groupby_agg_amd import lib
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "cpu_gpu_coprocess_groupby_agg_amd", "tga_compat_driver")
+GOLDEN = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "refgen_*.npz")))
+
+
+def run_phases(ctx, keys, vals, tile_len, P, capacity, min_load, max_load):
+    k = keys.copy()
+    v = vals.copy()
+    ps = ctypes.c_void_p()
+    ctx._check(lib().tga_compat_partition(ctx.h, k.ctypes.data, v.ctypes.data, k.size, tile_len, P, 4,
+                                          ctypes.byref(ps)))
+    # phase-1 invariants: per partition, segments point into the rewritten buffers
+    nparts = ctypes.c_size_t()
+    ctx._check(lib().tga_partset_count(ps, ctypes.byref(nparts)))
+    assert nparts.value == P
+    base = k.ctypes.data
+    total = 0
+    for p in range(P):
+        ns = ctypes.c_size_t()
+        rows = ctypes.c_uint64()
+        ctx._check(lib().tga_partset_info(ps, p, ctypes.byref(ns), ctypes.byref(rows)))
+        total += rows.value
+        for j in range(ns.value):
+            kp, vp_, ln = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_uint32()
+            ctx._check(lib().tga_partset_segment(ps, p, j, ctypes.byref(kp), ctypes.byref(vp_), ctypes.byref(ln)))
+            if ln.value:
+                off = (kp.value - base) // 4
+                assert 0 <= off and off + ln.value <= k.size
+    begin = (ctypes.c_size_t * P)()
+    num = (ctypes.c_size_t * P)()
+    nm = ctypes.c_size_t()
+    ok, ov = ctypes.c_void_p(), ctypes.c_void_p()
+    st = lib().tga_compat_intra_partition(ctx.h, ps, capacity, min_load, max_load, 4, ctypes.byref(ok),
+                                          ctypes.byref(ov), begin, num, ctypes.byref(nm))
+    lib().tga_partset_destroy(ps)
+    if st != 0:
+        return st, None
+    out_k, out_v = [], []
+    tot = sum(num[i] for i in range(nm.value))
+    K = np.ctypeslib.as_array(ctypes.cast(ok, ctypes.POINTER(ctypes.c_uint32)), shape=(max(total, 1),))
+    V = np.ctypeslib.as_array(ctypes.cast(ov, ctypes.POINTER(ctypes.c_uint32)), shape=(max(total, 1),))
+    for i in range(nm.value):
+        out_k.append(K[begin[i]:begin[i] + num[i]].copy())
+        out_v.append(V[begin[i]:begin[i] + num[i]].copy())
+    lib().tga_host_free(ok)
+    lib().tga_host_free(ov)
+    keys_o = np.concatenate(out_k) if out_k else np.zeros(0, np.uint32)
+    vals_o = np.concatenate(out_v) if out_v else np.zeros(0, np.uint32)
+    o = np.argsort(keys_o, kind="stable")
+    assert keys_o.size == tot
+    return 0, (keys_o[o], vals_o[o], total, nm.value)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=os.path.basename)
+def test_compat_phases_on_reference_workload(ctx, path):
+    z = np.load(path, allow_pickle=False)
+    keys, vals = z["keys"], z["vals"]
+    tile = max(1000, keys.size // 3)
+    st, (k, v, p1rows, nm) = run_phases(ctx, keys, vals, tile, 17, tile, tile * 3 // 5, tile * 4 // 5)
+    assert st == 0
+    assert k.size == int(z["cardinality"])          # the reference's printed observable
+    np.testing.assert_array_equal(k, z["out_keys"])
+    np.testing.assert_array_equal(v, z["out_sum_u32"])
+
+
+def test_compat_phases_large_and_sentinel(ctx):
+    n = 3_000_000
+    keys = O.gen_keys_u32(n, 200_000, 77, 1)
+    keys[::1000] = 0xFFFFFFFF  # the reference's sentinel value is a real key here
+    vals = O.gen_vals_i32(n, 0, 1 << 31, 78).view(np.uint32).copy()
+    tile = 1_000_000
+    st, (k, v, _, _) = run_phases(ctx, keys, vals, tile, 300, tile, tile * 3 // 5, tile * 4 // 5)
+    assert st == 0
+    ref = O.groupby(keys, [vals], ["count", ("sum", 0)])
+    np.testing.assert_array_equal(k, ref["keys"])
+    np.testing.assert_array_equal(v, (ref["aggs"][1] & 0xFFFFFFFF).astype(np.uint32))
+
+
+def test_compat_overflow_rule(ctx):
+    # every key distinct, one partition, small capacity: the reference prints
+    # "hash table overflow" and exits; the C ABI returns TGA_ERR_TABLE_OVERFLOW
+    n = 40_000
+    keys = np.arange(n, dtype=np.uint32)
+    vals = np.ones(n, np.uint32)
+    st, _ = run_phases(ctx, keys, vals, 40_000, 1, 10_000, 6_000, 8_000)
+    assert st == tga.ERR_TABLE_OVERFLOW
+    st, (k, v, _, _) = run_phases(ctx, keys, vals, 40_000, 1, 100_000, 60_000, 80_000)
+    assert st == 0 and k.size == n
+
+
+def test_partset_from_caller_segments(ctx):
+    keys = O.gen_keys_u32(50_000, 700, 5, 2)
+    vals = O.gen_vals_i32(50_000, 0, 1000, 6).view(np.uint32).copy()
+    ps = ctypes.c_void_p()
+    ctx._check(lib().tga_partset_create(ctx.h, 3, ctypes.byref(ps)))
+    for j, (a, b) in enumerate([(0, 20_000), (20_000, 20_000), (20_000, 50_000)]):
+        ctx._check(lib().tga_partset_push(ps, j % 3, keys.ctypes.data + 4 * a, vals.ctypes.data + 4 * a, b - a))
+    begin = (ctypes.c_size_t * 3)()
+    num = (ctypes.c_size_t * 3)()
+    nm = ctypes.c_size_t()
+    ok, ov = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx._check(lib().tga_compat_intra_partition(ctx.h, ps, 100_000, 60_000, 80_000, 1, ctypes.byref(ok),
+                                                ctypes.byref(ov), begin, num, ctypes.byref(nm)))
+    lib().tga_partset_destroy(ps)
+    # all three partitions merge into one (sizes add up to 50000 <= K)
+    assert nm.value == 1 and num[0] == 700
+    K = np.ctypeslib.as_array(ctypes.cast(ok, ctypes.POINTER(ctypes.c_uint32)), shape=(50_000,))[:700].copy()
+    V = np.ctypeslib.as_array(ctypes.cast(ov, ctypes.POINTER(ctypes.c_uint32)), shape=(50_000,))[:700].copy()
+    lib().tga_host_free(ok)
+    lib().tga_host_free(ov)
+    o = np.argsort(K)
+    ref = O.groupby(keys, [vals], [("sum", 0)])
+    np.testing.assert_array_equal(K[o], ref["keys"])
+    np.testing.assert_array_equal(V[o], (ref["aggs"][0] & 0xFFFFFFFF).astype(np.uint32))
+
+
+def test_cpp_compat_header_driver(ctx, tmp_path):
+    """The reference call sequence (test/test.cu:134-161) compiled against
+    include/tga_compat.hpp: phase 1, phase 2, result kv num."""
+    assert os.path.exists(DRIVER), "build() compiles the compat driver"
+    n = 200_000
+    keys = O.gen_keys_u32(n, 20_000, 9, 4)
+    vals = O.gen_vals_i32(n, 0, 1 << 31, 10).view(np.uint32).copy()
+    inp = tmp_path / "in.bin"
+    out = tmp_path / "out.bin"
+    np.concatenate([keys, vals]).tofile(inp)
+    r = subprocess.run([DRIVER, str(inp), str(out), str(n), "50000", "300"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "result kv num: 20000" in r.stdout
+    raw = np.fromfile(out, dtype=np.uint32)
+    g = raw.size // 2
+    k, v = raw[:g], raw[g:]
+    o = np.argsort(k)
+    ref = O.groupby(keys, [vals], [("sum", 0)])
+    np.testing.assert_array_equal(k[o], ref["keys"])
+    np.testing.assert_array_equal(v[o], (ref["aggs"][0] & 0xFFFFFFFF).astype(np.uint32))

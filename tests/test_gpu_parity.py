@@ -161,3 +161,28 @@ def test_scan_matches_prefix_sum(ctx):
         ctx.scan_exclusive_u32(d_in, d_out)
         exp = (O.scan(x.astype(np.uint64)) & 0xFFFFFFFF).astype(np.uint32)
         np.testing.assert_array_equal(d_out.to_numpy(), exp)
+
+
+@pytest.mark.parametrize("path", [tga.PATH_AUTO, tga.PATH_DIRECT, tga.PATH_PARTITIONED])
+def test_zipf_hot_keys_int_and_fp64(ctx, path):
+    """Config-3 shape at reduced size: Zipf(1.1) keys (hot-key contention),
+    an int32 column (SUM bit-exact) and an fp64 N(0, 1e3) column (SUM within
+    1e-12 of the compensated oracle, equal to math.fsum)."""
+    n, D = 1 << 22, 1 << 18
+    k = ctx.generate(tga.GEN_ZIPF_U64, n, a=D, x=1.1, seed=61, salt=7)
+    a = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=62)
+    f = ctx.generate(tga.GEN_NORMAL_F64, n, x=0.0, y=1e3, seed=63)
+    kh, ah, fh = k.to_numpy(), a.to_numpy(), f.to_numpy()
+    aggs = ["count", ("sum", 0), ("sum", 1)]
+    got = ctx.groupby(k, [a, f], aggs, path=path)
+    ref = O.groupby(kh, [ah, fh], aggs)
+    assert got["n_groups"] == ref["n_groups"]
+    np.testing.assert_array_equal(got["keys"], ref["keys"])
+    np.testing.assert_array_equal(got["aggs"][0], ref["aggs"][0])
+    np.testing.assert_array_equal(got["aggs"][1], ref["aggs"][1])
+    s, r = got["aggs"][2], ref["aggs"][2]
+    assert np.all(np.abs(s - r) <= 1e-12 * np.maximum(np.abs(r), 1e-300))
+    # the hottest key holds a large share of the rows: its fp64 sum is exact
+    top = int(np.argmax(got["aggs"][0]))
+    assert got["aggs"][0][top] > n // 20
+    assert s[top] == math.fsum(fh[kh == got["keys"][top]].tolist())
