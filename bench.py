@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--only", choices=["all", "headline", "copy"], default="all",
+                    help="headline: only the 2^28-row group-by (profiling runs); copy: only the copy reference")
     return ap.parse_args()
 
 
@@ -109,6 +111,23 @@ def headline(ctx, tga, args):
         "value": n / (ms * 1e-3),
     }
     return out, keys, vals
+
+
+def copy_bandwidth(ctx, tga, nbytes=1 << 31, reps=5):
+    """Device copy of nbytes (read + write), HIP-event timed: the measured
+    streaming ceiling beside the 8 TB/s specification."""
+    a = ctx.empty(nbytes // 8, tga.U64)
+    b = ctx.empty(nbytes // 8, tga.U64)
+    ctx.copy(b, a, nbytes)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(reps):
+        ctx.copy(b, a, nbytes)
+    n, ms = ctx.profile_entries()["copy16"]
+    ctx.profile(False)
+    a.free()
+    b.free()
+    return {"bytes_moved": 2 * nbytes, "avg_ms": round(ms / n, 4), "gbs": round(2 * nbytes / (ms / n * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(ctx, keys, vals):
@@ -283,9 +302,16 @@ def main():
     ctx = tga.Context(0)
     info = ctx.device_info()
     log(f"device {info}")
+    copy = copy_bandwidth(ctx, tga)
+    log(f"copy reference: {copy}")
+    if args.only == "copy":
+        print(json.dumps({"copy": copy}))
+        return
     h, keys, vals = headline(ctx, tga, args)
     log(f"headline: {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s, kernels {h['kernels_ms_per_step']}")
     cpu = None
+    if args.only == "headline":
+        args.no_cpu = args.no_batch = args.no_serve = True
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, keys, vals)
         log(f"cpu baseline: {cpu}")
@@ -320,6 +346,7 @@ def main():
         "op_frac_of_hbm_peak": h["op_frac_of_peak"],
         "kernels_ms_per_step": {k: round(v, 4) for k, v in h["kernels_ms_per_step"].items()},
         "cpu_baseline": cpu,
+        "copy_reference": copy,
         "device": info,
     }
     line.update(extra)

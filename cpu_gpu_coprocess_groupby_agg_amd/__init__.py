@@ -114,6 +114,7 @@ def lib():
             "tga_memcpy_h2d": [vp, vp, vp, u64],
             "tga_memcpy_d2h": [vp, vp, vp, u64],
             "tga_synchronize": [vp],
+            "tga_device_copy": [vp, vp, vp, u64],
             "tga_profile_enable": [vp, i32],
             "tga_profile_reset": [vp],
             "tga_profile_entry": [vp, i32, ctypes.c_char_p, sz, P(i64), P(ctypes.c_double)],
@@ -257,6 +258,9 @@ class Context:
         d = DeviceArray(self, n, dt)
         self._check(lib().tga_generate(self.h, kind, d.ptr, n, a & (2**64 - 1), b, x, y, seed, salt))
         return d
+
+    def copy(self, dst, src, nbytes):
+        self._check(lib().tga_device_copy(self.h, dst.ptr, src.ptr, nbytes))
 
     def synchronize(self):
         self._check(lib().tga_synchronize(self.h))

@@ -363,6 +363,19 @@ hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable
 
 static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 
+// Plan buffer layout of the partitioned path: pstart[P+1] u64 | cstart[P+1]
+// u32 | ticket u32 | split_rows u64 (8-byte aligned).
+static AggPlan part_plan(const DBuf &buf, uint32_t P) {
+  char *b = buf.as<char>();
+  AggPlan ap{};
+  ap.pstart = (uint64_t *)b;
+  ap.cstart = (uint32_t *)(b + (uint64_t)(P + 1) * 8);
+  ap.P = P;
+  ap.ticket = ap.cstart + (P + 1);
+  ap.split_rows = (uint64_t *)(b + (((uint64_t)(P + 1) * 12 + 4 + 7) & ~7ULL));
+  return ap;
+}
+
 // Aggregates rows [0, n) of device-resident columns into the global table g
 // (direct or partitioned path).  Temporaries are appended to `keep` and must
 // outlive the work queued on the context stream.
@@ -373,7 +386,8 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
   DBuf planbuf;  // pstart[P+1] u64 | cstart[P+1] u32 | ticket
   if (n > 0 && plan.path == TGA_PATH_DIRECT) {
     if (!planbuf.alloc(ctx, 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
-    AggPlan ap{planbuf.as<uint64_t>(), (uint32_t *)(planbuf.as<char>() + 16), 1, (uint32_t *)(planbuf.as<char>() + 32)};
+    AggPlan ap{planbuf.as<uint64_t>(), (uint32_t *)(planbuf.as<char>() + 16), 1, (uint32_t *)(planbuf.as<char>() + 32),
+               (uint64_t *)(planbuf.as<char>() + 40)};
     const uint64_t nch = (n + plan.chunk_rows - 1) / plan.chunk_rows;
     TGA_HIP(ctx, launch_plan_direct(ap, n, (uint32_t)nch, st));
     const unsigned grid = (unsigned)std::min<uint64_t>(plan.workgroups, nch);
@@ -464,11 +478,145 @@ tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, c
   return TGA_OK;
 }
 
+// Partitioned path with direct emission (n < 2^31): partitions that one work
+// item holds whole write final rows straight to the output; only split
+// partitions and LDS spills go through the global table, sized for them.
+static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, Plan &plan,
+                                  bool full_table, PlainOut *out, void *ext_keys, void *const *ext_cols,
+                                  uint64_t ext_cap, uint64_t *groups, bool *overflow) {
+  hipStream_t st = ctx->stream;
+  const Schema &s = q.schema;
+  const uint32_t P = 1u << plan.pbits;
+  const uint32_t ntiles = (uint32_t)((n + plan.tile_rows - 1) / plan.tile_rows);
+  const uint64_t cells = (uint64_t)P * ntiles;
+  DBuf cnt, offs, desc, pkeys, planbuf, counters;
+  DBuf pcols[kMaxCols];
+  if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
+      !pkeys.alloc(ctx, n * q.key_bytes) || !planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64) ||
+      !counters.alloc(ctx, 64))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  PartOut po{};
+  for (int c = 0; c < s.ncols; ++c) {
+    if (!pcols[c].alloc(ctx, n * (uint64_t)dtype_bytes(s.col_dtype[c])))
+      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+    po.cols[c] = pcols[c].p;
+  }
+  const uint64_t dwords = scan_tiles(cells) + 1;
+  uint32_t *ticket = (uint32_t *)(desc.as<uint64_t>() + dwords);
+  uint64_t *emit_count = counters.as<uint64_t>();
+  TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
+  TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
+  TGA_HIP(ctx, timed(ctx, "part_count", [&] {
+    return launch_part_count(q.key_bytes, d_keys, n, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "scan", [&] {
+    return launch_scan_u32(cnt.as<uint32_t>(), offs.as<uint32_t>(), cells, desc.as<uint64_t>(), ticket, st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+    return launch_part_scatter(q.key_bytes, d_keys, s, n, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
+                               pkeys.p, po, st);
+  }));
+  AggPlan ap = part_plan(planbuf, P);
+  TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
+    return launch_plan_chunks(offs.as<uint32_t>(), P, ntiles, n, plan.chunk_rows, ap, st);
+  }));
+  uint64_t split_rows = 0;
+  TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipStreamSynchronize(st));
+  // global table: split partitions and spills only
+  uint64_t S = plan.gslots;
+  if (!full_table) {
+    uint64_t want = 4096;
+    while (want < 2 * split_rows && want < S) want <<= 1;
+    S = std::min(S, want);
+  }
+  GTable g{};
+  std::vector<DBuf> gbufs;
+  DBuf flags;
+  if (gtable_alloc(ctx, q, S, g, gbufs, flags) != hipSuccess)
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table of %llu slots", (unsigned long long)S);
+  TGA_HIP(ctx, timed(ctx, "gtable_init", [&] { return launch_gtable_init(q.key_bytes, g, s, st); }));
+  // outputs
+  const uint64_t cap = ext_keys ? ext_cap : plan.gslots + 1;
+  PlainOut tmp;
+  OutDesc od{};
+  od.capacity = cap;
+  if (ext_keys) {
+    od.keys = ext_keys;
+    for (int a = 0; a < s.nacc; ++a) od.cols[a] = ext_cols[a];
+  } else {
+    if (!tmp.keys.alloc(ctx, std::max<uint64_t>(cap, 1) * q.key_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "output");
+    od.keys = tmp.keys.p;
+    for (int a = 0; a < s.nacc; ++a) {
+      if (!tmp.cols[a].alloc(ctx, std::max<uint64_t>(cap, 1) * (uint64_t)dtype_bytes(q.out_dtype[a])))
+        return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "output");
+      od.cols[a] = tmp.cols[a].p;
+    }
+  }
+  Schema sp = s;
+  for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
+  TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
+    return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
+                      plan.block, st, 1, &od, emit_count);
+  }));
+  // the global table's groups follow the emitted ones
+  const uint64_t ctiles = compact_tiles(g.S + 1);
+  DBuf cdesc;
+  if (!cdesc.alloc(ctx, (ctiles + 2) * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "compaction");
+  TGA_HIP(ctx, launch_fill_u64(cdesc.as<uint64_t>(), ctiles + 2, 0, st));
+  uint64_t *count = cdesc.as<uint64_t>() + ctiles + 1;
+  TGA_HIP(ctx, timed(ctx, "compact", [&] {
+    return launch_compact(q.key_bytes, g, s, od, cdesc.as<uint64_t>(), (uint32_t *)(cdesc.as<uint64_t>() + ctiles),
+                          count, st, emit_count);
+  }));
+  uint32_t hflags[kFlagWords];
+  uint64_t hcount = 0;
+  TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof hflags, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipMemcpyAsync(&hcount, count, 8, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipStreamSynchronize(st));
+  if (ctx->prof.on) ctx->prof.resolve();
+  *overflow = hflags[FLAG_OVERFLOW] != 0;
+  if (hflags[FLAG_OVERFLOW]) {
+    // either the spill table or the output was too small: caller retries
+    if (ext_keys && hcount > ext_cap) {
+      *overflow = false;
+      *groups = hcount;
+      return fail(ctx, TGA_ERR_CAPACITY, "%llu groups > capacity %llu", (unsigned long long)hcount,
+                  (unsigned long long)ext_cap);
+    }
+    return TGA_OK;
+  }
+  if (hflags[FLAG_RANGE]) return fail(ctx, TGA_ERR_RANGE, "fp64 SUM input outside [2^-96, 2^64) or not finite");
+  *groups = hcount;
+  if (out) {
+    out->keys = std::move(tmp.keys);
+    for (int a = 0; a < s.nacc; ++a) out->cols[a] = std::move(tmp.cols[a]);
+    out->groups = hcount;
+    out->capacity = cap;
+  }
+  return TGA_OK;
+}
+
 tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, const Plan &plan_in,
                      PlainOut *out, void *ext_keys, void *const *ext_cols, uint64_t ext_cap, uint64_t *groups) {
   Plan plan = plan_in;
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
+  if (plan.path == TGA_PATH_PARTITIONED && n > 0 && n < (1ULL << 31)) {
+    for (int attempt = 0; attempt < 6; ++attempt) {
+      bool overflow = false;
+      tga_status st2 = run_partitioned(ctx, q, d_keys, n, plan, attempt > 0, out, ext_keys, ext_cols, ext_cap, groups,
+                                       &overflow);
+      if (st2) return st2;
+      if (!overflow) {
+        ctx->last_plan = tga_options{};
+        plan_to_options(plan, ctx->last_plan);
+        return TGA_OK;
+      }
+      plan.gslots *= 4;  // the estimate was too low: larger tables, run again
+    }
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table kept overflowing");
+  }
   for (int attempt = 0; attempt < 6; ++attempt) {
     GTable g{};
     std::vector<DBuf> gbufs;
@@ -799,6 +947,17 @@ TGA_API tga_status tga_memcpy_d2h(tga_context *ctx, void *dst, const void *src, 
   hipSetDevice(ctx->device);
   TGA_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_device_copy(tga_context *ctx, void *dst, const void *src, uint64_t bytes) {
+  if (!ctx || (bytes && (!dst || !src)) || (bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+    return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  TGA_HIP(ctx, timed(ctx, "copy16", [&] { return launch_copy16(src, dst, bytes / 16, ctx->stream); }));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->prof.on) ctx->prof.resolve();
   return TGA_OK;
 }
 

@@ -64,6 +64,13 @@ struct GTable {
   uint32_t *flags;          // kFlagWords status words
 };
 
+// Dense output: keys plus one column per aggregate, `capacity` rows.
+struct OutDesc {
+  void *keys;
+  void *cols[kMaxAccs];
+  uint64_t capacity;
+};
+
 // Byte offsets of the SoA arrays of one table with `slots` entries plus one
 // extra entry (index `slots`) reserved for the key whose bits equal the empty
 // marker, so every key value is supported.

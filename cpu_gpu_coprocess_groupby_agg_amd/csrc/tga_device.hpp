@@ -457,4 +457,76 @@ __device__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g
   }
 }
 
+
+// Writes one LDS table slot (key + every accumulator, converted to the output
+// dtype) to output row `pos`.
+template <typename K>
+__device__ __forceinline__ void emit_row(const LdsTable<K> &t, const Schema &s, const OutDesc &od, uint32_t i, K key,
+                                         uint64_t pos) {
+  const uint32_t n = t.L.slots + 1;
+  ((K *)od.keys)[pos] = key;
+  for (int a = 0; a < s.nacc; ++a) {
+    const int kind = s.acc[a].kind;
+    switch (kind) {
+      case ACC_COUNT: ((uint64_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
+      case ACC_SUM_I: ((uint64_t *)od.cols[a])[pos] = t.template arr<uint64_t>(a)[i]; break;
+      case ACC_MIN_I32: case ACC_MAX_I32: ((uint32_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
+      case ACC_SUM_F: {
+        int64_t L[kLimbs];
+        const int64_t *p = t.template arr<int64_t>(a);
+#pragma unroll
+        for (int l = 0; l < kLimbs; ++l) L[l] = p[(size_t)l * n + i];
+        ((double *)od.cols[a])[pos] = fx_to_double(L);
+        break;
+      }
+      default: {
+        const int64_t v = t.template arr<int64_t>(a)[i];
+        const int dt = s.col_dtype[s.acc[a].col];
+        if (dt == DT_U64) ((uint64_t *)od.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
+        else if (dt == DT_F64) ((uint64_t *)od.cols[a])[pos] = ordered_to_f64_bits(v);
+        else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)od.cols[a])[pos] = (uint32_t)v;
+        else ((int64_t *)od.cols[a])[pos] = v;
+        break;
+      }
+    }
+  }
+}
+
+// Emits every occupied slot of the LDS table as final output rows: a
+// partition processed whole by one workgroup holds final groups (its keys
+// appear in no other partition).  Rows are reserved on *count with one
+// atomic per workgroup.  scratch: blockDim/64 + 1 words; s_base: one word.
+template <typename K>
+__device__ void lds_emit(const LdsTable<K> &t, const Schema &s, const OutDesc &od, uint64_t *count,
+                         uint32_t *flags, uint32_t *scratch, uint64_t *s_base) {
+  const uint32_t S = t.L.slots, n = S + 1;
+  const K *keys = t.keys();
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t b = threadIdx.x * per, e = min(n, b + per);
+  uint32_t local = 0;
+  for (uint32_t i = b; i < e; ++i) local += (i < S) ? (keys[i] != KeyTraits<K>::kEmpty) : (*t.sentinel != 0);
+  const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+  uint32_t x = local;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) scratch[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) { const uint32_t v = scratch[w]; scratch[w] = run; run += v; }
+    *s_base = run ? atomicAdd((unsigned long long *)count, (unsigned long long)run) : 0;
+    if (*s_base + run > od.capacity) atomicOr(&flags[FLAG_OVERFLOW], 1u);
+  }
+  __syncthreads();
+  uint64_t pos = *s_base + scratch[wave] + (x - local);
+  for (uint32_t i = b; i < e; ++i) {
+    const bool occ = (i < S) ? (keys[i] != KeyTraits<K>::kEmpty) : (*t.sentinel != 0);
+    if (!occ) continue;
+    if (pos < od.capacity) emit_row<K>(t, s, od, i, i < S ? keys[i] : KeyTraits<K>::kEmpty, pos);
+    ++pos;
+  }
+}
+
 }  // namespace tga

@@ -15,25 +15,24 @@ struct AggPlan {
   uint32_t *cstart;  // [P+1]
   uint32_t P;
   uint32_t *ticket;
+  uint64_t *split_rows;  // rows of partitions cut into more than one item (k_plan_chunks)
 };
 
 struct PartOut {
   void *cols[kMaxCols];
 };
 
-struct OutDesc {
-  void *keys;
-  void *cols[kMaxAccs];
-  uint64_t capacity;
-};
-
 enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_ZIPF_U64 = 4, GEN_NORMAL_F64 = 5 };
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t st);
 hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st);
+// emit != 0: work items that hold a whole partition write their groups
+// straight to `od` (rows reserved on *emit_count); the others flush into g.
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
-                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st);
+                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
+                      int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
 hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
                               const AggPlan &plan, hipStream_t st);
@@ -47,8 +46,9 @@ hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s,
                                uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
                                const PartOut &o, hipStream_t st);
 uint64_t compact_tiles(uint64_t slots_plus_one);
+// Output rows start at *base when base is non-null (after directly emitted groups).
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
-                          uint32_t *ticket, uint64_t *count_out, hipStream_t st);
+                          uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
                               hipStream_t st);
 hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,

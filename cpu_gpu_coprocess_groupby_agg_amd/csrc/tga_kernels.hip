@@ -32,6 +32,12 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Plain 16-byte streaming copy (bandwidth reference for the counters).
+__global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 template <typename K>
 __global__ void k_gtable_init(GTable g, Schema s) {
   const uint64_t n = g.S + 1;
@@ -66,9 +72,12 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 // are [pstart[p], pstart[p+1]).
 template <typename K>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
-                                              GTable g, int nbits, int max_probe) {
+                                              GTable g, int nbits, int max_probe, int emit, OutDesc od,
+                                              uint64_t *emit_count) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t s_item, s_sentinel;
+  __shared__ uint32_t scratch[1024 / kWave + 1];
+  __shared__ uint64_t s_base;
   LdsTable<K> t{smem, L, &s_sentinel};
   const uint32_t total = plan.cstart[plan.P];
   for (;;) {
@@ -89,7 +98,8 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     __syncthreads();
     agg_rows<K>(keys, s, b, e, t, g, nbits, max_probe);
     __syncthreads();
-    lds_flush<K>(t, s, g);
+    if (emit && nch == 1) lds_emit<K>(t, s, od, emit_count, g.flags, scratch, &s_base);
+    else lds_flush<K>(t, s, g);
     __syncthreads();
   }
 }
@@ -110,6 +120,8 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
 __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n,
                                                       uint64_t chmax, AggPlan plan) {
   __shared__ uint32_t scratch[17];
+  if (threadIdx.x == 0) *plan.split_rows = 0;
+  __syncthreads();
   extern __shared__ uint32_t cnt[];  // P + 1
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
     const uint64_t b = offs[(uint64_t)p * ntiles];
@@ -117,6 +129,7 @@ __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint
     plan.pstart[p] = b;
     const uint64_t len = e - b;
     cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
+    if (cnt[p] > 1) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
   }
   if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
   __syncthreads();
@@ -283,7 +296,8 @@ constexpr int kCompactThreads = 256, kCompactItems = 8;
 
 template <typename K>
 __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s, OutDesc o, uint64_t *desc,
-                                                             uint32_t *ticket, uint64_t *count_out) {
+                                                             uint32_t *ticket, uint64_t *count_out,
+                                                             const uint64_t *base) {
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_excl;
   __shared__ uint32_t wcnt[kCompactItems][kCompactThreads / kWave];
@@ -293,14 +307,14 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
   const uint64_t tile = s_tile;
   const uint64_t n = g.S + 1;
   const uint64_t TS = (uint64_t)kCompactThreads * kCompactItems;
-  const uint64_t base = tile * TS;
+  const uint64_t tile_base = tile * TS;
   const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
   const K *keys = (const K *)g.keys;
   bool occ[kCompactItems];
   uint32_t wpre[kCompactItems];
 #pragma unroll
   for (int i = 0; i < kCompactItems; ++i) {
-    const uint64_t slot = base + (uint64_t)i * kCompactThreads + threadIdx.x;
+    const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
     bool oc = false;
     if (slot < g.S) oc = keys[slot] != KeyTraits<K>::kEmpty;
     else if (slot == g.S) oc = g.flags[FLAG_SENTINEL] != 0;
@@ -320,8 +334,9 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
   if (wave == 0) {
     const uint64_t ex = lookback_exclusive(desc, tile, s_total);
     if (lane == 0) {
-      s_excl = ex;
-      if (base + TS >= n) *count_out = ex + s_total;  // last tile
+      const uint64_t b0 = base ? *base : 0;
+      s_excl = b0 + ex;
+      if (tile_base + TS >= n) *count_out = b0 + ex + s_total;  // last tile
     }
   }
   __syncthreads();
@@ -329,7 +344,7 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
 #pragma unroll
   for (int i = 0; i < kCompactItems; ++i) {
     if (!occ[i]) continue;
-    const uint64_t slot = base + (uint64_t)i * kCompactThreads + threadIdx.x;
+    const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
     const uint64_t pos = excl + wcnt[i][wave] + wpre[i];
     if (pos >= o.capacity) continue;
     ((K *)o.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
@@ -454,6 +469,12 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) 
   return hipGetLastError();
 }
 
+hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t st) {
+  if (n16 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy16, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st) {
   const uint64_t n = (g.S + 1) * kLimbs;
   if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, g, s);
@@ -469,19 +490,22 @@ static hipError_t set_lds_limit(const void *fn, size_t bytes) {
 }
 
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
-                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st) {
+                      const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
+                      int emit, const OutDesc *od, uint64_t *emit_count) {
+  OutDesc o{};
+  if (od) o = *od;
   const size_t lds = L.bytes;
   hipError_t e;
   if (key_bytes == 4) {
     e = set_lds_limit<uint32_t>((const void *)&k_agg<uint32_t>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_agg<uint32_t>, dim3(grid), dim3(block), lds, st, (const uint32_t *)keys, s, plan, L, g,
-                       nbits, max_probe);
+                       nbits, max_probe, emit, o, emit_count);
   } else {
     e = set_lds_limit<uint64_t>((const void *)&k_agg<uint64_t>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_agg<uint64_t>, dim3(grid), dim3(block), lds, st, (const uint64_t *)keys, s, plan, L, g,
-                       nbits, max_probe);
+                       nbits, max_probe, emit, o, emit_count);
   }
   return hipGetLastError();
 }
@@ -558,14 +582,14 @@ uint64_t compact_tiles(uint64_t slots_plus_one) {
 }
 
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
-                          uint32_t *ticket, uint64_t *count_out, hipStream_t st) {
+                          uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base) {
   const uint64_t tiles = compact_tiles(g.S + 1);
   if (key_bytes == 4)
     hipLaunchKernelGGL(k_compact<uint32_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
-                       count_out);
+                       count_out, base);
   else
     hipLaunchKernelGGL(k_compact<uint64_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
-                       count_out);
+                       count_out, base);
   return hipGetLastError();
 }
 

@@ -222,6 +222,9 @@ tga_status tga_device_alloc(tga_context *ctx, uint64_t bytes, void **out);
 tga_status tga_device_free(tga_context *ctx, void *p);
 tga_status tga_memcpy_h2d(tga_context *ctx, void *dst, const void *src, uint64_t bytes);
 tga_status tga_memcpy_d2h(tga_context *ctx, void *dst, const void *src, uint64_t bytes);
+/* Streaming device-to-device copy kernel (16-byte aligned, bytes % 16 == 0):
+ * the bandwidth and counter-scaling reference for the benchmark. */
+tga_status tga_device_copy(tga_context *ctx, void *dst, const void *src, uint64_t bytes);
 tga_status tga_synchronize(tga_context *ctx);
 
 /* Per-kernel timing with HIP events on the context stream. */
