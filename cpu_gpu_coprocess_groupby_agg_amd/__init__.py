@@ -310,6 +310,28 @@ class Context:
         finally:
             lib().tga_result_free(r.value)
 
+    def groupby_resident(self, key, values, aggs, n_rows=None, **opts):
+        """Runs the operator on DeviceArray columns and leaves the result in
+        HBM (no copy to the host); returns (result handle, group count).
+        Release the handle with free_result."""
+        kcol = key.column()
+        n = key.n if n_rows is None else n_rows
+        vcols = (Column * max(1, len(values)))(*[v.column() for v in values])
+        al = _agg_list(aggs)
+        ag = (Agg * len(al))(*[Agg(o, c) for o, c in al])
+        o = self.make_options(**opts)
+        r = ctypes.c_void_p()
+        self._check(lib().tga_groupby(self.h, ctypes.byref(kcol), vcols, len(values), n, ag, len(al),
+                                      ctypes.byref(o), ctypes.byref(r)))
+        g = ctypes.c_int64()
+        self._check(lib().tga_result_num_groups(r.value, ctypes.byref(g)))
+        return r.value, g.value
+
+    @staticmethod
+    def free_result(r):
+        if r:
+            lib().tga_result_free(r)
+
     def _fetch(self, r, key_dtype, dts, sort):
         g = ctypes.c_int64()
         self._check(lib().tga_result_num_groups(r, ctypes.byref(g)))

@@ -43,7 +43,9 @@ __device__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
   const uint64_t n = g.S + 1;
   K *keys = (K *)g.keys;
   for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) keys[i] = KeyTraits<K>::kEmpty;
-  for (int c = 0; c < Q.s.nacc; ++c) {
+  #pragma unroll
+  for (int c = 0; c < kMaxAccs; ++c) {
+    if (c >= Q.s.nacc) break;
     const int kind = Q.s.acc[c].kind;
     if (kind == ACC_SUM_F) {
       for (int l = 0; l < kLimbs; ++l)
@@ -146,13 +148,13 @@ __device__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_
   }
 }
 
-template <typename K>
+template <typename K, bool FX, int NC>
 __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e, char *smem,
                          uint32_t *sentinel) {
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
   __syncthreads();
-  agg_rows<K>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
+  agg_rows<K, FX, NC>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
   __syncthreads();
   lds_flush<K>(t, s, Q.g);
 }
@@ -184,9 +186,12 @@ __device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *
     __syncthreads();
     if (occ) {
       const uint64_t pos = *s_base + scratch[wave] + pre;
-      if (pos < Q.od.capacity) {
+      if (pos >= Q.od.capacity) atomicOr(&Q.g.flags[FLAG_OVERFLOW], 1u);
+      else {
         ((K *)Q.od.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
-        for (int c = 0; c < Q.s.nacc; ++c) {
+        #pragma unroll
+  for (int c = 0; c < kMaxAccs; ++c) {
+    if (c >= Q.s.nacc) break;
           const int kind = Q.s.acc[c].kind;
           if (kind == ACC_COUNT || kind == ACC_SUM_I) ((uint64_t *)Q.od.cols[c])[pos] = ((const uint64_t *)g.acc[c])[slot];
           else if (kind == ACC_MIN_I32 || kind == ACC_MAX_I32) ((uint32_t *)Q.od.cols[c])[pos] = ((const uint32_t *)g.acc[c])[slot];
@@ -196,7 +201,7 @@ __device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *
             ((double *)Q.od.cols[c])[pos] = fx_to_double(Lm);
           } else {
             const int64_t v = ((const int64_t *)g.acc[c])[slot];
-            const int dt = Q.s.col_dtype[Q.s.acc[c].col];
+            const int dt = Q.s.acc[c].col_dtype;
             if (dt == DT_U64) ((uint64_t *)Q.od.cols[c])[pos] = (uint64_t)v ^ (1ULL << 63);
             else if (dt == DT_F64) ((uint64_t *)Q.od.cols[c])[pos] = ordered_to_f64_bits(v);
             else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)Q.od.cols[c])[pos] = (uint32_t)v;
@@ -209,19 +214,21 @@ __device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *
   }
 }
 
-template <typename K>
+template <typename K, bool FX, int NC>
 __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
                          uint64_t *s_base) {
   switch (it.type) {
     case IT_INIT: item_init<K>(Q, it.a, it.b); break;
     case IT_COUNT: item_count<K>(Q, (uint32_t)it.a, (uint32_t *)smem); break;
-    case IT_DIRECT: item_agg<K>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel); break;
+    case IT_DIRECT: item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel); break;
     case IT_SCAN: item_scan(Q, (uint32_t *)smem, scratch); break;
     case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch); break;
     case IT_PAGG: {
       Schema sp = Q.s;
-      for (int c = 0; c < sp.ncols; ++c) sp.cols[c] = Q.pcols[c];
-      item_agg<K>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel);
+#pragma unroll
+      for (int c = 0; c < kMaxCols; ++c)
+        if (c < sp.ncols) sp.cols[c] = Q.pcols[c];
+      item_agg<K, FX, NC>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel);
       break;
     }
     case IT_COMPACT: item_compact<K>(Q, it.a, it.b, scratch, s_base); break;
@@ -229,6 +236,7 @@ __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t 
   }
 }
 
+template <bool FX, int NC>
 __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
                                                          uint32_t nitems, uint32_t *ticket) {
   extern __shared__ __align__(16) char smem[];
@@ -254,16 +262,19 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (Q.key_bytes == 4) run_item<uint32_t>(Q, it, smem, scratch, &s_sentinel, &s_base);
-    else run_item<uint64_t>(Q, it, smem, scratch, &s_sentinel, &s_base);
+    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base);
+    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_fetch_add(&Q.done[it.type], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-__global__ void k_batch_reset(uint32_t *done, uint64_t ndone, uint64_t *counts, uint64_t nq, uint32_t *ticket) {
+__global__ void k_batch_reset(uint32_t *done, uint64_t ndone, uint64_t *counts, uint64_t nq, uint32_t *ticket,
+                              uint32_t *flags) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ndone; i += (uint64_t)gridDim.x * blockDim.x) done[i] = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq * kFlagWords; i += (uint64_t)gridDim.x * blockDim.x)
+    flags[i] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) counts[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0;
 }
@@ -367,21 +378,8 @@ tga_status batch_prepare(tga_context *ctx, const tga_query *queries, int32_t n, 
   std::vector<uint64_t> h(todo.size() * M);
   TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (size_t j = 0; j < todo.size(); ++j) {
-    uint64_t *p = h.data() + j * M;
-    const uint64_t m = ms[j];
-    std::sort(p, p + m);
-    uint64_t f1 = 0, rest = 0;
-    for (uint64_t i = 0; i < m;) {
-      uint64_t k = i;
-      while (k < m && p[k] == p[i]) ++k;
-      if (k - i == 1) ++f1; else ++rest;
-      i = k;
-    }
-    const uint64_t rows = (uint64_t)queries[todo[j]].n_rows;
-    const double d = std::sqrt((double)rows / (double)m) * (double)f1 + (double)rest;
-    b.est[todo[j]] = (uint64_t)std::min<double>((double)rows, std::max(1.0, d));
-  }
+  for (size_t j = 0; j < todo.size(); ++j)
+    b.est[todo[j]] = estimate_from_sample(h.data() + j * M, ms[j], (uint64_t)queries[todo[j]].n_rows);
   return TGA_OK;
 }
 
@@ -416,7 +414,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
     Q.path = z.part ? TGA_PATH_PARTITIONED : TGA_PATH_DIRECT;
     Q.L = make_layout(q.schema, q.key_bytes, z.T);
     Q.nbits = lg2((uint64_t)z.T + 1);
-    Q.max_probe = (int)std::min<uint32_t>(z.T, 256);
+    Q.max_probe = (int)std::min<uint32_t>(z.T / 4, 64);  // in 4-slot buckets
     lds_need = std::max<size_t>(lds_need, Q.L.bytes);
     // global table; its status words live in the batch's flag block
     {
@@ -499,8 +497,13 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
     TGA_HIP(ctx, hipMemcpyAsync(b.ditems.p, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   b.lds = std::max<size_t>(lds_need, 1024);
+  // kernel variant: fp64 SUM anywhere in the batch / more than one value column
+  bool fx = false, multi = false;
+  for (int32_t i = 0; i < n; ++i) { fx |= b.spec[i].nfx > 0; multi |= b.spec[i].schema.ncols > 1; }
+  b.kernel = fx ? (const void *)&k_batch<true, kMaxCols> : multi ? (const void *)&k_batch<false, kMaxCols>
+                                                                 : (const void *)&k_batch<false, 1>;
   if (b.lds > 65536)
-    TGA_HIP(ctx, hipFuncSetAttribute((const void *)&k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+    TGA_HIP(ctx, hipFuncSetAttribute(b.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
   const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / b.lds));
   b.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->cus * per_cu, b.nitems));
   return TGA_OK;
@@ -510,12 +513,15 @@ tga_status batch_enqueue(BatchExec &b, hipStream_t st) {
   tga_context *ctx = b.ctx;
   const uint64_t n = (uint64_t)b.nq;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, b.ddone.as<uint32_t>(), n * kItemTypes,
-                     b.dcounts.as<uint64_t>(), n, b.dticket.as<uint32_t>());
+                     b.dcounts.as<uint64_t>(), n, b.dticket.as<uint32_t>(), b.dflags.as<uint32_t>());
   TGA_HIP(ctx, hipGetLastError());
   if (b.nitems) {
-    hipLaunchKernelGGL(k_batch, dim3(b.grid), dim3(kBatchThreads), b.lds, st, b.dq.as<BQuery>(), b.ditems.as<BItem>(),
-                       b.nitems, b.dticket.as<uint32_t>());
-    TGA_HIP(ctx, hipGetLastError());
+    const BQuery *qarg = b.dq.as<BQuery>();
+    const BItem *iarg = b.ditems.as<BItem>();
+    uint32_t narg = b.nitems;
+    uint32_t *targ = b.dticket.as<uint32_t>();
+    void *args[] = {&qarg, &iarg, &narg, &targ};
+    TGA_HIP(ctx, hipLaunchKernel(b.kernel, dim3(b.grid), dim3(kBatchThreads), args, b.lds, st));
   }
   if (n) {
     TGA_HIP(ctx, hipMemcpyAsync(b.h_counts, b.dcounts.p, n * 8, hipMemcpyDeviceToHost, st));

@@ -59,6 +59,7 @@ struct BatchExec {
   uint32_t nitems = 0;
   size_t lds = 0;
   unsigned grid = 0;
+  const void *kernel = nullptr;  // k_batch variant for this batch
   uint64_t *h_counts = nullptr;  // pinned: group count per query
   uint32_t *h_flags = nullptr;   // pinned: status words per query
   ~BatchExec();

@@ -193,8 +193,10 @@ std::string build_spec(const tga_column *key, const tga_column *values, int32_t 
     }
     s.acc[a].kind = kind;
     s.acc[a].col = col;
+    s.acc[a].col_dtype = op == OP_COUNT ? 0 : s.col_dtype[col];
   }
   q.nfx = nfx;
+  s.nfx = nfx;
   return "";
 }
 
@@ -273,12 +275,15 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     p.workgroups = W;
     p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(1, (n + W - 1) / W);
   } else {
-    uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : tmax;
+    // the largest table one workgroup's LDS holds (any slot count: the slot
+    // is a multiply-shift of the hash), filled to at most 3/4 on average
+    uint32_t tfull = (uint32_t)std::min<uint64_t>(65535, (lds_max - 512) / sb - 1);
+    tfull &= ~63u;
+    uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
     int pbits = o.partition_bits;
     if (pbits <= 0) {
-      // groups per partition at most half a table
-      const uint64_t P = next_pow2((est * 2 + T - 1) / T);
+      const uint64_t P = next_pow2((est * 4 + 3 * (uint64_t)T - 1) / (3 * (uint64_t)T));
       pbits = std::max(1, ilog2(P));
     }
     pbits = std::min(pbits, 12);
@@ -294,9 +299,10 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     const uint64_t P = 1ULL << pbits;
     p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(65536, 2 * ((n + P - 1) / P));
   }
+  p.table_slots = std::max<uint32_t>(4, p.table_slots & ~3u);  // whole 4-slot buckets
   p.layout = make_layout(q.schema, q.key_bytes, p.table_slots);
   p.nbits = ilog2((uint64_t)p.table_slots + 1);
-  p.max_probe = (int)std::min<uint32_t>(p.table_slots, 256);
+  p.max_probe = (int)std::min<uint32_t>(p.table_slots / 4, 64);  // in buckets
   // global table at load <= 1/2 of the estimate, at most 2 n slots
   uint64_t gs = next_pow2(std::max<uint64_t>(1024, est * 2));
   gs = std::min<uint64_t>(gs, next_pow2(std::max<uint64_t>(1024, 2 * n)));
@@ -319,6 +325,29 @@ static void plan_to_options(const Plan &p, tga_options &o) {
 // ---------------------------------------------------------------------------
 // cardinality estimate
 // ---------------------------------------------------------------------------
+uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n) {
+  if (m == 0 || n == 0) return 1;
+  std::sort(v, v + m);
+  uint64_t f1 = 0, d = 0;
+  for (uint64_t i = 0; i < m;) {
+    uint64_t j = i;
+    while (j < m && v[j] == v[i]) ++j;
+    if (j - i == 1) ++f1;
+    ++d;
+    i = j;
+  }
+  if (m >= n) return d;  // the sample is the whole input
+  // GEE (Charikar et al. 2000): sqrt(n/m) f1 + (d - f1); good under skew
+  const double gee = std::sqrt((double)n / (double)m) * (double)f1 + (double)(d - f1);
+  // equal-frequency model: f1/m = exp(-m/D) for D >> ... ; good for uniform keys,
+  // where GEE badly underestimates when most sampled keys are singletons
+  double uni = (double)n;
+  if (f1 < m) uni = -(double)m / std::log((double)f1 / (double)m + 1e-300);
+  if (f1 == 0) uni = (double)d;
+  const double e = std::max(gee, std::min(uni, (double)n));
+  return (uint64_t)std::min<double>((double)n, std::max<double>((double)d, e));
+}
+
 tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est) {
   if (n == 0) { *est = 1; return TGA_OK; }
   const uint64_t m = std::min<uint64_t>(n, 16384);
@@ -328,16 +357,7 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   std::vector<uint64_t> h(m);
   TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::sort(h.begin(), h.end());
-  uint64_t f1 = 0, rest = 0;
-  for (uint64_t i = 0; i < m;) {
-    uint64_t j = i;
-    while (j < m && h[j] == h[i]) ++j;
-    if (j - i == 1) ++f1; else ++rest;
-    i = j;
-  }
-  const double d = std::sqrt((double)n / (double)m) * (double)f1 + (double)rest;
-  *est = (uint64_t)std::min<double>((double)n, std::max<double>(1.0, d));
+  *est = estimate_from_sample(h.data(), m, n);
   return TGA_OK;
 }
 
@@ -430,8 +450,7 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
                                    pkeys.p, po, st);
       }));
       if (!planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
-      AggPlan ap{planbuf.as<uint64_t>(), (uint32_t *)(planbuf.as<uint64_t>() + (P + 1)), P,
-                 (uint32_t *)(planbuf.as<uint64_t>() + (P + 1)) + (P + 1)};
+      AggPlan ap = part_plan(planbuf, P);
       TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
         return launch_plan_chunks(offs.as<uint32_t>(), P, ntiles, m, plan.chunk_rows, ap, st);
       }));
@@ -575,8 +594,8 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   TGA_HIP(ctx, hipMemcpyAsync(&hcount, count, 8, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
   if (ctx->prof.on) ctx->prof.resolve();
-  *overflow = hflags[FLAG_OVERFLOW] != 0;
-  if (hflags[FLAG_OVERFLOW]) {
+  *overflow = hflags[FLAG_OVERFLOW] != 0 || (!ext_keys && hcount > cap);
+  if (*overflow) {
     // either the spill table or the output was too small: caller retries
     if (ext_keys && hcount > ext_cap) {
       *overflow = false;
@@ -655,6 +674,11 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
       if (c) return c;
     }
     if (ctx->prof.on) ctx->prof.resolve();
+    if (ext_keys && hcount > ext_cap) {
+      *groups = hcount;
+      return fail(ctx, TGA_ERR_CAPACITY, "%llu groups > capacity %llu", (unsigned long long)hcount,
+                  (unsigned long long)ext_cap);
+    }
     if (hflags[FLAG_OVERFLOW]) {
       // the estimate was too low: grow the global table and run again
       plan.gslots *= 4;

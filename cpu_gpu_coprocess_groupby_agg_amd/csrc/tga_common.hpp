@@ -45,6 +45,8 @@ enum : int { FLAG_SENTINEL = 0, FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_GROUPS =
 struct Acc {
   int32_t kind;
   int32_t col;
+  int32_t col_dtype;  // dtype of column `col` (copied here so kernels index no arrays)
+  int32_t pad;
 };
 
 struct Schema {
@@ -54,6 +56,7 @@ struct Schema {
   int32_t nacc;
   Acc acc[kMaxAccs];
   int32_t fx_slot[kMaxAccs];  // ACC_SUM_F accumulators: index among them
+  int32_t nfx;                // number of ACC_SUM_F accumulators
 };
 
 // Global table in HBM: the LDS table's SoA shape with S regular slots.
@@ -162,55 +165,59 @@ TGA_HD int fx_split(uint64_t b, int64_t c[3], int *range) {
   return i0;
 }
 
+TGA_HD int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
 // Exact value of sum_i L[i] * 2^(32 i - kFxShift), rounded to nearest even.
+// The limbs are combined into a 192-bit two's-complement integer (hi:lo),
+// whose magnitude is rounded once; only fixed-index operations, so the
+// device code keeps everything in registers.
 TGA_HD double fx_to_double(const int64_t *L) {
-  constexpr int W = kLimbs + 2;
-  uint32_t w[W];
-  __int128 carry = 0;
-  for (int i = 0; i < kLimbs; ++i) {
-    __int128 t = (__int128)L[i] + carry;
-    w[i] = (uint32_t)(t & 0xffffffff);
-    carry = t >> 32;
+  static_assert(kLimbs == 5, "fx_to_double combines exactly five limbs");
+  const __int128 t = (__int128)L[0] + ((__int128)L[1] << 32);
+  __int128 u = (__int128)L[2] + ((__int128)L[3] << 32) + ((__int128)L[4] << 64);
+  u += t >> 64;                 // V = u * 2^64 + lo
+  uint64_t lo = (uint64_t)t;
+  const bool neg = u < 0;
+  unsigned __int128 hi;
+  if (neg) {                    // magnitude of the 192-bit value
+    lo = ~lo + 1;
+    hi = ~(unsigned __int128)u + (lo == 0 ? 1 : 0);
+  } else {
+    hi = (unsigned __int128)u;
   }
-  w[kLimbs] = (uint32_t)(carry & 0xffffffff);
-  w[kLimbs + 1] = (uint32_t)((carry >> 32) & 0xffffffff);
-  bool neg = carry < 0;
-  if (neg) {  // two's-complement negate of the W-word integer
-    uint64_t cy = 1;
-    for (int i = 0; i < W; ++i) {
-      uint64_t t = (uint64_t)(uint32_t)~w[i] + cy;
-      w[i] = (uint32_t)t;
-      cy = t >> 32;
+  if (hi == 0 && lo == 0) return 0.0;
+  int hb;                       // highest set bit of hi:lo
+  uint64_t mant;                // the 64 bits from hb down
+  bool sticky;                  // any bit below them
+  if (hi != 0) {
+    const uint64_t h1 = (uint64_t)(hi >> 64), h0 = (uint64_t)hi;
+    hb = h1 ? 191 - clz64(h1) : 127 - clz64(h0);
+    const int sh = hb - 63;     // 1 .. 128
+    if (sh >= 64) {
+      const int r = sh - 64;    // 0 .. 64
+      mant = (uint64_t)(hi >> r);
+      const unsigned __int128 mask = r ? ((((unsigned __int128)1) << r) - 1) : 0;
+      sticky = lo != 0 || (hi & mask) != 0;
+    } else {
+      mant = (uint64_t)((hi << (64 - sh)) | (unsigned __int128)(lo >> sh));
+      sticky = (lo & ((1ULL << sh) - 1)) != 0;
     }
+  } else {
+    hb = 63 - clz64(lo);
+    mant = lo << (63 - hb);
+    sticky = false;
   }
-  int top = -1;
-  for (int i = W - 1; i >= 0; --i)
-    if (w[i]) { top = i; break; }
-  if (top < 0) return 0.0;
-  int hb = 32 * top + 31 - __builtin_clz(w[top]);  // highest set bit
-  // top 64 bits of the magnitude, plus a sticky bit for everything below
-  uint64_t mant = 0;
-  bool sticky = false;
-  for (int bit = 0; bit < 64; ++bit) {
-    int pos = hb - bit;
-    uint64_t v = 0;
-    if (pos >= 0) v = (w[pos >> 5] >> (pos & 31)) & 1u;
-    mant = (mant << 1) | v;
-  }
-  for (int pos = hb - 64; pos >= 0; --pos)
-    if ((w[pos >> 5] >> (pos & 31)) & 1u) { sticky = true; break; }
   uint64_t q = mant >> 11;
-  uint64_t rem = mant & 0x7ff;
+  const uint64_t rem = mant & 0x7ff;
   if (rem > 0x400 || (rem == 0x400 && (sticky || (q & 1)))) {
     ++q;
     if (q == (1ULL << 53)) { q >>= 1; ++hb; }
   }
-  // value = q * 2^(hb - 52 - kFxShift), always a normal double here
+  // value = q * 2^(hb - 52 - kFxShift), a normal double for every window value
   int ex = hb - 52 - kFxShift;
   double r = (double)q;
-  // scale by 2^ex exactly
-  while (ex > 0) { int s = ex > 60 ? 60 : ex; r *= (double)(1ULL << s); ex -= s; }
-  while (ex < 0) { int s = -ex > 60 ? 60 : -ex; r /= (double)(1ULL << s); ex += s; }
+  while (ex > 0) { const int k = ex > 60 ? 60 : ex; r *= (double)(1ULL << k); ex -= k; }
+  while (ex < 0) { const int k = -ex > 60 ? 60 : -ex; r /= (double)(1ULL << k); ex += k; }
   return neg ? -r : r;
 }
 

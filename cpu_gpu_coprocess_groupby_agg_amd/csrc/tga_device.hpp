@@ -8,6 +8,7 @@
 namespace tga {
 
 constexpr int kWave = 64;
+constexpr uint64_t kMaxGlobalProbe = 4096;
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int l = (int)__lane_id();
@@ -59,9 +60,10 @@ __device__ __forceinline__ uint64_t load_cell(const Schema &s, int c, int64_t ro
 __device__ __forceinline__ uint64_t acc_operand(const Schema &s, int a, const uint64_t *cell) {
   const Acc ac = s.acc[a];
   if (ac.kind == ACC_COUNT) return 1;
-  const uint64_t raw = cell[ac.col];
+  // fixed-index select keeps `cell` in registers
+  const uint64_t raw = ac.col == 0 ? cell[0] : ac.col == 1 ? cell[1] : ac.col == 2 ? cell[2] : cell[3];
   if (ac.kind == ACC_MIN_I64 || ac.kind == ACC_MAX_I64) {
-    const int dt = s.col_dtype[ac.col];
+    const int dt = ac.col_dtype;
     if (dt == DT_U64) return raw ^ (1ULL << 63);
     if (dt == DT_F64) return (uint64_t)f64_bits_to_ordered(raw);
   }
@@ -111,11 +113,13 @@ struct LdsTable {
 };
 
 template <typename K>
-__device__ void lds_init(const LdsTable<K> &t, const Schema &s) {
+__device__ __forceinline__ void lds_init(const LdsTable<K> &t, const Schema &s) {
   const uint32_t n = t.L.slots + 1;
   K *k = t.keys();
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) k[i] = KeyTraits<K>::kEmpty;
-  for (int a = 0; a < s.nacc; ++a) {
+  #pragma unroll
+  for (int a = 0; a < kMaxAccs; ++a) {
+    if (a >= s.nacc) break;
     const int kind = s.acc[a].kind;
     if (kind == ACC_SUM_F) {
       int64_t *p = t.template arr<int64_t>(a);
@@ -133,24 +137,52 @@ __device__ void lds_init(const LdsTable<K> &t, const Schema &s) {
   if (threadIdx.x == 0) *t.sentinel = 0;
 }
 
-// Slot of `key`, inserting it if absent; -1 when `max_probe` slots were full.
+// Slot of `key`, inserting it if absent; -1 when `max_probe` buckets were
+// full.  The table is probed in buckets of 4 slots read with one 16-byte
+// (u32 keys) or two 16-byte (u64 keys) LDS loads, so a lane needs about one
+// LDS round trip per lookup even at 3/4 load (one slot at a time, the
+// longest probe among the 64 lanes of a wave sets the wave's cost).  Empty
+// slots of a bucket are claimed in index order with CAS, so a key never sits
+// behind an empty slot of its probe sequence.
+template <typename K>
+__device__ __forceinline__ void lds_bucket(const K *keys, uint32_t s0, K *k4) {
+  if constexpr (sizeof(K) == 8) {
+    const uint4 a = *(const uint4 *)(keys + s0);
+    const uint4 b = *(const uint4 *)(keys + s0 + 2);
+    k4[0] = ((uint64_t)a.y << 32) | a.x;
+    k4[1] = ((uint64_t)a.w << 32) | a.z;
+    k4[2] = ((uint64_t)b.y << 32) | b.x;
+    k4[3] = ((uint64_t)b.w << 32) | b.z;
+  } else {
+    const uint4 a = *(const uint4 *)(keys + s0);
+    k4[0] = a.x; k4[1] = a.y; k4[2] = a.z; k4[3] = a.w;
+  }
+}
+
 template <typename K>
 __device__ __forceinline__ int lds_find_or_insert(const LdsTable<K> &t, K key, uint64_t h, int max_probe) {
-  const uint32_t S = t.L.slots;
+  const uint32_t S = t.L.slots;  // a multiple of 4
   if (key == KeyTraits<K>::kEmpty) {
     *t.sentinel = 1;
     return (int)S;
   }
   K *keys = t.keys();
-  uint32_t s = hash_lds_slot(h, S);
+  const uint32_t nb = S >> 2;
+  uint32_t b = hash_lds_slot(h, nb);
   for (int i = 0; i < max_probe; ++i) {
-    K cur = keys[s];
-    if (cur == key) return (int)s;
-    if (cur == KeyTraits<K>::kEmpty) {
-      const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
-      if (prev == KeyTraits<K>::kEmpty || prev == key) return (int)s;
+    const uint32_t s0 = b << 2;
+    K k4[4];
+    lds_bucket<K>(keys, s0, k4);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (k4[j] == key) return (int)(s0 + j);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (k4[j] != KeyTraits<K>::kEmpty) continue;
+      const K prev = atomicCAS(&keys[s0 + j], KeyTraits<K>::kEmpty, key);
+      if (prev == KeyTraits<K>::kEmpty || prev == key) return (int)(s0 + j);
     }
-    s = (s + 1 == S) ? 0 : s + 1;
+    b = (b + 1 == nb) ? 0 : b + 1;
   }
   return -1;
 }
@@ -192,7 +224,10 @@ __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint
   K *keys = (K *)g.keys;
   const uint64_t S = g.S;
   uint64_t s = hash_global_slot(h, S);
-  for (uint64_t i = 0; i < S; ++i) {
+  // Tables are sized for load <= 1/2; a probe this long means the table is
+  // overloaded, so stop and let the host grow it (never scan the whole table).
+  const uint64_t limit = S < kMaxGlobalProbe ? S : kMaxGlobalProbe;
+  for (uint64_t i = 0; i < limit; ++i) {
     const K cur = keys[s];
     if (cur == key) return (int64_t)s;
     if (cur == KeyTraits<K>::kEmpty) {
@@ -306,7 +341,7 @@ __device__ uint64_t lookback_exclusive(uint64_t *desc, uint64_t tile, uint64_t a
 
 // Exclusive scan of n u32 values held in LDS, in place; returns the total.
 // scratch needs blockDim/64 + 1 words.  All threads must call it.
-__device__ inline uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
+__device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t per = (n + nt - 1) / nt;
   const uint32_t b = tid * per, e = min(n, b + per);
@@ -336,11 +371,14 @@ __device__ inline uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, u
 }
 
 // Aggregates rows [b, e) into the LDS table.  Rows whose key finds no free
-// slot within max_probe go straight to the global table.
-template <typename K>
-__device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b, uint64_t e,
+// slot within max_probe go straight to the global table.  FX: the schema has
+// exact fp64 SUM accumulators (their limb vectors cost registers, so the
+// common integer case is compiled without them).
+template <typename K, bool FX, int NC = kMaxCols>
+__device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b, uint64_t e,
                          const LdsTable<K> &t, const GTable &g, int nbits, int max_probe) {
   constexpr int U = 4;
+  constexpr int NF = FX ? kMaxFx : 1;
   const int lane = (int)__lane_id();
   const int wave = (int)(threadIdx.x >> 6);
   const int nw = (int)(blockDim.x >> 6);
@@ -354,7 +392,7 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
       const bool act = row < e;
       kk[u] = act ? keys[row] : KeyTraits<K>::kEmpty;
 #pragma unroll
-      for (int c = 0; c < kMaxCols; ++c) cell[u][c] = (act && c < ncol) ? load_cell(s, c, row) : 0;
+      for (int c = 0; c < kMaxCols; ++c) cell[u][c] = (c < NC && act && c < ncol) ? load_cell(s, c, row) : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -365,13 +403,22 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
       const uint64_t h = key_hash((uint64_t)key);
       const int slot = act ? lds_find_or_insert(t, key, h, max_probe) : -1;
       uint64_t ops[kMaxAccs];
-      int64_t lv[kMaxFx][kLimbs];
+      int64_t lv[NF][kLimbs];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int l = 0; l < kLimbs; ++l) lv[f][l] = 0;
 #pragma unroll
       for (int a = 0; a < kMaxAccs; ++a) {
         ops[a] = 0;
         if (a < nacc && act) {
           ops[a] = acc_operand(s, a, cell[u]);
-          if (s.acc[a].kind == ACC_SUM_F) fx_expand(ops[a], lv[s.fx_slot[a]], g.flags);
+          if constexpr (FX) {
+            if (s.acc[a].kind == ACC_SUM_F) {
+              if (s.fx_slot[a] == 0) fx_expand(ops[a], lv[0], g.flags);
+              else fx_expand(ops[a], lv[NF - 1], g.flags);
+            }
+          }
         }
       }
       const bool spill = act && slot < 0;
@@ -383,8 +430,11 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
             for (int a = 0; a < kMaxAccs; ++a) {
               if (a >= nacc) break;
               const int kind = s.acc[a].kind;
-              if (kind == ACC_SUM_F) g_apply_limbs(g, a, gs, lv[s.fx_slot[a]]);
-              else g_apply(g, a, kind, gs, ops[a]);
+              if (kind == ACC_SUM_F) {
+                if constexpr (FX) g_apply_limbs(g, a, gs, s.fx_slot[a] == 0 ? lv[0] : lv[NF - 1]);
+              } else {
+                g_apply(g, a, kind, gs, ops[a]);
+              }
             }
           }
         }
@@ -400,17 +450,19 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
         for (int a = 0; a < kMaxAccs; ++a) {
           if (a >= nacc) break;
           const int kind = s.acc[a].kind;
-          if (kind == ACC_SUM_F) {
-            const int f = s.fx_slot[a];
+          if (kind != ACC_SUM_F) {
+            const uint64_t o = shfl64(ops[a], src);
+            if (rest) ops[a] = acc_fold(kind, ops[a], o);
+          }
+        }
+        if constexpr (FX) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int l = 0; l < kLimbs; ++l) {
               const int64_t o = (int64_t)shfl64((uint64_t)lv[f][l], src);
               if (rest) lv[f][l] += o;
             }
-          } else {
-            const uint64_t o = shfl64(ops[a], src);
-            if (rest) ops[a] = acc_fold(kind, ops[a], o);
-          }
         }
         if (rest) rest &= rest - 1;
       }
@@ -419,8 +471,11 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
         for (int a = 0; a < kMaxAccs; ++a) {
           if (a >= nacc) break;
           const int kind = s.acc[a].kind;
-          if (kind == ACC_SUM_F) lds_apply_limbs(t, a, slot, lv[s.fx_slot[a]]);
-          else lds_apply(t, a, kind, slot, ops[a]);
+          if (kind == ACC_SUM_F) {
+            if constexpr (FX) lds_apply_limbs(t, a, slot, s.fx_slot[a] == 0 ? lv[0] : lv[NF - 1]);
+          } else {
+            lds_apply(t, a, kind, slot, ops[a]);
+          }
         }
       }
     }
@@ -428,7 +483,7 @@ __device__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b
 }
 
 template <typename K>
-__device__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g) {
+__device__ __forceinline__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g) {
   const uint32_t S = t.L.slots, n = S + 1;
   const K *keys = t.keys();
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -439,7 +494,9 @@ __device__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g
     if (!occ) continue;
     const int64_t gs = g_find_or_insert<K>(g, key, key_hash((uint64_t)key));
     if (gs < 0) continue;
-    for (int a = 0; a < s.nacc; ++a) {
+    #pragma unroll
+  for (int a = 0; a < kMaxAccs; ++a) {
+    if (a >= s.nacc) break;
       const int kind = s.acc[a].kind;
       if (kind == ACC_SUM_F) {
         int64_t lv[kLimbs];
@@ -465,7 +522,9 @@ __device__ __forceinline__ void emit_row(const LdsTable<K> &t, const Schema &s, 
                                          uint64_t pos) {
   const uint32_t n = t.L.slots + 1;
   ((K *)od.keys)[pos] = key;
-  for (int a = 0; a < s.nacc; ++a) {
+  #pragma unroll
+  for (int a = 0; a < kMaxAccs; ++a) {
+    if (a >= s.nacc) break;
     const int kind = s.acc[a].kind;
     switch (kind) {
       case ACC_COUNT: ((uint64_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
@@ -481,7 +540,7 @@ __device__ __forceinline__ void emit_row(const LdsTable<K> &t, const Schema &s, 
       }
       default: {
         const int64_t v = t.template arr<int64_t>(a)[i];
-        const int dt = s.col_dtype[s.acc[a].col];
+        const int dt = s.acc[a].col_dtype;
         if (dt == DT_U64) ((uint64_t *)od.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
         else if (dt == DT_F64) ((uint64_t *)od.cols[a])[pos] = ordered_to_f64_bits(v);
         else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)od.cols[a])[pos] = (uint32_t)v;
@@ -497,7 +556,7 @@ __device__ __forceinline__ void emit_row(const LdsTable<K> &t, const Schema &s, 
 // appear in no other partition).  Rows are reserved on *count with one
 // atomic per workgroup.  scratch: blockDim/64 + 1 words; s_base: one word.
 template <typename K>
-__device__ void lds_emit(const LdsTable<K> &t, const Schema &s, const OutDesc &od, uint64_t *count,
+__device__ __forceinline__ void lds_emit(const LdsTable<K> &t, const Schema &s, const OutDesc &od, uint64_t *count,
                          uint32_t *flags, uint32_t *scratch, uint64_t *s_base) {
   const uint32_t S = t.L.slots, n = S + 1;
   const K *keys = t.keys();

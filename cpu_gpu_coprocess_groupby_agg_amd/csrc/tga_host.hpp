@@ -155,6 +155,8 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint6
 // Cardinality estimate from a strided key sample (GEE estimator,
 // Charikar et al. 2000).  Needs the context lock.
 tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est);
+// Distinct-count estimate of n rows from a sample of m keys (sorted in place).
+uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

@@ -45,7 +45,9 @@ __global__ void k_gtable_init(GTable g, Schema s) {
   const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   K *keys = (K *)g.keys;
   for (uint64_t i = t0; i < n; i += stride) keys[i] = KeyTraits<K>::kEmpty;
-  for (int a = 0; a < s.nacc; ++a) {
+  #pragma unroll
+  for (int a = 0; a < kMaxAccs; ++a) {
+    if (a >= s.nacc) break;
     const int kind = s.acc[a].kind;
     if (kind == ACC_SUM_F) {
       uint64_t *p = (uint64_t *)g.acc[a];
@@ -70,7 +72,7 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 // Persistent: workgroups pull work items from plan.ticket until none remain.
 // Item -> (segment p, slice j of nch) through plan.cstart; rows of segment p
 // are [pstart[p], pstart[p+1]).
-template <typename K>
+template <typename K, bool FX, int NC>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
                                               GTable g, int nbits, int max_probe, int emit, OutDesc od,
                                               uint64_t *emit_count) {
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
     lds_init(t, s);
     __syncthreads();
-    agg_rows<K>(keys, s, b, e, t, g, nbits, max_probe);
+    agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe);
     __syncthreads();
     if (emit && nch == 1) lds_emit<K>(t, s, od, emit_count, g.flags, scratch, &s_base);
     else lds_flush<K>(t, s, g);
@@ -120,7 +122,7 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
 __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n,
                                                       uint64_t chmax, AggPlan plan) {
   __shared__ uint32_t scratch[17];
-  if (threadIdx.x == 0) *plan.split_rows = 0;
+  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
   __syncthreads();
   extern __shared__ uint32_t cnt[];  // P + 1
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint
     plan.pstart[p] = b;
     const uint64_t len = e - b;
     cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
-    if (cnt[p] > 1) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
+    if (cnt[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
   }
   if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
   __syncthreads();
@@ -346,9 +348,14 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
     if (!occ[i]) continue;
     const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
     const uint64_t pos = excl + wcnt[i][wave] + wpre[i];
-    if (pos >= o.capacity) continue;
+    if (pos >= o.capacity) {  // output too small: the caller grows it or reports the count
+      atomicOr(&g.flags[FLAG_OVERFLOW], 1u);
+      continue;
+    }
     ((K *)o.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
-    for (int a = 0; a < s.nacc; ++a) {
+    #pragma unroll
+  for (int a = 0; a < kMaxAccs; ++a) {
+    if (a >= s.nacc) break;
       const int kind = s.acc[a].kind;
       switch (kind) {
         case ACC_COUNT: case ACC_SUM_I:
@@ -366,7 +373,7 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
         }
         default: {  // ACC_MIN_I64 / ACC_MAX_I64 back to the column dtype
           const int64_t v = ((const int64_t *)g.acc[a])[slot];
-          const int dt = s.col_dtype[s.acc[a].col];
+          const int dt = s.acc[a].col_dtype;
           if (dt == DT_U64) ((uint64_t *)o.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
           else if (dt == DT_F64) ((uint64_t *)o.cols[a])[pos] = ordered_to_f64_bits(v);
           else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)o.cols[a])[pos] = (uint32_t)v;
@@ -489,25 +496,37 @@ static hipError_t set_lds_limit(const void *fn, size_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+template <typename K, bool FX, int NC>
+static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan &plan, const Layout &L, const GTable &g,
+                               int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st, int emit,
+                               const OutDesc &o, uint64_t *emit_count) {
+  const size_t lds = L.bytes;
+  hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_agg<K, FX, NC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g, nbits,
+                     max_probe, emit, o, emit_count);
+  return hipGetLastError();
+}
+
+// One kernel per (key width, fp64 SUM present, column count class): each
+// variant gets its own register allocation.
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
                       int emit, const OutDesc *od, uint64_t *emit_count) {
   OutDesc o{};
   if (od) o = *od;
-  const size_t lds = L.bytes;
-  hipError_t e;
-  if (key_bytes == 4) {
-    e = set_lds_limit<uint32_t>((const void *)&k_agg<uint32_t>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_agg<uint32_t>, dim3(grid), dim3(block), lds, st, (const uint32_t *)keys, s, plan, L, g,
-                       nbits, max_probe, emit, o, emit_count);
-  } else {
-    e = set_lds_limit<uint64_t>((const void *)&k_agg<uint64_t>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_agg<uint64_t>, dim3(grid), dim3(block), lds, st, (const uint64_t *)keys, s, plan, L, g,
-                       nbits, max_probe, emit, o, emit_count);
-  }
-  return hipGetLastError();
+  const bool fx = s.nfx > 0;
+  const bool one = s.ncols <= 1;
+#define TGA_AGG_CASE(K)                                                                                     \
+  if (fx) return launch_agg_t<K, true, kMaxCols>(keys, s, plan, L, g, nbits, max_probe, grid, block, st,   \
+                                                 emit, o, emit_count);                                     \
+  if (one) return launch_agg_t<K, false, 1>(keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, \
+                                            emit_count);                                                    \
+  return launch_agg_t<K, false, kMaxCols>(keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o,  \
+                                          emit_count);
+  if (key_bytes == 4) { TGA_AGG_CASE(uint32_t) }
+  TGA_AGG_CASE(uint64_t)
+#undef TGA_AGG_CASE
 }
 
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st) {
