@@ -243,10 +243,17 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
   __shared__ uint32_t s_item, s_sentinel;
   __shared__ uint32_t scratch[kBatchThreads / kWave + 1];
   __shared__ uint64_t s_base;
+  // the item finished last by this workgroup; its completion is published
+  // together with the next ticket fetch (one lane-0 region per iteration)
+  uint32_t prev_q = 0xffffffffu, prev_type = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
+    if (threadIdx.x == 0) {
+      if (prev_q != 0xffffffffu)
+        __hip_atomic_fetch_add(&qs[prev_q].done[prev_type], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_item = atomicAdd(ticket, 1u);
+    }
     __syncthreads();
-    const uint32_t i = s_item;
+    const uint32_t i = wave_uniform(s_item);
     if (i >= nitems) break;
     const BItem it = items[i];
     const BQuery &Q = qs[it.q];
@@ -266,7 +273,8 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
     else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(&Q.done[it.type], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    prev_q = it.q;
+    prev_type = it.type;
   }
 }
 

@@ -10,6 +10,14 @@ namespace tga {
 constexpr int kWave = 64;
 constexpr uint64_t kMaxGlobalProbe = 4096;
 
+// A value every work-item read from the same LDS word after a barrier, as a
+// wave-uniform scalar.  Work-queue loops exit on such a value; read from LDS
+// it is a per-lane register, and a loop with a per-lane exit may be compiled
+// so that a wave re-enters the loop's barrier while some of its lanes are
+// still parked at the exit (e.g. lane 0 before it fetches the next ticket).
+// A scalar exit condition keeps every lane of the wave on one path.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int l = (int)__lane_id();
   return l == 0 ? 0ULL : (~0ULL >> (64 - l));
@@ -303,7 +311,7 @@ __device__ __forceinline__ uint64_t lb_load(uint64_t *p) {
 
 // Called by all 64 lanes of one wave.  Returns the exclusive prefix of tile
 // `tile` and publishes its inclusive prefix.
-__device__ uint64_t lookback_exclusive(uint64_t *desc, uint64_t tile, uint64_t agg) {
+__device__ inline uint64_t lookback_exclusive(uint64_t *desc, uint64_t tile, uint64_t agg) {
   const int lane = (int)__lane_id();
   if (tile == 0) {
     if (lane == 0) lb_store(&desc[0], kLbInc | (agg & kLbVal));

@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(plan.ticket, 1u);
     __syncthreads();
-    const uint32_t item = s_item;
+    const uint32_t item = wave_uniform(s_item);
     if (item >= total) break;
     uint32_t lo = 0, hi = plan.P;
     while (hi - lo > 1) {
