@@ -154,7 +154,21 @@ __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uin
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
   __syncthreads();
-  agg_rows<K, FX, NC>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
+  if constexpr (FX) {
+    agg_rows<K, FX, NC>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
+  } else {
+    const void *cols[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
+    if constexpr (NC == 1) {
+      if (dtype_bytes((int)(Q.ap.coldt & 15u)) == 8)
+        agg_rows_int<K, 1, 8, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+      else
+        agg_rows_int<K, 1, 4, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+    } else {
+      agg_rows_int<K, NC, 0, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+    }
+  }
   __syncthreads();
   lds_flush<K>(t, s, Q.g);
 }
@@ -313,7 +327,7 @@ QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t
   const uint64_t G = std::max<uint64_t>(1, std::min<uint64_t>(est, std::max<uint64_t>(n, 1)));
   const uint32_t sb = lds_slot_bytes(q.schema, q.key_bytes);
   uint32_t tmax = 256;
-  while ((uint64_t)(tmax * 2 + 1) * sb + 256 <= lds_max && tmax < 8192) tmax *= 2;
+  while ((uint64_t)(tmax * 2 + 2) * sb + 256 <= lds_max && tmax < 8192) tmax *= 2;
   z.part = q.schema.ncols == 1 && dtype_bytes(q.schema.col_dtype[0]) == 4 && 2 * G > tmax && n > 65536;
   z.T = z.part ? tmax : (uint32_t)std::min<uint64_t>(tmax, std::max<uint64_t>(256, np2(2 * G)));
   z.S = std::min<uint64_t>(np2(std::max<uint64_t>(256, 2 * G)), np2(std::max<uint64_t>(256, 2 * n)));
@@ -423,6 +437,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
     Q.L = make_layout(q.schema, q.key_bytes, z.T);
     Q.nbits = lg2((uint64_t)z.T + 1);
     Q.max_probe = (int)std::min<uint32_t>(z.T / 4, 64);  // in 4-slot buckets
+    Q.ap = make_acc_prog(q.schema, Q.L);
     lds_need = std::max<size_t>(lds_need, Q.L.bytes);
     // global table; its status words live in the batch's flag block
     {

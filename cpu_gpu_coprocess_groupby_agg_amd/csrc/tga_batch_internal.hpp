@@ -16,6 +16,7 @@ struct BQuery {
   Schema s;
   GTable g;
   Layout L;
+  AccProg ap;
   OutDesc od;
   const void *keys;
   uint64_t n;

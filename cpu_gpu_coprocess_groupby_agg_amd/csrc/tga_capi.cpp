@@ -209,7 +209,10 @@ uint32_t lds_slot_bytes(const Schema &s, int key_bytes) {
 Layout make_layout(const Schema &s, int key_bytes, uint32_t slots) {
   Layout L{};
   L.slots = slots;
-  const uint64_t n = (uint64_t)slots + 1;
+  // slots + 2 entries: index `slots` holds the key equal to the empty marker,
+  // index slots + 1 is a discard entry that takes the atomics of rows with no
+  // slot, so the update loop needs no per-lane branches
+  const uint64_t n = (uint64_t)slots + 2;
   uint64_t off = 0;
   auto place = [&](uint64_t bytes) {
     const uint64_t o = off;
@@ -246,7 +249,8 @@ static int ilog2(uint64_t x) {
 // Picks the path, the LDS table size, the partition fanout, tile and grid
 // from (rows, estimated groups) and the device; every explicit option wins.
 // ---------------------------------------------------------------------------
-Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est, const tga_options *opt) {
+Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est, const tga_options *opt,
+              double top_share) {
   tga_options o{};
   if (opt) o = *opt;
   Plan p;
@@ -256,7 +260,7 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   if (est > n && n > 0) est = n;
   // largest power-of-two table fitting one workgroup's LDS
   uint32_t tmax = 64;
-  while ((uint64_t)(tmax * 2 + 1) * sb + 256 <= lds_max) tmax *= 2;
+  while ((uint64_t)(tmax * 2 + 2) * sb + 256 <= lds_max) tmax *= 2;
 
   int path = o.path;
   if (path == TGA_PATH_AUTO) path = (est * 2 <= tmax) ? TGA_PATH_DIRECT : TGA_PATH_PARTITIONED;
@@ -277,7 +281,7 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   } else {
     // the largest table one workgroup's LDS holds (any slot count: the slot
     // is a multiply-shift of the hash), filled to at most 3/4 on average
-    uint32_t tfull = (uint32_t)std::min<uint64_t>(65535, (lds_max - 512) / sb - 1);
+    uint32_t tfull = (uint32_t)std::min<uint64_t>(65535, (lds_max - 512) / sb - 2);
     tfull &= ~63u;
     uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
@@ -303,6 +307,10 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   p.layout = make_layout(q.schema, q.key_bytes, p.table_slots);
   p.nbits = ilog2((uint64_t)p.table_slots + 1);
   p.max_probe = (int)std::min<uint32_t>(p.table_slots / 4, 64);  // in buckets
+  // Wave folding pays when one key is a large share of the rows (a wave then
+  // holds several copies of it); otherwise the per-row ballots cost more than
+  // the LDS spends serialising the rare same-address lanes.
+  p.fold = o.wave_fold == 1 ? 1 : o.wave_fold == 2 ? 0 : (top_share >= 1.0 / 16 ? 1 : 0);
   // global table at load <= 1/2 of the estimate, at most 2 n slots
   uint64_t gs = next_pow2(std::max<uint64_t>(1024, est * 2));
   gs = std::min<uint64_t>(gs, next_pow2(std::max<uint64_t>(1024, 2 * n)));
@@ -318,6 +326,7 @@ static void plan_to_options(const Plan &p, tga_options &o) {
   o.tile_rows = (int32_t)p.tile_rows;
   o.workgroups = (int32_t)p.workgroups;
   o.block_threads = (int32_t)p.block;
+  o.wave_fold = p.fold ? 1 : 2;
   o.chunk_rows = (int64_t)p.chunk_rows;
   o.est_groups = (int64_t)(p.gslots / 2);
 }
@@ -325,17 +334,20 @@ static void plan_to_options(const Plan &p, tga_options &o) {
 // ---------------------------------------------------------------------------
 // cardinality estimate
 // ---------------------------------------------------------------------------
-uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n) {
+uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_share) {
+  if (top_share) *top_share = 0.0;
   if (m == 0 || n == 0) return 1;
   std::sort(v, v + m);
-  uint64_t f1 = 0, d = 0;
+  uint64_t f1 = 0, d = 0, top = 0;
   for (uint64_t i = 0; i < m;) {
     uint64_t j = i;
     while (j < m && v[j] == v[i]) ++j;
     if (j - i == 1) ++f1;
+    top = std::max<uint64_t>(top, j - i);
     ++d;
     i = j;
   }
+  if (top_share) *top_share = (double)top / (double)m;
   if (m >= n) return d;  // the sample is the whole input
   // GEE (Charikar et al. 2000): sqrt(n/m) f1 + (d - f1); good under skew
   const double gee = std::sqrt((double)n / (double)m) * (double)f1 + (double)(d - f1);
@@ -348,7 +360,9 @@ uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n) {
   return (uint64_t)std::min<double>((double)n, std::max<double>((double)d, e));
 }
 
-tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est) {
+tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est,
+                           double *top_share) {
+  if (top_share) *top_share = 0.0;
   if (n == 0) { *est = 1; return TGA_OK; }
   const uint64_t m = std::min<uint64_t>(n, 16384);
   DBuf buf;
@@ -357,7 +371,7 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   std::vector<uint64_t> h(m);
   TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *est = estimate_from_sample(h.data(), m, n);
+  *est = estimate_from_sample(h.data(), m, n, top_share);
   return TGA_OK;
 }
 
@@ -412,7 +426,8 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
     TGA_HIP(ctx, launch_plan_direct(ap, n, (uint32_t)nch, st));
     const unsigned grid = (unsigned)std::min<uint64_t>(plan.workgroups, nch);
     TGA_HIP(ctx, timed(ctx, "agg_direct", [&] {
-      return launch_agg(q.key_bytes, d_keys, s, ap, plan.layout, g, plan.nbits, plan.max_probe, grid, plan.block, st);
+      return launch_agg(q.key_bytes, d_keys, s, ap, plan.layout, g, plan.nbits, plan.max_probe, grid, plan.block, st,
+                        0, nullptr, nullptr, plan.fold);
     }));
   } else if (n > 0) {
     // partitioned path, in super-chunks of < 2^31 rows (u32 offsets)
@@ -458,7 +473,7 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
       for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
       TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
         return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
-                          plan.block, st);
+                          plan.block, st, 0, nullptr, nullptr, plan.fold);
       }));
       keep.push_back(std::move(planbuf));
       keep.push_back(std::move(cnt));
@@ -576,7 +591,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
   TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
     return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
-                      plan.block, st, 1, &od, emit_count);
+                      plan.block, st, 1, &od, emit_count, plan.fold);
   }));
   // the global table's groups follow the emitted ones
   const uint64_t ctiles = compact_tiles(g.S + 1);
@@ -746,11 +761,12 @@ tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *va
   if (st) return st;
   for (int c = 0; c < n_values; ++c) q.schema.cols[c] = sg.vals[c].data;
   uint64_t est = opt && opt->est_groups > 0 ? (uint64_t)opt->est_groups : 0;
+  double top = 0.0;
   if (!est) {
-    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est);
+    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est, &top);
     if (st) return st;
   }
-  plan = autotune(ctx, q, (uint64_t)n_rows, est, opt);
+  plan = autotune(ctx, q, (uint64_t)n_rows, est, opt, top);
   return TGA_OK;
 }
 }  // namespace

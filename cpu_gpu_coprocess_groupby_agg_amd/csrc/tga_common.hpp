@@ -84,6 +84,21 @@ struct Layout {
   uint32_t bytes;
 };
 
+
+// The accumulator program of one table in a few words: all the row loop of
+// the aggregation kernels needs, so that loop keeps a handful of scalar
+// registers live instead of the whole Schema / table descriptors.
+enum : uint32_t { MAP_RAW = 0, MAP_U64 = 1, MAP_F64 = 2 };
+struct AccProg {
+  uint32_t nacc;
+  uint32_t kinds;          // 4 bits per accumulator: AccKind
+  uint32_t cols;           // 2 bits per accumulator: its value column
+  uint32_t maps;           // 2 bits per accumulator: operand map (MAP_*)
+  uint32_t ncols;
+  uint32_t coldt;          // 4 bits per value column: Dtype
+  uint32_t off[kMaxAccs];  // LDS byte offset of each accumulator array
+};
+
 TGA_HD int dtype_bytes(int dt) { return (dt == DT_U32 || dt == DT_I32) ? 4 : 8; }
 
 TGA_HD int acc_lds_bytes(int kind) {
@@ -131,6 +146,24 @@ TGA_HD uint64_t hash_global_slot(uint64_t h, uint64_t slots) {
 // Order-preserving maps onto int64 for MIN/MAX.
 TGA_HD int64_t f64_bits_to_ordered(uint64_t b) { return (int64_t)(b ^ ((uint64_t)((int64_t)b >> 63) & 0x7fffffffffffffffULL)); }
 TGA_HD uint64_t ordered_to_f64_bits(int64_t o) { return (uint64_t)o ^ ((uint64_t)(o >> 63) & 0x7fffffffffffffffULL); }
+
+// Builds the accumulator program of a schema over one table layout.
+TGA_HD AccProg make_acc_prog(const Schema &s, const Layout &L) {
+  AccProg p{};
+  p.nacc = (uint32_t)s.nacc;
+  p.ncols = (uint32_t)s.ncols;
+  for (int c = 0; c < s.ncols && c < kMaxCols; ++c) p.coldt |= (uint32_t)s.col_dtype[c] << (4 * c);
+  for (int a = 0; a < s.nacc && a < kMaxAccs; ++a) {
+    const Acc &ac = s.acc[a];
+    p.kinds |= (uint32_t)ac.kind << (4 * a);
+    p.cols |= (uint32_t)(ac.col & 3) << (2 * a);
+    uint32_t m = MAP_RAW;
+    if (ac.kind == ACC_MIN_I64 || ac.kind == ACC_MAX_I64) m = ac.col_dtype == DT_U64 ? MAP_U64 : ac.col_dtype == DT_F64 ? MAP_F64 : MAP_RAW;
+    p.maps |= m << (2 * a);
+    p.off[a] = L.acc_off[a];
+  }
+  return p;
+}
 
 // Splits a double into at most three signed 32-bit chunks of its fixed-point
 // image; returns the first limb index, or -1 when nothing is added.  Sets

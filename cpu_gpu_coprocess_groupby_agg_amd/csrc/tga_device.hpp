@@ -105,8 +105,9 @@ __device__ __forceinline__ uint64_t acc_fold(int kind, uint64_t a, uint64_t b) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS table: keys[slots+1] plus one SoA array per accumulator; index `slots`
-// holds the key equal to the empty marker.
+// LDS table: keys[slots+2] plus one SoA array per accumulator (fp64 SUM limbs
+// at a stride of slots+1); index `slots` holds the key equal to the empty
+// marker, index slots+1 is a discard entry for rows that have no slot.
 // ---------------------------------------------------------------------------
 
 template <typename K>
@@ -484,6 +485,228 @@ __device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schem
           } else {
             lds_apply(t, a, kind, slot, ops[a]);
           }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row loop for integer accumulators
+// ---------------------------------------------------------------------------
+
+// Raw 64-bit image of value column c at `row` (dtype packed in coldt, 4 bits
+// per column).  VW = 4 or 8: every value column has that width, so the load
+// is branch-free; VW = 0: the width is read from coldt.
+template <int VW>
+__device__ __forceinline__ uint64_t load_col(const void *const *cols, uint32_t coldt, int c, uint64_t row) {
+  const uint32_t dt = (coldt >> (4 * c)) & 15u;
+  if constexpr (VW == 4) {
+    const uint32_t v = ((const uint32_t *)cols[c])[row];
+    return dt == DT_I32 ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
+  } else if constexpr (VW == 8) {
+    return ((const uint64_t *)cols[c])[row];
+  } else {
+    if (dt == DT_U32) return (uint64_t)((const uint32_t *)cols[c])[row];
+    if (dt == DT_I32) return (uint64_t)(int64_t)((const int32_t *)cols[c])[row];
+    return ((const uint64_t *)cols[c])[row];
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ uint64_t pick_col(const uint64_t *raw, uint32_t c) {
+  if constexpr (NC == 1) return raw[0];
+  else return c == 0 ? raw[0] : c == 1 ? raw[1] : c == 2 ? raw[2] : raw[3];
+}
+
+// Operand of accumulator `kind` (operand map `map`) for one raw cell.
+__device__ __forceinline__ uint64_t prog_operand(uint32_t kind, uint32_t map, uint64_t raw) {
+  if (kind == ACC_COUNT) return 1;
+  if (map == MAP_U64) return raw ^ (1ULL << 63);
+  if (map == MAP_F64) return (uint64_t)f64_bits_to_ordered(raw);
+  return raw;
+}
+
+// One row whose key found no LDS slot goes to the global table.  Out of line
+// on purpose: the row loop then carries none of the global-table or schema
+// descriptors (they are read here, from `g` and `s`).
+template <typename K>
+__device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, uint64_t h, uint64_t raw0,
+                                       uint64_t raw1, uint64_t raw2, uint64_t raw3) {
+  const int64_t gs = g_find_or_insert<K>(*g, key, h);
+  if (gs < 0) return;
+  const uint64_t cell[kMaxCols] = {raw0, raw1, raw2, raw3};
+  for (int a = 0; a < s->nacc; ++a) g_apply(*g, a, s->acc[a].kind, gs, acc_operand(*s, a, cell));
+}
+
+// Aggregates rows [b, e) into the LDS table for schemas without fp64 SUM.
+//
+// Per wave and step, U rows per lane: the keys' first buckets are read for
+// all U rows before any is resolved (U LDS round trips in flight), keys not
+// in their first bucket probe on / insert, rows with no slot go to the global
+// table, and the accumulators are updated with return-less LDS atomics, one
+// uniform dispatch per accumulator per step.
+//
+// FOLD: before the atomics, lanes of a wave holding the same slot are found
+// with one ballot per slot bit and folded into their lowest lane, so each
+// slot gets one atomic per wave.  That pays off only when one key fills a
+// large share of the rows (heavy hitters); with many groups, duplicate keys
+// inside a wave are rare and the LDS serialises the few same-address lanes
+// itself, so the tuner compiles the loop without it.
+template <typename K, int NC, int VW, bool FOLD>
+__device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b,
+                                             uint64_t e, const LdsTable<K> &t, const AccProg &ap, int nbits,
+                                             int max_probe, const GTable *gsp, const Schema *ssp) {
+  constexpr int U = 4;
+  const int lane = (int)__lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  const int nw = (int)(blockDim.x >> 6);
+  const uint32_t S = t.L.slots;
+  const uint32_t nb = S >> 2;
+  K *tk = t.keys();
+  char *tb = t.base;
+  const uint32_t ncols = ap.ncols, coldt = ap.coldt;
+  for (uint64_t base = b + (uint64_t)wave * kWave * U; base < e; base += (uint64_t)nw * kWave * U) {
+    K kk[U];
+    uint64_t raw[U][NC];
+    uint64_t hh[U];
+    int slot[U];
+    // loads: rows past the end re-read the last row (in bounds, no branch),
+    // so all U key and value loads are in flight together
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t row = base + (uint64_t)u * kWave + lane;
+      const uint64_t r = row < e ? row : e - 1;
+      kk[u] = keys[r];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
+    }
+    // first buckets of all U rows, read before any is resolved
+    uint32_t s0[U];
+    K k4[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      hh[u] = key_hash((uint64_t)kk[u]);
+      s0[u] = hash_lds_slot(hh[u], nb) << 2;
+      lds_bucket<K>(tk, s0[u], k4[u]);
+    }
+    bool sent = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool act = base + (uint64_t)u * kWave + lane < e;
+      int m = -1;
+#pragma unroll
+      for (int j = 3; j >= 0; --j) m = k4[u][j] == kk[u] ? (int)(s0[u] + j) : m;
+      const bool empty_key = kk[u] == KeyTraits<K>::kEmpty;
+      sent |= act && empty_key;
+      slot[u] = !act ? -1 : empty_key ? (int)S : m;
+    }
+    if (sent) *t.sentinel = 1;
+    // keys past their first bucket, and first sightings: probe on / insert
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t row = base + (uint64_t)u * kWave + lane;
+      if (row < e && slot[u] < 0) slot[u] = lds_find_or_insert(t, kk[u], hh[u], max_probe);
+    }
+    // rows without an LDS slot (table full): the global table
+    bool spill = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) spill |= (base + (uint64_t)u * kWave + lane < e) && slot[u] < 0;
+    if (__ballot(spill)) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)
+          spill_row<K>(gsp, ssp, kk[u], hh[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
+                       NC > 2 ? raw[u][NC > 2 ? 2 : 0] : 0, NC > 3 ? raw[u][NC > 3 ? 3 : 0] : 0);
+      }
+    }
+    if constexpr (FOLD) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool on = slot[u] >= 0;
+        if (!__ballot(on)) continue;
+        const uint64_t m = match_mask((uint32_t)slot[u], nbits, on);
+        const bool leader = on && ((m & lanemask_lt()) == 0);
+        uint64_t rest = leader ? (m & (m - 1)) : 0ULL;
+        uint64_t ops[kMaxAccs];
+#pragma unroll
+        for (int a = 0; a < kMaxAccs; ++a) {
+          ops[a] = 0;
+          if ((uint32_t)a < ap.nacc)
+            ops[a] = prog_operand((ap.kinds >> (4 * a)) & 15u, (ap.maps >> (2 * a)) & 3u,
+                                  pick_col<NC>(raw[u], (ap.cols >> (2 * a)) & 3u));
+        }
+        while (__ballot(rest != 0)) {
+          const int src = rest ? (int)__builtin_ctzll(rest) : lane;
+#pragma unroll
+          for (int a = 0; a < kMaxAccs; ++a) {
+            if ((uint32_t)a >= ap.nacc) break;
+            const uint64_t o = shfl64(ops[a], src);
+            if (rest) ops[a] = acc_fold((int)((ap.kinds >> (4 * a)) & 15u), ops[a], o);
+          }
+          if (rest) rest &= rest - 1;
+        }
+        // leaders update their slot with the folded operands; every other
+        // lane updates the discard entry
+        {
+          const int sl = leader ? slot[u] : (int)S + 1;
+#pragma unroll
+          for (int a = 0; a < kMaxAccs; ++a) {
+            if ((uint32_t)a >= ap.nacc) break;
+            const uint32_t kind = (ap.kinds >> (4 * a)) & 15u;
+            char *arr = tb + ap.off[a];
+            switch (kind) {
+              case ACC_COUNT: atomicAdd((uint32_t *)arr + sl, (uint32_t)ops[a]); break;
+              case ACC_SUM_I: atomicAdd((unsigned long long *)arr + sl, (unsigned long long)ops[a]); break;
+              case ACC_MIN_I32: atomicMin((int *)arr + sl, (int)(int32_t)ops[a]); break;
+              case ACC_MAX_I32: atomicMax((int *)arr + sl, (int)(int32_t)ops[a]); break;
+              case ACC_MIN_I64: atomicMin((long long *)arr + sl, (long long)ops[a]); break;
+              case ACC_MAX_I64: atomicMax((long long *)arr + sl, (long long)ops[a]); break;
+              default: break;
+            }
+          }
+        }
+      }
+    } else {
+      // rows without a slot (past the end, or spilled) update the discard entry
+      int sl[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) sl[u] = slot[u] >= 0 ? slot[u] : (int)S + 1;
+#pragma unroll
+      for (int a = 0; a < kMaxAccs; ++a) {
+        if ((uint32_t)a >= ap.nacc) break;
+        const uint32_t kind = (ap.kinds >> (4 * a)) & 15u;
+        const uint32_t col = (ap.cols >> (2 * a)) & 3u;
+        const uint32_t map = (ap.maps >> (2 * a)) & 3u;
+        char *arr = tb + ap.off[a];
+        switch (kind) {
+          case ACC_COUNT:
+#pragma unroll
+            for (int u = 0; u < U; ++u) atomicAdd((uint32_t *)arr + sl[u], 1u);
+            break;
+          case ACC_SUM_I:
+#pragma unroll
+            for (int u = 0; u < U; ++u) atomicAdd((unsigned long long *)arr + sl[u], (unsigned long long)pick_col<NC>(raw[u], col));
+            break;
+          case ACC_MIN_I32:
+#pragma unroll
+            for (int u = 0; u < U; ++u) atomicMin((int *)arr + sl[u], (int)(int32_t)pick_col<NC>(raw[u], col));
+            break;
+          case ACC_MAX_I32:
+#pragma unroll
+            for (int u = 0; u < U; ++u) atomicMax((int *)arr + sl[u], (int)(int32_t)pick_col<NC>(raw[u], col));
+            break;
+          case ACC_MIN_I64:
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              atomicMin((long long *)arr + sl[u], (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col)));
+            break;
+          case ACC_MAX_I64:
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              atomicMax((long long *)arr + sl[u], (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col)));
+            break;
+          default: break;
         }
       }
     }

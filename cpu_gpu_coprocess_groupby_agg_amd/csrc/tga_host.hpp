@@ -145,18 +145,21 @@ struct Plan {
   Layout layout{};
   int nbits = 0;
   int max_probe = 0;
+  int fold = 0;  // fold equal keys of a wave before the LDS atomics
 };
 
 // Autotuner: plan from (rows, estimated groups, schema, device), honouring
 // any explicit field of `opt`.
 Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint64_t est_groups,
-              const tga_options *opt);
+              const tga_options *opt, double top_share = 0.0);
 
 // Cardinality estimate from a strided key sample (GEE estimator,
 // Charikar et al. 2000).  Needs the context lock.
-tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est);
+tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est,
+                           double *top_share = nullptr);
 // Distinct-count estimate of n rows from a sample of m keys (sorted in place).
-uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n);
+// *top_share (optional): the largest share of the sample one key takes.
+uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_share = nullptr);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

@@ -72,14 +72,21 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 // Persistent: workgroups pull work items from plan.ticket until none remain.
 // Item -> (segment p, slice j of nch) through plan.cstart; rows of segment p
 // are [pstart[p], pstart[p+1]).
-template <typename K, bool FX, int NC>
+template <typename K, bool FX, int NC, int VW, bool FOLD>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
-                                              GTable g, int nbits, int max_probe, int emit, OutDesc od,
+                                              GTable g, AccProg ap, int nbits, int max_probe, int emit, OutDesc od,
                                               uint64_t *emit_count) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t s_item, s_sentinel;
   __shared__ uint32_t scratch[1024 / kWave + 1];
   __shared__ uint64_t s_base;
+  // descriptors the out-of-line global-table path reads
+  __shared__ GTable s_g;
+  __shared__ Schema s_s;
+  if (threadIdx.x == 0) {
+    s_g = g;
+    s_s = s;
+  }
   LdsTable<K> t{smem, L, &s_sentinel};
   const uint32_t total = plan.cstart[plan.P];
   for (;;) {
@@ -98,7 +105,14 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
     lds_init(t, s);
     __syncthreads();
-    agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe);
+    if constexpr (FX) {
+      agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe);
+    } else {
+      const void *cols[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
+      agg_rows_int<K, NC, VW, FOLD>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s);
+    }
     __syncthreads();
     if (emit && nch == 1) lds_emit<K>(t, s, od, emit_count, g.flags, scratch, &s_base);
     else lds_flush<K>(t, s, g);
@@ -496,37 +510,46 @@ static hipError_t set_lds_limit(const void *fn, size_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename K, bool FX, int NC>
+template <typename K, bool FX, int NC, int VW, bool FOLD>
 static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan &plan, const Layout &L, const GTable &g,
                                int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st, int emit,
                                const OutDesc &o, uint64_t *emit_count) {
   const size_t lds = L.bytes;
-  hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC, VW, FOLD>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_agg<K, FX, NC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g, nbits,
-                     max_probe, emit, o, emit_count);
+  const AccProg ap = make_acc_prog(s, L);
+  hipLaunchKernelGGL((k_agg<K, FX, NC, VW, FOLD>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g,
+                     ap, nbits, max_probe, emit, o, emit_count);
   return hipGetLastError();
 }
 
-// One kernel per (key width, fp64 SUM present, column count class): each
-// variant gets its own register allocation.
+// One kernel per (key width, fp64 SUM present, value columns, wave folding):
+// each variant gets its own register allocation, and a single value column
+// of a known width is read without branches.  fp64 SUM accumulators always
+// fold (their limb vectors make per-row atomics costly).
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
-                      int emit, const OutDesc *od, uint64_t *emit_count) {
+                      int emit, const OutDesc *od, uint64_t *emit_count, int fold) {
   OutDesc o{};
   if (od) o = *od;
   const bool fx = s.nfx > 0;
   const bool one = s.ncols <= 1;
-#define TGA_AGG_CASE(K)                                                                                     \
-  if (fx) return launch_agg_t<K, true, kMaxCols>(keys, s, plan, L, g, nbits, max_probe, grid, block, st,   \
-                                                 emit, o, emit_count);                                     \
-  if (one) return launch_agg_t<K, false, 1>(keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, \
-                                            emit_count);                                                    \
-  return launch_agg_t<K, false, kMaxCols>(keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o,  \
-                                          emit_count);
+  const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
+#define TGA_AGG_ARGS keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, emit_count
+#define TGA_AGG_CASE(K)                                                                         \
+  if (fx) return launch_agg_t<K, true, kMaxCols, 0, true>(TGA_AGG_ARGS);                        \
+  if (one) {                                                                                    \
+    if (w8) return fold ? launch_agg_t<K, false, 1, 8, true>(TGA_AGG_ARGS)                      \
+                        : launch_agg_t<K, false, 1, 8, false>(TGA_AGG_ARGS);                    \
+    return fold ? launch_agg_t<K, false, 1, 4, true>(TGA_AGG_ARGS)                              \
+                : launch_agg_t<K, false, 1, 4, false>(TGA_AGG_ARGS);                            \
+  }                                                                                             \
+  return fold ? launch_agg_t<K, false, kMaxCols, 0, true>(TGA_AGG_ARGS)                         \
+              : launch_agg_t<K, false, kMaxCols, 0, false>(TGA_AGG_ARGS);
   if (key_bytes == 4) { TGA_AGG_CASE(uint32_t) }
   TGA_AGG_CASE(uint64_t)
 #undef TGA_AGG_CASE
+#undef TGA_AGG_ARGS
 }
 
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st) {

@@ -1,0 +1,233 @@
+// Runs the plain operator (tga_groupby) on the CPU execution model over a set
+// of cases that reach every aggregation-kernel variant -- direct and
+// partitioned paths, 32/64-bit keys, one or several value columns of each
+// width, wave folding on and off, the LDS-full spill path, the key equal to
+// the empty marker, exact fp64 SUM -- and checks each result against the
+// oracle (tga_o_groupby): integer aggregates bit-exact, fp64 SUM within 1e-12
+// relative.  Usage: plain_driver [case-name-substring]
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/tga_groupby.h"
+
+extern "C" int64_t tga_o_groupby(const void *keys, int key_bytes, int64_t n, const void *const *cols,
+                                 const int *col_dtypes, int ncols, const int *agg_ops, const int *agg_cols, int naggs,
+                                 uint64_t *out_keys, void *const *out_aggs, int64_t out_cap);
+extern "C" void tga_o_gen_keys_u32(uint32_t *out, int64_t n, uint64_t distinct, uint64_t seed, uint64_t salt);
+extern "C" void tga_o_gen_keys_u64(uint64_t *out, int64_t n, uint64_t distinct, uint64_t seed, uint64_t salt);
+extern "C" void tga_o_gen_vals_i32(int32_t *out, int64_t n, int64_t lo, int64_t hi, uint64_t seed);
+extern "C" uint64_t tga_o_splitmix(uint64_t seed, uint64_t i);
+
+namespace {
+
+int dt_bytes(int dt) { return (dt == TGA_U32 || dt == TGA_I32) ? 4 : 8; }
+int out_dt(int op, int dt) { return op == TGA_COUNT ? TGA_I64 : op == TGA_SUM ? (dt == TGA_F64 ? TGA_F64 : TGA_I64) : dt; }
+
+struct Case {
+  std::string name;
+  int key_bits;
+  std::vector<uint64_t> keys;
+  std::vector<std::vector<unsigned char>> cols;  // raw bytes
+  std::vector<int> col_dt;
+  std::vector<tga_agg> aggs;
+  tga_options opt;
+};
+
+std::vector<unsigned char> col_i32(int64_t n, int64_t lo, int64_t hi, uint64_t seed) {
+  std::vector<unsigned char> b(n * 4);
+  tga_o_gen_vals_i32((int32_t *)b.data(), n, lo, hi, seed);
+  return b;
+}
+std::vector<unsigned char> col_u64(int64_t n, uint64_t seed) {
+  std::vector<unsigned char> b(n * 8);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t v = tga_o_splitmix(seed, (uint64_t)i);
+    std::memcpy(&b[i * 8], &v, 8);
+  }
+  return b;
+}
+std::vector<unsigned char> col_f64(int64_t n, uint64_t seed) {
+  std::vector<unsigned char> b(n * 8);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t r = tga_o_splitmix(seed, (uint64_t)i);
+    const double v = ((double)(r >> 11) / 9007199254740992.0 - 0.5) * 2000.0;
+    std::memcpy(&b[i * 8], &v, 8);
+  }
+  return b;
+}
+std::vector<uint64_t> keys_u32(int64_t n, uint64_t d, uint64_t seed) {
+  std::vector<uint32_t> k(n);
+  tga_o_gen_keys_u32(k.data(), n, d, seed, 3);
+  return std::vector<uint64_t>(k.begin(), k.end());
+}
+std::vector<uint64_t> keys_u64(int64_t n, uint64_t d, uint64_t seed) {
+  std::vector<uint64_t> k(n);
+  tga_o_gen_keys_u64(k.data(), n, d, seed, 5);
+  return k;
+}
+std::vector<uint64_t> keys_skew(int64_t n, uint64_t seed) {
+  // about half the rows on one key, the rest over 5000 keys
+  std::vector<uint64_t> k(n);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t r = tga_o_splitmix(seed, (uint64_t)i);
+    k[i] = (r & 1) ? 0x1234567890ULL : (r >> 1) % 5000 * 0x9E3779B97F4A7C15ULL;
+  }
+  return k;
+}
+
+tga_options opts(int path, int fold, int pbits = 0, int slots = 0) {
+  tga_options o;
+  tga_options_default(&o);
+  o.path = path;
+  o.wave_fold = fold;
+  o.partition_bits = pbits;
+  o.table_slots = slots;
+  return o;
+}
+
+int run_case(tga_context *ctx, const Case &c) {
+  const int64_t n = (int64_t)c.keys.size();
+  const int kb = c.key_bits / 8;
+  std::vector<unsigned char> kbuf(std::max<int64_t>(1, n) * kb);
+  for (int64_t i = 0; i < n; ++i) {
+    if (kb == 4) { const uint32_t v = (uint32_t)c.keys[i]; std::memcpy(&kbuf[i * 4], &v, 4); }
+    else std::memcpy(&kbuf[i * 8], &c.keys[i], 8);
+  }
+  const tga_column key{kbuf.data(), kb == 4 ? TGA_U32 : TGA_U64, TGA_HOST};
+  std::vector<tga_column> vals;
+  for (size_t j = 0; j < c.cols.size(); ++j) vals.push_back(tga_column{(void *)c.cols[j].data(), c.col_dt[j], TGA_HOST});
+  tga_result *r = nullptr;
+  tga_status st = tga_groupby(ctx, &key, vals.data(), (int32_t)vals.size(), n, c.aggs.data(), (int32_t)c.aggs.size(),
+                              &c.opt, &r);
+  if (st) {
+    std::printf("%s: tga_groupby -> %d (%s)\n", c.name.c_str(), (int)st, tga_last_error(ctx));
+    return 1;
+  }
+  int64_t g = 0;
+  tga_result_num_groups(r, &g);
+  // oracle
+  const int na = (int)c.aggs.size();
+  std::vector<uint64_t> ok(std::max<int64_t>(1, n));
+  std::vector<std::vector<unsigned char>> oa(na, std::vector<unsigned char>(std::max<int64_t>(1, n) * 8));
+  std::vector<const void *> cp;
+  for (auto &col : c.cols) cp.push_back(col.data());
+  std::vector<int> ops(na), acs(na);
+  std::vector<void *> outs(na);
+  for (int a = 0; a < na; ++a) { ops[a] = c.aggs[a].op; acs[a] = c.aggs[a].column; outs[a] = oa[a].data(); }
+  std::vector<uint64_t> kw(c.keys);  // key bits, widened
+  const int64_t G = tga_o_groupby(kw.data(), 8, n, cp.data(), c.col_dt.data(), (int)c.cols.size(), ops.data(),
+                                  acs.data(), na, ok.data(), outs.data(), (int64_t)ok.size());
+  int bad = 0;
+  if (g != G) {
+    std::printf("%s: %lld groups, oracle %lld\n", c.name.c_str(), (long long)g, (long long)G);
+    tga_result_free(r);
+    return 1;
+  }
+  std::vector<unsigned char> gk(std::max<int64_t>(1, g) * kb);
+  if (g) tga_result_copy_to_host(r, -1, gk.data(), g);
+  std::vector<std::vector<unsigned char>> ga(na);
+  for (int a = 0; a < na; ++a) {
+    const int ob = dt_bytes(out_dt(c.aggs[a].op, c.aggs[a].op == TGA_COUNT ? TGA_I64 : c.col_dt[c.aggs[a].column]));
+    ga[a].resize(std::max<int64_t>(1, g) * ob);
+    if (g) tga_result_copy_to_host(r, a, ga[a].data(), g);
+  }
+  tga_result_free(r);
+  auto key_at = [&](int64_t i) -> uint64_t {
+    if (kb == 4) { uint32_t v; std::memcpy(&v, &gk[i * 4], 4); return v; }
+    uint64_t v; std::memcpy(&v, &gk[i * 8], 8); return v;
+  };
+  std::vector<int64_t> ord(g);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return key_at(x) < key_at(y); });
+  for (int64_t i = 0; i < g && bad < 5; ++i) {
+    const int64_t j = ord[i];
+    if (key_at(j) != ok[i]) { std::printf("%s: key %lld differs\n", c.name.c_str(), (long long)i); ++bad; continue; }
+    for (int a = 0; a < na; ++a) {
+      const int cdt = c.aggs[a].op == TGA_COUNT ? TGA_I64 : c.col_dt[c.aggs[a].column];
+      const int odt = out_dt(c.aggs[a].op, cdt);
+      const int ob = dt_bytes(odt);
+      if (c.aggs[a].op == TGA_SUM && odt == TGA_F64) {
+        double x, y;
+        std::memcpy(&x, &ga[a][j * 8], 8);
+        std::memcpy(&y, &oa[a][i * 8], 8);
+        if (!(std::fabs(x - y) <= 1e-12 * std::max(std::fabs(y), 1e-300))) {
+          std::printf("%s: agg %d row %lld: %.17g vs %.17g\n", c.name.c_str(), a, (long long)i, x, y);
+          ++bad;
+        }
+      } else if (std::memcmp(&ga[a][j * ob], &oa[a][i * ob], ob) != 0) {
+        std::printf("%s: agg %d row %lld differs\n", c.name.c_str(), a, (long long)i);
+        ++bad;
+      }
+    }
+  }
+  std::printf("%s: %lld rows, %lld groups %s\n", c.name.c_str(), (long long)n, (long long)g, bad ? "FAIL" : "ok");
+  return bad ? 1 : 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  const std::string filter = argc > 1 ? argv[1] : "";
+  tga_context *ctx = nullptr;
+  if (tga_context_create(0, 0, &ctx)) { std::fprintf(stderr, "context\n"); return 1; }
+  const tga_agg a4[4] = {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}};
+  std::vector<Case> cases;
+  const int64_t N = 200000;
+  for (int fold : {1, 2}) {
+    const std::string f = fold == 1 ? "fold" : "nofold";
+    cases.push_back({"direct_u32_" + f, 32, keys_u32(N, 1000, 1), {col_i32(N, -(1 << 20), 1 << 20, 2)}, {TGA_I32},
+                     {a4[0], a4[1]}, opts(TGA_PATH_DIRECT, fold)});
+    cases.push_back({"part_u64_" + f, 64, keys_u64(N, 40000, 3), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 4)},
+                     {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, fold)});
+    cases.push_back({"part_u32_w8_" + f, 32, keys_u32(N, 30000, 5), {col_u64(N, 6)}, {TGA_I64},
+                     {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, fold)});
+    cases.push_back({"spill_u64_" + f, 64, keys_u64(N, 20000, 7), {col_i32(N, -1000, 1000, 8)}, {TGA_I32},
+                     {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, fold, 2, 256)});
+    cases.push_back({"direct_spill_u32_" + f, 32, keys_u32(N, 5000, 9), {col_i32(N, -1000, 1000, 10)}, {TGA_I32},
+                     {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_DIRECT, fold, 0, 256)});
+    cases.push_back({"skew_u64_" + f, 64, keys_skew(N, 11), {col_i32(N, -(1 << 30), 1 << 30, 12)}, {TGA_I32},
+                     {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_AUTO, fold)});
+    // several columns of every dtype
+    Case m{"multi_" + f, 64, keys_u64(N, 3000, 13),
+           {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 14), col_u64(N, 15), col_u64(N, 16), col_f64(N, 17)},
+           {TGA_I32, TGA_U64, TGA_I64, TGA_F64},
+           {{TGA_SUM, 0}, {TGA_MIN, 1}, {TGA_MAX, 1}, {TGA_MIN, 2}, {TGA_MAX, 2}, {TGA_MIN, 3}, {TGA_MAX, 3},
+            {TGA_COUNT, 0}},
+           opts(TGA_PATH_AUTO, fold)};
+    cases.push_back(m);
+  }
+  {
+    // keys equal to the empty marker and to zero, all paths
+    std::vector<uint64_t> k = keys_u64(N, 7000, 19);
+    for (int64_t i = 0; i < N; i += 7) k[i] = ~0ULL;
+    for (int64_t i = 3; i < N; i += 11) k[i] = 0;
+    for (int path : {TGA_PATH_DIRECT, TGA_PATH_PARTITIONED})
+      cases.push_back({std::string("sentinel_u64_") + (path == TGA_PATH_DIRECT ? "direct" : "part"), 64, k,
+                       {col_i32(N, -5, 5, 20)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(path, 2)});
+    std::vector<uint64_t> k32 = keys_u32(N, 900, 21);
+    for (int64_t i = 0; i < N; i += 5) k32[i] = 0xffffffffULL;
+    cases.push_back({"sentinel_u32", 32, k32, {col_i32(N, -5, 5, 22)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
+                     opts(TGA_PATH_AUTO, 0)});
+  }
+  // exact fp64 SUM (folding kernel variant)
+  cases.push_back({"fp64_sum", 64, keys_u64(N, 2000, 23), {col_f64(N, 24)}, {TGA_F64},
+                   {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}}, opts(TGA_PATH_AUTO, 0)});
+  cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
+                   {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
+  int bad = 0, ran = 0;
+  for (const Case &c : cases) {
+    if (!filter.empty() && c.name.find(filter) == std::string::npos) continue;
+    std::fflush(stdout);
+    bad += run_case(ctx, c);
+    ++ran;
+  }
+  tga_context_destroy(ctx);
+  std::printf(bad ? "FAIL %d of %d\n" : "OK %d\n", bad ? bad : ran, ran);
+  return bad ? 1 : 0;
+}

@@ -14,6 +14,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 EMU = os.path.join(HERE, "emu")
 DRIVER = os.path.join(EMU, "_build", "batch_driver")
+PLAIN = os.path.join(EMU, "_build", "plain_driver")
 
 
 @pytest.fixture(scope="module")
@@ -36,3 +37,14 @@ def test_batch_mix_many_resident_workgroups(driver):
     env = dict(os.environ, EMU_WATCHDOG_S="120", EMU_CUS="256", EMU_RESIDENT="32")
     r = subprocess.run([driver, "0", "1"], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+
+
+def test_plain_operator_kernel_variants_on_cpu_model(driver):
+    """Every aggregation-kernel variant (direct / partitioned, 32/64-bit keys,
+    one or four value columns, 4- and 8-byte values, wave folding on and off,
+    the LDS-full spill path, the empty-marker key, exact fp64 SUM) against
+    the oracle."""
+    env = dict(os.environ, EMU_WATCHDOG_S="120")
+    r = subprocess.run([PLAIN], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert r.stdout.strip().splitlines()[-1].startswith("OK")
