@@ -280,14 +280,17 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(1, (n + W - 1) / W);
   } else {
     // the largest table one workgroup's LDS holds (any slot count: the slot
-    // is a multiply-shift of the hash), filled to at most 3/4 on average
+    // is a multiply-shift of the hash), filled to about 2/5 on average: the
+    // first 4-slot bucket then holds nearly every key, so almost no row
+    // leaves the branch-free lookup (at 3/4 fill the aggregation runs ~4x
+    // slower on the 2^28-row, 2^20-group workload)
     uint32_t tfull = (uint32_t)std::min<uint64_t>(65535, (lds_max - 512) / sb - 2);
     tfull &= ~63u;
     uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
     int pbits = o.partition_bits;
     if (pbits <= 0) {
-      const uint64_t P = next_pow2((est * 4 + 3 * (uint64_t)T - 1) / (3 * (uint64_t)T));
+      const uint64_t P = next_pow2((est * 5 + 2 * (uint64_t)T - 1) / (2 * (uint64_t)T));
       pbits = std::max(1, ilog2(P));
     }
     pbits = std::min(pbits, 12);
@@ -311,6 +314,9 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   // holds several copies of it); otherwise the per-row ballots cost more than
   // the LDS spends serialising the rare same-address lanes.
   p.fold = o.wave_fold == 1 ? 1 : o.wave_fold == 2 ? 0 : (top_share >= 1.0 / 16 ? 1 : 0);
+  p.opt = o;
+  p.top_share = top_share;
+  p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < 12;
   // global table at load <= 1/2 of the estimate, at most 2 n slots
   uint64_t gs = next_pow2(std::max<uint64_t>(1024, est * 2));
   gs = std::min<uint64_t>(gs, next_pow2(std::max<uint64_t>(1024, 2 * n)));
@@ -515,19 +521,24 @@ tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, c
 // Partitioned path with direct emission (n < 2^31): partitions that one work
 // item holds whole write final rows straight to the output; only split
 // partitions and LDS spills go through the global table, sized for them.
+// One partitioned execution.  *overflow: the global table or the output was
+// too small (retry with larger ones).  *replan_est > 0: partitions processed
+// whole overflowed their LDS table and were abandoned; the value estimates
+// the group count for a finer partitioning (retry with it).
 static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, Plan &plan,
                                   bool full_table, PlainOut *out, void *ext_keys, void *const *ext_cols,
-                                  uint64_t ext_cap, uint64_t *groups, bool *overflow) {
+                                  uint64_t ext_cap, uint64_t *groups, bool *overflow, uint64_t *replan_est) {
+  *replan_est = 0;
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
   const uint32_t P = 1u << plan.pbits;
   const uint32_t ntiles = (uint32_t)((n + plan.tile_rows - 1) / plan.tile_rows);
   const uint64_t cells = (uint64_t)P * ntiles;
-  DBuf cnt, offs, desc, pkeys, planbuf, counters;
+  DBuf cnt, offs, desc, pkeys, planbuf, counters, ovf;
   DBuf pcols[kMaxCols];
   if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
       !pkeys.alloc(ctx, n * q.key_bytes) || !planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64) ||
-      !counters.alloc(ctx, 64))
+      !counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(P + 1) * 8))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   for (int c = 0; c < s.ncols; ++c) {
@@ -540,6 +551,8 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   uint64_t *emit_count = counters.as<uint64_t>();
   TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
   TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
+  const bool may_abandon = plan.may_replan;
+  if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, 0, st));
   TGA_HIP(ctx, timed(ctx, "part_count", [&] {
     return launch_part_count(q.key_bytes, d_keys, n, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
   }));
@@ -591,7 +604,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
   TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
     return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
-                      plan.block, st, 1, &od, emit_count, plan.fold);
+                      plan.block, st, 1, &od, emit_count, plan.fold, may_abandon ? ovf.as<uint32_t>() : nullptr);
   }));
   // the global table's groups follow the emitted ones
   const uint64_t ctiles = compact_tiles(g.S + 1);
@@ -605,10 +618,31 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   }));
   uint32_t hflags[kFlagWords];
   uint64_t hcount = 0;
+  uint32_t novf = 0;
   TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof hflags, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipMemcpyAsync(&hcount, count, 8, hipMemcpyDeviceToHost, st));
+  if (may_abandon) TGA_HIP(ctx, hipMemcpyAsync(&novf, ovf.p, 4, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
   if (ctx->prof.on) ctx->prof.resolve();
+  if (novf) {
+    // Abandoned partitions: a table of T slots was full after r of the
+    // partition's R rows, so it holds about T * R / r groups.  The groups of
+    // the finished partitions are exact (hcount counts the emitted rows).
+    novf = std::min(novf, P);
+    std::vector<uint32_t> list(2 * (size_t)novf);
+    std::vector<uint64_t> pst(P + 1);
+    TGA_HIP(ctx, hipMemcpyAsync(list.data(), ovf.as<uint32_t>() + 2, list.size() * 4, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipMemcpyAsync(pst.data(), ap.pstart, (P + 1) * 8, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipStreamSynchronize(st));
+    double est = (double)hcount;
+    for (uint32_t i = 0; i < novf; ++i) {
+      const uint32_t p = list[2 * i], r = std::max<uint32_t>(1, list[2 * i + 1]);
+      const double R = (double)(pst[p + 1] - pst[p]);
+      est += (double)plan.table_slots * std::max(1.0, R / (double)r);
+    }
+    *replan_est = (uint64_t)std::min<double>((double)n, est);
+    return TGA_OK;
+  }
   *overflow = hflags[FLAG_OVERFLOW] != 0 || (!ext_keys && hcount > cap);
   if (*overflow) {
     // either the spill table or the output was too small: caller retries
@@ -637,17 +671,37 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
   if (plan.path == TGA_PATH_PARTITIONED && n > 0 && n < (1ULL << 31)) {
-    for (int attempt = 0; attempt < 6; ++attempt) {
+    bool grown = false;
+    for (int attempt = 0; attempt < 8; ++attempt) {
       bool overflow = false;
-      tga_status st2 = run_partitioned(ctx, q, d_keys, n, plan, attempt > 0, out, ext_keys, ext_cols, ext_cap, groups,
-                                       &overflow);
+      uint64_t replan = 0;
+      tga_status st2 = run_partitioned(ctx, q, d_keys, n, plan, grown, out, ext_keys, ext_cols, ext_cap, groups,
+                                       &overflow, &replan);
       if (st2) return st2;
+      if (replan) {
+        // the estimate was too low for the fanout: plan again from the
+        // group count the abandoned partitions imply, at least doubling P
+        const int old_bits = plan.pbits;
+        Plan np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &plan.opt, plan.top_share);
+        if (np.path != TGA_PATH_PARTITIONED) np = plan;
+        if (np.pbits <= old_bits) {
+          tga_options o2 = plan.opt;
+          o2.partition_bits = std::min(12, old_bits + 1);
+          o2.est_groups = (int64_t)replan;
+          np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &o2, plan.top_share);
+          np.may_replan = np.pbits < 12;
+          np.opt = plan.opt;
+        }
+        plan = np;
+        continue;
+      }
       if (!overflow) {
         ctx->last_plan = tga_options{};
         plan_to_options(plan, ctx->last_plan);
         return TGA_OK;
       }
       plan.gslots *= 4;  // the estimate was too low: larger tables, run again
+      grown = true;
     }
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table kept overflowing");
   }

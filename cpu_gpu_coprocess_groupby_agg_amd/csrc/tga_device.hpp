@@ -553,10 +553,17 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 // large share of the rows (heavy hitters); with many groups, duplicate keys
 // inside a wave are rare and the LDS serialises the few same-address lanes
 // itself, so the tuner compiles the loop without it.
+//
+// giveup (LDS, two words; null = spill mode): a partition processed whole by
+// one workgroup whose groups do not fit the table is abandoned instead of
+// spilled -- giveup[0] is set, every wave leaves the loop, and giveup[1]
+// receives the rows the workgroup had consumed, from which the host sizes a
+// finer partitioning.
 template <typename K, int NC, int VW, bool FOLD>
 __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b,
                                              uint64_t e, const LdsTable<K> &t, const AccProg &ap, int nbits,
-                                             int max_probe, const GTable *gsp, const Schema *ssp) {
+                                             int max_probe, const GTable *gsp, const Schema *ssp,
+                                             uint32_t *giveup = nullptr) {
   constexpr int U = 4;
   const int lane = (int)__lane_id();
   const int wave = (int)(threadIdx.x >> 6);
@@ -566,7 +573,10 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   K *tk = t.keys();
   char *tb = t.base;
   const uint32_t ncols = ap.ncols, coldt = ap.coldt;
+  uint32_t consumed = 0;
   for (uint64_t base = b + (uint64_t)wave * kWave * U; base < e; base += (uint64_t)nw * kWave * U) {
+    if (giveup && wave_uniform(*(volatile uint32_t *)giveup)) break;
+    consumed += kWave * U;
     K kk[U];
     uint64_t raw[U][NC];
     uint64_t hh[U];
@@ -613,11 +623,15 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
 #pragma unroll
     for (int u = 0; u < U; ++u) spill |= (base + (uint64_t)u * kWave + lane < e) && slot[u] < 0;
     if (__ballot(spill)) {
+      if (giveup) {
+        if (spill) *giveup = 1;
+      } else {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)
-          spill_row<K>(gsp, ssp, kk[u], hh[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
-                       NC > 2 ? raw[u][NC > 2 ? 2 : 0] : 0, NC > 3 ? raw[u][NC > 3 ? 3 : 0] : 0);
+        for (int u = 0; u < U; ++u) {
+          if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)
+            spill_row<K>(gsp, ssp, kk[u], hh[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
+                         NC > 2 ? raw[u][NC > 2 ? 2 : 0] : 0, NC > 3 ? raw[u][NC > 3 ? 3 : 0] : 0);
+        }
       }
     }
     if constexpr (FOLD) {
@@ -711,6 +725,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
       }
     }
   }
+  if (giveup && lane == 0) atomicAdd(giveup + 1, consumed);
 }
 
 template <typename K>

@@ -146,6 +146,11 @@ struct Plan {
   int nbits = 0;
   int max_probe = 0;
   int fold = 0;  // fold equal keys of a wave before the LDS atomics
+  // re-planning: the options and skew the plan came from, and whether the
+  // fanout was the tuner's (so a too-small estimate may raise it)
+  tga_options opt{};
+  double top_share = 0.0;
+  bool may_replan = false;
 };
 
 // Autotuner: plan from (rows, estimated groups, schema, device), honouring

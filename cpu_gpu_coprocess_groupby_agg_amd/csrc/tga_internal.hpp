@@ -32,7 +32,8 @@ hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, h
 // straight to `od` (rows reserved on *emit_count); the others flush into g.
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
-                      int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0);
+                      int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0,
+                      uint32_t *ovf = nullptr);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
 hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
                               const AggPlan &plan, hipStream_t st);

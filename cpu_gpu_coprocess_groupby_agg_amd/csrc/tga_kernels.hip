@@ -75,7 +75,7 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 template <typename K, bool FX, int NC, int VW, bool FOLD>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
                                               GTable g, AccProg ap, int nbits, int max_probe, int emit, OutDesc od,
-                                              uint64_t *emit_count) {
+                                              uint64_t *emit_count, uint32_t *ovf) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t s_item, s_sentinel;
   __shared__ uint32_t scratch[1024 / kWave + 1];
@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
   // descriptors the out-of-line global-table path reads
   __shared__ GTable s_g;
   __shared__ Schema s_s;
+  __shared__ uint32_t s_abort[2];  // table full / rows consumed (abort mode)
   if (threadIdx.x == 0) {
     s_g = g;
     s_s = s;
@@ -103,7 +104,15 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     const uint32_t j = item - plan.cstart[p], nch = plan.cstart[p + 1] - plan.cstart[p];
     const uint64_t sb = plan.pstart[p], se = plan.pstart[p + 1], len = se - sb;
     const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
+    // a partition processed whole may be abandoned when its groups overflow
+    // the table (ovf set): it is then recorded in ovf = {count, (p, rows)...}
+    const bool whole = emit && nch == 1;
+    const bool abandon = !FX && whole && ovf != nullptr;
     lds_init(t, s);
+    if (threadIdx.x == 0) {
+      s_abort[0] = 0;
+      s_abort[1] = 0;
+    }
     __syncthreads();
     if constexpr (FX) {
       agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe);
@@ -111,11 +120,23 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
       const void *cols[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
-      agg_rows_int<K, NC, VW, FOLD>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s);
+      agg_rows_int<K, NC, VW, FOLD>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
+                                    abandon ? s_abort : nullptr);
     }
     __syncthreads();
-    if (emit && nch == 1) lds_emit<K>(t, s, od, emit_count, g.flags, scratch, &s_base);
-    else lds_flush<K>(t, s, g);
+    if (abandon && s_abort[0]) {
+      if (threadIdx.x == 0) {
+        const uint32_t i = atomicAdd(ovf, 1u);
+        if (i < plan.P) {
+          ovf[2 + 2 * i] = p;
+          ovf[3 + 2 * i] = s_abort[1];
+        }
+      }
+    } else if (whole) {
+      lds_emit<K>(t, s, od, emit_count, g.flags, scratch, &s_base);
+    } else {
+      lds_flush<K>(t, s, g);
+    }
     __syncthreads();
   }
 }
@@ -513,13 +534,13 @@ static hipError_t set_lds_limit(const void *fn, size_t bytes) {
 template <typename K, bool FX, int NC, int VW, bool FOLD>
 static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan &plan, const Layout &L, const GTable &g,
                                int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st, int emit,
-                               const OutDesc &o, uint64_t *emit_count) {
+                               const OutDesc &o, uint64_t *emit_count, uint32_t *ovf) {
   const size_t lds = L.bytes;
   hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC, VW, FOLD>, lds);
   if (e != hipSuccess) return e;
   const AccProg ap = make_acc_prog(s, L);
   hipLaunchKernelGGL((k_agg<K, FX, NC, VW, FOLD>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g,
-                     ap, nbits, max_probe, emit, o, emit_count);
+                     ap, nbits, max_probe, emit, o, emit_count, ovf);
   return hipGetLastError();
 }
 
@@ -529,13 +550,13 @@ static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan 
 // fold (their limb vectors make per-row atomics costly).
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
-                      int emit, const OutDesc *od, uint64_t *emit_count, int fold) {
+                      int emit, const OutDesc *od, uint64_t *emit_count, int fold, uint32_t *ovf) {
   OutDesc o{};
   if (od) o = *od;
   const bool fx = s.nfx > 0;
   const bool one = s.ncols <= 1;
   const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
-#define TGA_AGG_ARGS keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, emit_count
+#define TGA_AGG_ARGS keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, emit_count, ovf
 #define TGA_AGG_CASE(K)                                                                         \
   if (fx) return launch_agg_t<K, true, kMaxCols, 0, true>(TGA_AGG_ARGS);                        \
   if (one) {                                                                                    \

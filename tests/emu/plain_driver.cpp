@@ -81,9 +81,10 @@ std::vector<uint64_t> keys_skew(int64_t n, uint64_t seed) {
   return k;
 }
 
-tga_options opts(int path, int fold, int pbits = 0, int slots = 0) {
+tga_options opts(int path, int fold, int pbits = 0, int slots = 0, int64_t est = 0) {
   tga_options o;
   tga_options_default(&o);
+  o.est_groups = est;
   o.path = path;
   o.wave_fold = fold;
   o.partition_bits = pbits;
@@ -166,7 +167,10 @@ int run_case(tga_context *ctx, const Case &c) {
       }
     }
   }
-  std::printf("%s: %lld rows, %lld groups %s\n", c.name.c_str(), (long long)n, (long long)g, bad ? "FAIL" : "ok");
+  tga_options plan;
+  tga_last_plan(ctx, &plan);
+  std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots) %s\n", c.name.c_str(), (long long)n,
+              (long long)g, plan.path, plan.partition_bits, plan.table_slots, bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }
 
@@ -218,6 +222,12 @@ int main(int argc, char **argv) {
   // exact fp64 SUM (folding kernel variant)
   cases.push_back({"fp64_sum", 64, keys_u64(N, 2000, 23), {col_f64(N, 24)}, {TGA_F64},
                    {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}}, opts(TGA_PATH_AUTO, 0)});
+  // a hint far below the truth: partitions overflow their tables, are
+  // abandoned, and the fanout is planned again (twice: 2^4 -> 2^7 -> 2^9)
+  cases.push_back({"replan_u64", 64, keys_u64(N, 150000, 27), {col_i32(N, -9, 9, 28)}, {TGA_I32},
+                   {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 0, 1024, 2000)});
+  cases.push_back({"replan_u32_fold", 32, keys_u32(N, 120000, 29), {col_i32(N, -9, 9, 30)}, {TGA_I32},
+                   {a4[0], a4[1]}, opts(TGA_PATH_PARTITIONED, 1, 0, 2048, 500)});
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
   int bad = 0, ran = 0;
