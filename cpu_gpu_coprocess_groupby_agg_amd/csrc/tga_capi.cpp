@@ -310,10 +310,12 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   p.layout = make_layout(q.schema, q.key_bytes, p.table_slots);
   p.nbits = ilog2((uint64_t)p.table_slots + 1);
   p.max_probe = (int)std::min<uint32_t>(p.table_slots / 4, 64);  // in buckets
-  // Wave folding pays when one key is a large share of the rows (a wave then
-  // holds several copies of it); otherwise the per-row ballots cost more than
-  // the LDS spends serialising the rare same-address lanes.
-  p.fold = o.wave_fold == 1 ? 1 : o.wave_fold == 2 ? 0 : (top_share >= 1.0 / 16 ? 1 : 0);
+  // Wave folding (one ballot per slot bit, then a serial fold into the
+  // lowest lane) is opt-in: on MI355X the LDS serialises same-address atomics
+  // of one wave faster than the fold costs, even when every row has the same
+  // key (2^26 rows, one key: 0.52 ms without, 6.9 ms with; Zipf(1.1) keys:
+  // 2x slower with).
+  p.fold = o.wave_fold == 1 ? 1 : 0;
   p.opt = o;
   p.top_share = top_share;
   p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < 12;
@@ -535,10 +537,12 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   const uint32_t ntiles = (uint32_t)((n + plan.tile_rows - 1) / plan.tile_rows);
   const uint64_t cells = (uint64_t)P * ntiles;
   DBuf cnt, offs, desc, pkeys, planbuf, counters, ovf;
+  // at most one abandoned record per work item: partitions plus split chunks
+  const uint32_t ovf_cap = P + (uint32_t)std::min<uint64_t>(1u << 24, n / std::max<uint64_t>(plan.chunk_rows, 1) + 1);
   DBuf pcols[kMaxCols];
   if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
       !pkeys.alloc(ctx, n * q.key_bytes) || !planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64) ||
-      !counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(P + 1) * 8))
+      !counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(ovf_cap + 1) * 8))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   for (int c = 0; c < s.ncols; ++c) {
@@ -552,7 +556,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
   TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
   const bool may_abandon = plan.may_replan;
-  if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, 0, st));
+  if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, (uint64_t)ovf_cap << 32, st));
   TGA_HIP(ctx, timed(ctx, "part_count", [&] {
     return launch_part_count(q.key_bytes, d_keys, n, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
   }));
@@ -625,20 +629,17 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   TGA_HIP(ctx, hipStreamSynchronize(st));
   if (ctx->prof.on) ctx->prof.resolve();
   if (novf) {
-    // Abandoned partitions: a table of T slots was full after r of the
-    // partition's R rows, so it holds about T * R / r groups.  The groups of
-    // the finished partitions are exact (hcount counts the emitted rows).
-    novf = std::min(novf, P);
+    // Abandoned work items: a table of T slots was full after r of the
+    // item's R rows, so the item holds about T * R / r groups; the finished
+    // ones are counted exactly (hcount).
+    novf = std::min(novf, ovf_cap);
     std::vector<uint32_t> list(2 * (size_t)novf);
-    std::vector<uint64_t> pst(P + 1);
     TGA_HIP(ctx, hipMemcpyAsync(list.data(), ovf.as<uint32_t>() + 2, list.size() * 4, hipMemcpyDeviceToHost, st));
-    TGA_HIP(ctx, hipMemcpyAsync(pst.data(), ap.pstart, (P + 1) * 8, hipMemcpyDeviceToHost, st));
     TGA_HIP(ctx, hipStreamSynchronize(st));
     double est = (double)hcount;
     for (uint32_t i = 0; i < novf; ++i) {
-      const uint32_t p = list[2 * i], r = std::max<uint32_t>(1, list[2 * i + 1]);
-      const double R = (double)(pst[p + 1] - pst[p]);
-      est += (double)plan.table_slots * std::max(1.0, R / (double)r);
+      const double r = (double)std::max<uint32_t>(1, list[2 * i]), R = (double)list[2 * i + 1];
+      est += (double)plan.table_slots * std::max(1.0, R / r);
     }
     *replan_est = (uint64_t)std::min<double>((double)n, est);
     return TGA_OK;
@@ -698,6 +699,7 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
       if (!overflow) {
         ctx->last_plan = tga_options{};
         plan_to_options(plan, ctx->last_plan);
+        ctx->last_plan.attempts = attempt + 1;
         return TGA_OK;
       }
       plan.gslots *= 4;  // the estimate was too low: larger tables, run again
@@ -763,6 +765,7 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
     }
     ctx->last_plan = tga_options{};
     plan_to_options(plan, ctx->last_plan);
+    ctx->last_plan.attempts = attempt + 1;
     if (ext_keys && hcount > ext_cap) return fail(ctx, TGA_ERR_CAPACITY, "%llu groups > capacity %llu",
                                                  (unsigned long long)hcount, (unsigned long long)ext_cap);
     return TGA_OK;

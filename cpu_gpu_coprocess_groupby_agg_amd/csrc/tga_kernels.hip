@@ -104,10 +104,11 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     const uint32_t j = item - plan.cstart[p], nch = plan.cstart[p + 1] - plan.cstart[p];
     const uint64_t sb = plan.pstart[p], se = plan.pstart[p + 1], len = se - sb;
     const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
-    // a partition processed whole may be abandoned when its groups overflow
-    // the table (ovf set): it is then recorded in ovf = {count, (p, rows)...}
+    // with ovf set, a work item whose groups overflow the LDS table is
+    // abandoned and recorded in ovf = {count, capacity, (rows consumed, rows)...}
+    // instead of spilling: the host then plans a finer partitioning
     const bool whole = emit && nch == 1;
-    const bool abandon = !FX && whole && ovf != nullptr;
+    const bool abandon = !FX && ovf != nullptr;
     lds_init(t, s);
     if (threadIdx.x == 0) {
       s_abort[0] = 0;
@@ -127,9 +128,9 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     if (abandon && s_abort[0]) {
       if (threadIdx.x == 0) {
         const uint32_t i = atomicAdd(ovf, 1u);
-        if (i < plan.P) {
-          ovf[2 + 2 * i] = p;
-          ovf[3 + 2 * i] = s_abort[1];
+        if (i < ovf[1]) {
+          ovf[2 + 2 * i] = s_abort[1];
+          ovf[3 + 2 * i] = (uint32_t)(e - b);
         }
       }
     } else if (whole) {

@@ -79,7 +79,8 @@ typedef struct tga_options {
   int32_t block_threads;/* threads per workgroup of the aggregation kernel */
   int64_t chunk_rows;   /* max rows per aggregation work item */
   int32_t wave_fold;    /* 0 = tuner decides, 1 = fold equal keys of a wave before the LDS atomics, 2 = never */
-  int32_t reserved[7];
+  int32_t attempts;     /* tga_last_plan only: executions the last call needed (finer fanout / larger tables) */
+  int32_t reserved[6];
 } tga_options;
 
 typedef struct tga_context tga_context;

@@ -228,6 +228,14 @@ int main(int argc, char **argv) {
                    {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 0, 1024, 2000)});
   cases.push_back({"replan_u32_fold", 32, keys_u32(N, 120000, 29), {col_i32(N, -9, 9, 30)}, {TGA_I32},
                    {a4[0], a4[1]}, opts(TGA_PATH_PARTITIONED, 1, 0, 2048, 500)});
+  {
+    // half the rows on one key (its partition is split into chunks) plus a
+    // wide tail, with a far too small hint: split chunks are abandoned too
+    std::vector<uint64_t> k = keys_u64(N, 90000, 31);
+    for (int64_t i = 0; i < N; i += 2) k[i] = 77;
+    cases.push_back({"replan_skew_split", 64, k, {col_i32(N, -9, 9, 32)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
+                     opts(TGA_PATH_PARTITIONED, 2, 0, 1024, 1000)});
+  }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
   int bad = 0, ran = 0;
