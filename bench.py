@@ -50,7 +50,72 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--only", choices=["all", "headline", "copy"], default="all",
                     help="headline: only the 2^28-row group-by (profiling runs); copy: only the copy reference")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that fill roofline.traffic")
     return ap.parse_args()
+
+
+# profiler name of a timed step -> kernel symbol it launches
+KERNEL_SYMBOL = {"part_scatter": "k_part_scatter", "agg_partitions": "k_agg", "agg_direct": "k_agg",
+                 "part_count": "k_part_count", "scan": "k_scan_u32", "compact": "k_compact", "copy16": "k_copy16"}
+
+
+def pmc_pass(counter, argv, workdir):
+    """One rocprofv3 counter pass over a child `python3 bench.py ...`; returns
+    {kernel symbol prefix: [per-dispatch values in KiB]} or None."""
+    import csv
+    import shutil
+    import subprocess
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    out = os.path.join(workdir, counter.lower())
+    cmd = [exe, "--pmc", counter, "-f", "csv", "-d", out, "-o", "run", "--", sys.executable, os.path.abspath(__file__)]
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        r = subprocess.run(cmd + argv, cwd=ROOT, capture_output=True, text=True, timeout=240, env=env)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        log(f"rocprofv3 {counter} pass failed ({r.returncode}): {r.stderr[-400:]}")
+        return None
+    vals = {}
+    for root, _, files in os.walk(out):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for row in csv.DictReader(open(os.path.join(root, f))):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row["Kernel_Name"].replace("void ", "").split("(")[0].split("<")[0]
+                    vals.setdefault(name.split("::")[-1], []).append(float(row["Counter_Value"]))
+    return vals
+
+
+def measure_traffic(dom_name, args):
+    """HBM bytes per launch of the dominant kernel from FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for the copy kernel
+    moving a known 2 GiB in and 2 GiB out."""
+    import tempfile
+    sym = KERNEL_SYMBOL.get(dom_name)
+    if sym is None:
+        return None, None
+    head = ["--only", "headline", "--steps", "1", "--warmup", "1", "--no-traffic", "--rows-log2", str(args.rows_log2)]
+    copy = ["--only", "copy", "--no-traffic"]
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as wd:
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            h = pmc_pass(counter, head, os.path.join(wd, "h"))
+            c = pmc_pass(counter, copy, os.path.join(wd, "c"))
+            if not h or not c or sym not in h or "k_copy16" not in c:
+                return None, None
+            copy_kib = float(np.median(c["k_copy16"]))
+            scale = (1 << 31) / 1024.0 / copy_kib  # true bytes / counted bytes on the copy kernel
+            res[counter] = (float(np.median(h[sym])) * 1024.0 * scale, scale)
+    traffic = res["FETCH_SIZE"][0] + res["WRITE_SIZE"][0]
+    detail = {"fetch_bytes": round(res["FETCH_SIZE"][0]), "write_bytes": round(res["WRITE_SIZE"][0]),
+              "fetch_scale": round(res["FETCH_SIZE"][1], 3), "write_scale": round(res["WRITE_SIZE"][1], 3),
+              "kernel": sym}
+    return traffic, detail
 
 
 def headline(ctx, tga, args):
@@ -82,6 +147,17 @@ def headline(ctx, tga, args):
     assert np.all(r["aggs"][2] <= r["aggs"][3])
     ms = dt * 1e3 / args.steps
     plan = ctx.last_plan()
+    # the same step without the cardinality hint: the estimate comes from a
+    # device key sample (reported beside the headline, not as it)
+    ctx.free_result(ctx.groupby_resident(keys, [vals], aggs)[0])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r2, _ = ctx.groupby_resident(keys, [vals], aggs)
+        ctx.free_result(r2)
+    ctx.synchronize()
+    ms_nohint = (time.perf_counter() - t0) * 1e3 / args.steps
+    plan_nohint = ctx.last_plan()
     # per-kernel HIP-event times over the same number of steps
     ctx.profile(True)
     ctx.profile_reset()
@@ -108,6 +184,8 @@ def headline(ctx, tga, args):
     op_alg = n * row_b + G * 32
     out = {
         "n": n, "groups": G, "ms_per_step": ms, "plan": plan,
+        "no_hint": {"ms_per_step": round(ms_nohint, 4), "rows_per_s": round(n / (ms_nohint * 1e-3), 1),
+                    "partition_bits": plan_nohint["partition_bits"], "attempts": plan_nohint["attempts"]},
         "kernels_ms_per_step": {k: t / args.steps for k, (c, t) in kern.items()},
         "dominant": dom_name, "dominant_avg_ms": dom_avg_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -315,6 +393,12 @@ def main():
         return
     h, keys, vals = headline(ctx, tga, args)
     log(f"headline: {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s, kernels {h['kernels_ms_per_step']}")
+    if not args.no_traffic and args.only == "all":
+        traffic, detail = measure_traffic(h["dominant"], args)
+        if traffic:
+            h["roofline"]["traffic"] = round(traffic)
+            h["roofline"]["traffic_detail"] = detail
+        log(f"traffic of {h['dominant']}: {traffic} {detail}")
     cpu = None
     if args.only == "headline":
         args.no_cpu = args.no_batch = args.no_serve = True
@@ -350,6 +434,8 @@ def main():
                    "rows": h["n"], "groups": h["groups"], "plan": h["plan"]},
         "roofline": h["roofline"],
         "op_frac_of_hbm_peak": h["op_frac_of_peak"],
+        "dominant_kernel": {"name": h["dominant"], "avg_ms": round(h["dominant_avg_ms"], 4)},
+        "no_hint": h["no_hint"],
         "kernels_ms_per_step": {k: round(v, 4) for k, v in h["kernels_ms_per_step"].items()},
         "cpu_baseline": cpu,
         "copy_reference": copy,

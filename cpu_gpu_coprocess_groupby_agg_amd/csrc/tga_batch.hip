@@ -148,11 +148,16 @@ __device__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_
   }
 }
 
+// DIRECT items (a row range) flush their table into the query's global
+// table.  PAGG items (a whole partition: its keys occur in no other item)
+// emit their table straight into the query's output, unless a row had to
+// spill to the global table; then the table is flushed there as well.
 template <typename K, bool FX, int NC>
 __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e, char *smem,
-                         uint32_t *sentinel) {
+                         uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch, uint64_t *s_base) {
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
+  if (threadIdx.x == 0) *spilled = FX ? 1u : 0u;  // fp64 limbs: always through the global table
   __syncthreads();
   if constexpr (FX) {
     agg_rows<K, FX, NC>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
@@ -162,15 +167,19 @@ __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uin
     for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
     if constexpr (NC == 1) {
       if (dtype_bytes((int)(Q.ap.coldt & 15u)) == 8)
-        agg_rows_int<K, 1, 8, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+        agg_rows_int<K, 1, 8, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
+                                     nullptr, spilled);
       else
-        agg_rows_int<K, 1, 4, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+        agg_rows_int<K, 1, 4, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
+                                     nullptr, spilled);
     } else {
-      agg_rows_int<K, NC, 0, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s);
+      agg_rows_int<K, NC, 0, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
+                                    nullptr, spilled);
     }
   }
   __syncthreads();
-  lds_flush<K>(t, s, Q.g);
+  if (whole && wave_uniform(*spilled) == 0) lds_emit<K>(t, s, Q.od, Q.count_out, Q.g.flags, scratch, s_base);
+  else lds_flush<K>(t, s, Q.g);
 }
 
 // Compacts table slots [a, b) of one query; output rows are reserved with an
@@ -230,11 +239,11 @@ __device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *
 
 template <typename K, bool FX, int NC>
 __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
-                         uint64_t *s_base) {
+                         uint64_t *s_base, uint32_t *spilled) {
   switch (it.type) {
     case IT_INIT: item_init<K>(Q, it.a, it.b); break;
     case IT_COUNT: item_count<K>(Q, (uint32_t)it.a, (uint32_t *)smem); break;
-    case IT_DIRECT: item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel); break;
+    case IT_DIRECT: item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel, false, spilled, scratch, s_base); break;
     case IT_SCAN: item_scan(Q, (uint32_t *)smem, scratch); break;
     case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch); break;
     case IT_PAGG: {
@@ -242,7 +251,8 @@ __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t 
 #pragma unroll
       for (int c = 0; c < kMaxCols; ++c)
         if (c < sp.ncols) sp.cols[c] = Q.pcols[c];
-      item_agg<K, FX, NC>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel);
+      item_agg<K, FX, NC>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel, true, spilled, scratch,
+                          s_base);
       break;
     }
     case IT_COMPACT: item_compact<K>(Q, it.a, it.b, scratch, s_base); break;
@@ -254,7 +264,7 @@ template <bool FX, int NC>
 __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
                                                          uint32_t nitems, uint32_t *ticket) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ uint32_t s_item, s_sentinel;
+  __shared__ uint32_t s_item, s_sentinel, s_spilled;
   __shared__ uint32_t scratch[kBatchThreads / kWave + 1];
   __shared__ uint64_t s_base;
   // the item finished last by this workgroup; its completion is published
@@ -283,8 +293,8 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base);
-    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base);
+    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
+    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     prev_q = it.q;
@@ -332,6 +342,9 @@ QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t
   z.T = z.part ? tmax : (uint32_t)std::min<uint64_t>(tmax, std::max<uint64_t>(256, np2(2 * G)));
   z.S = std::min<uint64_t>(np2(std::max<uint64_t>(256, 2 * G)), np2(std::max<uint64_t>(256, 2 * n)));
   z.cap = std::min<uint64_t>(z.S + 1, n + 1);
+  // partitioned: whole partitions emit into the output; the global table
+  // only takes the rows of partitions that spill (a miss is rerun alone)
+  if (z.part) z.S = std::max<uint64_t>(4096, z.S / 8);
   if (z.part) {
     z.pbits = std::max(1, std::min(12, lg2(np2((2 * G + z.T - 1) / z.T))));
     z.tile_rows = kBatchThreads * 16;

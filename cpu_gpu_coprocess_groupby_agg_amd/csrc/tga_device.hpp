@@ -550,20 +550,21 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 // FOLD: before the atomics, lanes of a wave holding the same slot are found
 // with one ballot per slot bit and folded into their lowest lane, so each
 // slot gets one atomic per wave.  That pays off only when one key fills a
-// large share of the rows (heavy hitters); with many groups, duplicate keys
-// inside a wave are rare and the LDS serialises the few same-address lanes
-// itself, so the tuner compiles the loop without it.
+// large share of the rows; measured on MI355X the LDS serialises the
+// same-address lanes of one wave faster than the fold costs, so it is
+// opt-in (tga_options.wave_fold).
 //
-// giveup (LDS, two words; null = spill mode): a partition processed whole by
-// one workgroup whose groups do not fit the table is abandoned instead of
-// spilled -- giveup[0] is set, every wave leaves the loop, and giveup[1]
-// receives the rows the workgroup had consumed, from which the host sizes a
-// finer partitioning.
+// giveup (LDS, two words; null = spill mode): a work item whose groups do
+// not fit the table is abandoned instead of spilled -- giveup[0] is set,
+// every wave leaves the loop, and giveup[1] receives the rows the workgroup
+// had consumed, from which the host sizes a finer partitioning.
+// spilled (LDS word, optional, spill mode): set when a row went to the
+// global table.
 template <typename K, int NC, int VW, bool FOLD>
 __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b,
                                              uint64_t e, const LdsTable<K> &t, const AccProg &ap, int nbits,
                                              int max_probe, const GTable *gsp, const Schema *ssp,
-                                             uint32_t *giveup = nullptr) {
+                                             uint32_t *giveup = nullptr, uint32_t *spilled = nullptr) {
   constexpr int U = 4;
   const int lane = (int)__lane_id();
   const int wave = (int)(threadIdx.x >> 6);
@@ -626,6 +627,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
       if (giveup) {
         if (spill) *giveup = 1;
       } else {
+        if (spilled && spill) *spilled = 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)

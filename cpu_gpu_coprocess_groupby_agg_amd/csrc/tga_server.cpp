@@ -60,6 +60,9 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
+// rows per flight before a new flight is started (one 2^26-row request fills it)
+constexpr uint64_t kFlightRows = 1ULL << 26;
+
 struct tga_server {
   tga_context *ctx = nullptr;
   uint64_t budget = 0;
@@ -141,10 +144,17 @@ bool tga_server::launch_one() {
     for (int s = 0; s < nslots; ++s)
       if (!slot_busy[s]) { slot = s; break; }
     if (slot < 0) return false;
+    // A flight holds at most max_batch requests and about kFlightRows rows,
+    // so a large request travels (nearly) alone and the small requests
+    // admitted after it do not wait for it: they go out in the next flight,
+    // on another stream, and overlap with it on the device.
+    uint64_t rows = 0;
     while (!pending.empty() && (int)take.size() < max_batch) {
       const Req &r = pending.front();
       if (reserved + bytes + r.bytes > budget) break;  // oldest first: wait for memory
+      if (!take.empty() && rows + (uint64_t)r.q.n_rows > kFlightRows) break;
       bytes += r.bytes;
+      rows += (uint64_t)r.q.n_rows;
       take.push_back(std::move(pending.front()));
       pending.pop_front();
     }
