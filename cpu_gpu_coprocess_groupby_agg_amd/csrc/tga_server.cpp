@@ -5,7 +5,8 @@
 //     the memory budget and a stream slot is free,
 //   * turns each admitted group into one batched persistent launch (the work
 //     queue of tga_batch.hip) on its own stream, so batches overlap on the
-//     device while new requests keep arriving,
+//     device while new requests keep arriving; a request of >= 2^24 rows
+//     runs alone through the plain operator instead,
 //   * polls the streams' events, finishes completed batches (overflowed
 //     queries rerun alone) and hands each request its own result,
 // and records latency = completion - submission for p50/p99.
@@ -60,8 +61,12 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
-// rows per flight before a new flight is started (one 2^26-row request fills it)
+// rows per flight before a new flight is started
 constexpr uint64_t kFlightRows = 1ULL << 26;
+// A request this large fills the GPU by itself: it runs alone through the
+// plain operator (device-wide scans, direct emission), while the flights of
+// small requests already in the streams keep running beside it.
+constexpr uint64_t kSoloRows = 1ULL << 24;
 
 struct tga_server {
   tga_context *ctx = nullptr;
@@ -85,7 +90,51 @@ struct tga_server {
   void loop();
   bool finish_ready();
   bool launch_one();
+  bool run_solo();
 };
+
+bool tga_server::run_solo() {
+  Req r;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    if (pending.empty()) return false;
+    const Req &f = pending.front();
+    if ((uint64_t)f.q.n_rows < kSoloRows || reserved + f.bytes > budget) return false;
+    r = std::move(pending.front());
+    pending.pop_front();
+    reserved += r.bytes;
+    peak = std::max(peak, reserved);
+  }
+  Done d{nullptr, TGA_OK};
+  {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    hipSetDevice(ctx->device);
+    QuerySpec spec;
+    build_spec(&r.q.key, r.vals.data(), (int32_t)r.vals.size(), r.aggs.data(), (int32_t)r.aggs.size(), spec);
+    const Plan p = autotune(ctx, spec, (uint64_t)r.q.n_rows, r.est, nullptr);
+    PlainOut po;
+    uint64_t G = 0;
+    d.st = run_plain(ctx, spec, r.q.key.data, (uint64_t)r.q.n_rows, p, &po, nullptr, nullptr, 0, &G);
+    if (!d.st) {
+      auto *res = new tga_result();
+      res->ctx = ctx;
+      res->key_dtype = spec.key_dtype;
+      res->naggs = spec.schema.nacc;
+      for (int a = 0; a < spec.schema.nacc; ++a) res->dtype[a] = spec.out_dtype[a];
+      res->out = std::move(po);
+      d.r = res;
+    }
+  }
+  const auto now = clk::now();
+  std::lock_guard<std::mutex> g(mu);
+  done[r.ticket] = d;
+  lat_us.push_back(std::chrono::duration<double, std::micro>(now - r.t_submit).count());
+  ++completed;
+  ++launches;
+  reserved -= r.bytes;
+  cv_done.notify_all();
+  return true;
+}
 
 bool tga_server::finish_ready() {
   bool any = false;
@@ -152,6 +201,7 @@ bool tga_server::launch_one() {
     while (!pending.empty() && (int)take.size() < max_batch) {
       const Req &r = pending.front();
       if (reserved + bytes + r.bytes > budget) break;  // oldest first: wait for memory
+      if ((uint64_t)r.q.n_rows >= kSoloRows) break;      // runs alone (run_solo)
       if (!take.empty() && rows + (uint64_t)r.q.n_rows > kFlightRows) break;
       bytes += r.bytes;
       rows += (uint64_t)r.q.n_rows;
@@ -204,6 +254,7 @@ void tga_server::loop() {
   for (;;) {
     bool did = false;
     if (!flights.empty()) did |= finish_ready();
+    did |= run_solo();
     did |= launch_one();
     std::unique_lock<std::mutex> g(mu);
     if (stop && pending.empty() && flights.empty()) break;

@@ -131,6 +131,10 @@ def _result_arrays(r, key_dtype):
 @pytest.mark.parametrize("budget", [64 << 30, 96 << 20])
 def test_server_mix_equals_alone(ctx, budget):
     specs = [s for s in SPECS if s[0] > 0] * 3
+    if budget > 1 << 30:
+        # requests of >= 2^24 rows run alone through the plain operator, beside the batched flights
+        specs[5:5] = [(1 << 24, 1 << 16, 64, True)]
+        specs += [(1 << 24, 5000, 32, False)]
     qs, keep, host = _queries(ctx, specs)
     srv = ctypes.c_void_p()
     ctx._check(lib().tga_server_create(ctx.h, budget, 8, ctypes.byref(srv)))
