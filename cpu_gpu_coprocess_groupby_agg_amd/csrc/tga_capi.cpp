@@ -405,6 +405,13 @@ hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable
 
 static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 
+// Persistent scatter grid: as many workgroups as the LDS lets run at once.
+static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan) {
+  const size_t lds = scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
+  const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / std::max<size_t>(lds, 1)));
+  return (unsigned)(ctx->cus * per_cu);
+}
+
 // Plan buffer layout of the partitioned path: pstart[P+1] u64 | cstart[P+1]
 // u32 | ticket u32 | split_rows u64 (8-byte aligned).
 static AggPlan part_plan(const DBuf &buf, uint32_t P) {
@@ -470,7 +477,7 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
       }));
       TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
         return launch_part_scatter(q.key_bytes, kbase, sc, m, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
-                                   pkeys.p, po, st);
+                                   pkeys.p, po, st, scatter_grid(ctx, q, plan));
       }));
       if (!planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
       AggPlan ap = part_plan(planbuf, P);
@@ -565,7 +572,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   }));
   TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
     return launch_part_scatter(q.key_bytes, d_keys, s, n, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
-                               pkeys.p, po, st);
+                               pkeys.p, po, st, scatter_grid(ctx, q, plan));
   }));
   AggPlan ap = part_plan(planbuf, P);
   TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {

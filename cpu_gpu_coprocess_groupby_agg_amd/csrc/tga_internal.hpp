@@ -43,9 +43,10 @@ uint64_t scan_tiles(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
                            hipStream_t st);
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows);
+// grid: persistent workgroups (each takes every grid-th tile); 0 = one per tile
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
                                uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
-                               const PartOut &o, hipStream_t st);
+                               const PartOut &o, hipStream_t st, unsigned grid = 0);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,

@@ -180,12 +180,30 @@ __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint
 // (b) partitioning
 // ---------------------------------------------------------------------------
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup w runs on
+// XCD w % 8, each XCD with its own L2).  This renumbering gives the
+// workgroups of one XCD a contiguous range of items, so neighbouring items
+// -- whose outputs share cache lines -- meet in one L2.
+constexpr uint32_t kXcds = 8;
+// grid for `count` items: a multiple of the XCD count
+__host__ __device__ inline uint32_t xcd_grid(uint32_t count) { return (count + kXcds - 1) / kXcds * kXcds; }
+// item of workgroup w: XCD x = w % 8 owns items [x*per, (x+1)*per), its k-th
+// workgroup (w / 8) takes the k-th; padding workgroups get an item >= count
+__device__ __forceinline__ uint32_t xcd_local_index(uint32_t w, uint32_t count) {
+  const uint32_t per = (count + kXcds - 1) / kXcds;
+  return (w % kXcds) * per + w / kXcds;
+}
+
+// Per-tile partition histogram, cnt[p][tile]; tiles renumbered so that the
+// counters of neighbouring tiles (adjacent words of cnt[p][*]) are written
+// from one XCD.
 template <typename K>
 __global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys, uint64_t n, int pbits,
                                                      uint32_t tile_rows, uint32_t ntiles, uint32_t *cnt) {
   extern __shared__ uint32_t hist[];
   const uint32_t P = 1u << pbits;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_local_index(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;  // uniform per workgroup
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
   __syncthreads();
   const uint64_t b = (uint64_t)tile * tile_rows;
@@ -243,9 +261,19 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_u32(const uint32_t *__res
   }
 }
 
-// Tile of blockDim*ITEMS rows: rank rows by partition in LDS, then write each
-// partition's run contiguously at its scanned offset.
-template <typename K, int ITEMS>
+// Tiles of blockDim*ITEMS rows (the tiles of k_part_count): rank each row
+// inside its partition with an LDS histogram atomic, counting-sort the tile
+// into LDS, then write every partition's run contiguously at its scanned
+// offset (the LDS acts as the write-combining buffer).  Persistent and
+// XCD-aware: a workgroup takes one contiguous range of tiles.  The run of
+// partition p from tile t ends where the run from tile t+1 begins, so the
+// partial cache lines at the run ends are completed by the same workgroup a
+// moment later, in the same XCD's L2, and reach HBM as whole lines (with one
+// tile per workgroup, neighbouring tiles run on different XCDs, whose L2s
+// cannot merge them).  The next tile's rows are loaded into registers as
+// soon as the current tile is staged, so those reads are in flight during
+// the current tile's write phase.  NC = 1: one value column.
+template <typename K, int ITEMS, int NC>
 __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
                                                        uint32_t ntiles, const uint32_t *__restrict__ offs, K *okeys,
                                                        PartOut o) {
@@ -259,70 +287,92 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
   // stage layout: K keys[TR] | per column [TR] cells | u16 pid[TR]
   K *skey = (K *)stage;
   char *scol = stage + (size_t)TR * sizeof(K);
-  const int ncol = s.ncols;
-  uint32_t colb[kMaxCols];
-  size_t coff[kMaxCols];
+  const int ncol = NC == 1 ? 1 : s.ncols;
+  uint32_t colb[NC];
+  size_t coff[NC];
   size_t acc_off = 0;
 #pragma unroll
-  for (int c = 0; c < kMaxCols; ++c) {
+  for (int c = 0; c < NC; ++c) {
     colb[c] = c < ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
     coff[c] = acc_off;
     acc_off += (size_t)TR * colb[c];
   }
   uint16_t *spid = (uint16_t *)(scol + acc_off);
 
-  const uint32_t tile = blockIdx.x;
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-  __syncthreads();
-  const uint64_t b = (uint64_t)tile * TR;
   K kk[ITEMS];
-  uint32_t pp[ITEMS], rk[ITEMS];
-  uint64_t cell[ITEMS][kMaxCols];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    const bool act = row < n;
-    kk[i] = act ? keys[row] : KeyTraits<K>::kEmpty;
-#pragma unroll
-    for (int c = 0; c < kMaxCols; ++c) cell[i][c] = (act && c < ncol) ? load_cell(s, c, row) : 0;
+  uint64_t cell[ITEMS][NC];
+  // chunk of consecutive tiles; chunks renumbered so the chunks of one XCD
+  // are neighbours too (the grid is a multiple of the XCD count)
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
+  const uint32_t tend = min(ntiles, tile + per);
+  if (tile >= tend) return;
+  // rows past the end re-read the last row (in bounds); they are not staged
+#define TGA_SCATTER_LOAD(T)                                                                     \
+  {                                                                                             \
+    const uint64_t b0 = (uint64_t)(T) * TR;                                                     \
+    _Pragma("unroll") for (int i = 0; i < ITEMS; ++i) {                                         \
+      const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;                         \
+      const uint64_t r = row < n ? row : n - 1;                                                 \
+      kk[i] = keys[r];                                                                          \
+      _Pragma("unroll") for (int c = 0; c < NC; ++c) {                                          \
+        if (c >= ncol) { cell[i][c] = 0; continue; }                                            \
+        cell[i][c] = colb[c] == 4 ? (uint64_t)((const uint32_t *)s.cols[c])[r]                  \
+                                  : ((const uint64_t *)s.cols[c])[r];                           \
+      }                                                                                         \
+    }                                                                                           \
   }
+  TGA_SCATTER_LOAD(tile)
+  for (;;) {
+    const uint64_t b = (uint64_t)tile * TR;
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+    __syncthreads();
+    uint32_t pp[ITEMS], rk[ITEMS];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    pp[i] = hash_partition(key_hash((uint64_t)kk[i]), pbits);
-    rk[i] = row < n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
-  }
-  __syncthreads();
-  block_exclusive_scan_lds(hist, P, scratch);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = offs[(uint64_t)p * ntiles + tile];
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+      pp[i] = hash_partition(key_hash((uint64_t)kk[i]), pbits);
+      rk[i] = row < n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
+    }
+    __syncthreads();
+    block_exclusive_scan_lds(hist, P, scratch);
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = offs[(uint64_t)p * ntiles + tile];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    if (row < n) {
-      const uint32_t lp = hist[pp[i]] + rk[i];
-      skey[lp] = kk[i];
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
+      if (row < n) {
+        const uint32_t lp = hist[pp[i]] + rk[i];
+        skey[lp] = kk[i];
 #pragma unroll
-      for (int c = 0; c < kMaxCols; ++c) {
-        if (c >= ncol) break;
-        if (colb[c] == 4) ((uint32_t *)(scol + coff[c]))[lp] = (uint32_t)cell[i][c];
-        else ((uint64_t *)(scol + coff[c]))[lp] = cell[i][c];
+        for (int c = 0; c < NC; ++c) {
+          if (c >= ncol) break;
+          if (colb[c] == 4) ((uint32_t *)(scol + coff[c]))[lp] = (uint32_t)cell[i][c];
+          else ((uint64_t *)(scol + coff[c]))[lp] = cell[i][c];
+        }
+        spid[lp] = (uint16_t)pp[i];
       }
-      spid[lp] = (uint16_t)pp[i];
     }
-  }
-  __syncthreads();
-  const uint32_t rows = (uint32_t)min((uint64_t)TR, n - b);
-  for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
-    const uint32_t p = spid[j];
-    const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
-    okeys[dst] = skey[j];
+    __syncthreads();
+    // the next tile's reads go out before this tile's writes
+    const uint32_t next = tile + 1;
+    if (next < tend) TGA_SCATTER_LOAD(next)
+    const uint32_t rows = (uint32_t)min((uint64_t)TR, n - b);
+    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+      const uint32_t p = spid[j];
+      const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
+      okeys[dst] = skey[j];
 #pragma unroll
-    for (int c = 0; c < kMaxCols; ++c) {
-      if (c >= ncol) break;
-      if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[j];
-      else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[j];
+      for (int c = 0; c < NC; ++c) {
+        if (c >= ncol) break;
+        if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[j];
+        else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[j];
+      }
     }
+    __syncthreads();
+    if (next >= tend) break;
+    tile = next;
   }
+#undef TGA_SCATTER_LOAD
 }
 
 // ---------------------------------------------------------------------------
@@ -594,10 +644,12 @@ hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pb
                              uint32_t ntiles, uint32_t *cnt, hipStream_t st) {
   const size_t lds = (size_t)4 << pbits;
   if (key_bytes == 4)
-    hipLaunchKernelGGL(k_part_count<uint32_t>, dim3(ntiles), dim3(512), lds, st, (const uint32_t *)keys, n, pbits,
+    hipLaunchKernelGGL(k_part_count<uint32_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint32_t *)keys, n,
+                       pbits,
                        tile_rows, ntiles, cnt);
   else
-    hipLaunchKernelGGL(k_part_count<uint64_t>, dim3(ntiles), dim3(512), lds, st, (const uint64_t *)keys, n, pbits,
+    hipLaunchKernelGGL(k_part_count<uint64_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint64_t *)keys, n,
+                       pbits,
                        tile_rows, ntiles, cnt);
   return hipGetLastError();
 }
@@ -619,25 +671,31 @@ size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t til
   return b + row * tile_rows;
 }
 
-hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
-                               const PartOut &o, hipStream_t st) {
+template <typename K, int NC>
+static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                   uint32_t ntiles, const uint32_t *offs, void *okeys, const PartOut &o,
+                                   unsigned grid, hipStream_t st) {
   constexpr int ITEMS = 8;
   const unsigned block = tile_rows / ITEMS;
-  const size_t lds = scatter_lds_bytes(s, key_bytes, pbits, tile_rows);
-  hipError_t e;
-  if (key_bytes == 4) {
-    e = set_lds_limit<uint32_t>((const void *)&k_part_scatter<uint32_t, ITEMS>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_part_scatter<uint32_t, ITEMS>), dim3(ntiles), dim3(block), lds, st,
-                       (const uint32_t *)keys, s, n, pbits, ntiles, offs, (uint32_t *)okeys, o);
-  } else {
-    e = set_lds_limit<uint64_t>((const void *)&k_part_scatter<uint64_t, ITEMS>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_part_scatter<uint64_t, ITEMS>), dim3(ntiles), dim3(block), lds, st,
-                       (const uint64_t *)keys, s, n, pbits, ntiles, offs, (uint64_t *)okeys, o);
-  }
+  const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n, pbits,
+                     ntiles, offs, (K *)okeys, o);
   return hipGetLastError();
+}
+
+hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
+                               const PartOut &o, hipStream_t st, unsigned grid) {
+  if (grid == 0 || grid > ntiles) grid = ntiles;
+  grid = xcd_grid(grid);
+  const bool one = s.ncols <= 1;
+  if (key_bytes == 4)
+    return one ? launch_scatter_t<uint32_t, 1>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st)
+               : launch_scatter_t<uint32_t, kMaxCols>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st);
+  return one ? launch_scatter_t<uint64_t, 1>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st)
+             : launch_scatter_t<uint64_t, kMaxCols>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st);
 }
 
 uint64_t compact_tiles(uint64_t slots_plus_one) {
