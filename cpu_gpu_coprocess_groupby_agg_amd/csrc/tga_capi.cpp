@@ -425,6 +425,61 @@ static AggPlan part_plan(const DBuf &buf, uint32_t P) {
   return ap;
 }
 
+// Buffers of one partitioning pass; they must outlive the work queued on
+// the stream.
+struct PartBufs {
+  DBuf cnt, offs, total, pbase, desc, pkeys, planbuf;
+  DBuf pcols[kMaxCols];
+  AggPlan ap{};
+  Schema sp{};  // the schema over the partitioned copy
+  void keep(std::vector<DBuf> &k) {
+    for (DBuf *b : {&cnt, &offs, &total, &pbase, &desc, &pkeys, &planbuf}) k.push_back(std::move(*b));
+    for (auto &c : pcols)
+      if (c.p) k.push_back(std::move(c));
+  }
+};
+
+// (b) + (c) for rows [0, m) of keys / s.cols: tile histogram, column scan,
+// scatter into the partitioned copy, work-item plan.
+static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s, uint64_t m,
+                                 const Plan &plan, PartBufs &pb) {
+  hipStream_t st = ctx->stream;
+  const uint32_t P = 1u << plan.pbits;
+  const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
+  const uint64_t cells = (uint64_t)P * ntiles;
+  const uint64_t dwords = colscan_desc_words(ntiles, P);
+  if (!pb.cnt.alloc(ctx, cells * 4) || !pb.offs.alloc(ctx, cells * 4) || !pb.total.alloc(ctx, (uint64_t)P * 4) ||
+      !pb.pbase.alloc(ctx, (uint64_t)(P + 1) * 4) || !pb.desc.alloc(ctx, (dwords + 1) * 8) ||
+      !pb.pkeys.alloc(ctx, m * q.key_bytes) || !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  PartOut po{};
+  pb.sp = s;
+  for (int c = 0; c < s.ncols; ++c) {
+    if (!pb.pcols[c].alloc(ctx, m * (uint64_t)dtype_bytes(s.col_dtype[c])))
+      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+    po.cols[c] = pb.pcols[c].p;
+    pb.sp.cols[c] = pb.pcols[c].p;
+  }
+  TGA_HIP(ctx, launch_fill_u64(pb.desc.as<uint64_t>(), dwords + 1, 0, st));
+  TGA_HIP(ctx, timed(ctx, "part_count", [&] {
+    return launch_part_count(q.key_bytes, keys, m, plan.pbits, plan.tile_rows, ntiles, pb.cnt.as<uint32_t>(), st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "scan", [&] {
+    return launch_colscan(pb.cnt.as<uint32_t>(), ntiles, P, pb.desc.as<uint64_t>(),
+                          (uint32_t *)(pb.desc.as<uint64_t>() + dwords), pb.offs.as<uint32_t>(),
+                          pb.total.as<uint32_t>(), pb.pbase.as<uint32_t>(), st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+    return launch_part_scatter(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, ntiles, pb.offs.as<uint32_t>(),
+                               pb.pbase.as<uint32_t>(), pb.pkeys.p, po, st, scatter_grid(ctx, q, plan));
+  }));
+  pb.ap = part_plan(pb.planbuf, P);
+  TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
+    return launch_plan_chunks(pb.pbase.as<uint32_t>(), P, m, plan.chunk_rows, pb.ap, st);
+  }));
+  return TGA_OK;
+}
+
 // Aggregates rows [0, n) of device-resident columns into the global table g
 // (direct or partitioned path).  Temporaries are appended to `keep` and must
 // outlive the work queued on the context stream.
@@ -447,55 +502,20 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
   } else if (n > 0) {
     // partitioned path, in super-chunks of < 2^31 rows (u32 offsets)
     const uint64_t kSuper = 1ULL << 31;
-    const uint32_t P = 1u << plan.pbits;
     for (uint64_t r0 = 0; r0 < n; r0 += kSuper) {
       const uint64_t m = std::min<uint64_t>(kSuper, n - r0);
-      const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
-      const uint64_t cells = (uint64_t)P * ntiles;
-      DBuf cnt, offs, desc, pkeys;
-      DBuf pcols[kMaxCols];
-      if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
-          !pkeys.alloc(ctx, m * q.key_bytes))
-        return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
       Schema sc = s;
-      PartOut po{};
-      const char *kbase = (const char *)d_keys + r0 * q.key_bytes;
-      for (int c = 0; c < s.ncols; ++c) {
-        const uint64_t cb = (uint64_t)dtype_bytes(s.col_dtype[c]);
-        if (!pcols[c].alloc(ctx, m * cb)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
-        po.cols[c] = pcols[c].p;
-        sc.cols[c] = (const char *)s.cols[c] + r0 * cb;
+      for (int c = 0; c < s.ncols; ++c) sc.cols[c] = (const char *)s.cols[c] + r0 * (uint64_t)dtype_bytes(s.col_dtype[c]);
+      PartBufs pb;
+      {
+        tga_status e = partition_rows(ctx, q, (const char *)d_keys + r0 * q.key_bytes, sc, m, plan, pb);
+        if (e) return e;
       }
-      const uint64_t dwords = scan_tiles(cells) + 1;
-      uint32_t *ticket = (uint32_t *)(desc.as<uint64_t>() + dwords);
-      TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
-      TGA_HIP(ctx, timed(ctx, "part_count", [&] {
-        return launch_part_count(q.key_bytes, kbase, m, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
-      }));
-      TGA_HIP(ctx, timed(ctx, "scan", [&] {
-        return launch_scan_u32(cnt.as<uint32_t>(), offs.as<uint32_t>(), cells, desc.as<uint64_t>(), ticket, st);
-      }));
-      TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
-        return launch_part_scatter(q.key_bytes, kbase, sc, m, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
-                                   pkeys.p, po, st, scatter_grid(ctx, q, plan));
-      }));
-      if (!planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "plan");
-      AggPlan ap = part_plan(planbuf, P);
-      TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
-        return launch_plan_chunks(offs.as<uint32_t>(), P, ntiles, m, plan.chunk_rows, ap, st);
-      }));
-      Schema sp = s;
-      for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
       TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
-        return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
-                          plan.block, st, 0, nullptr, nullptr, plan.fold);
+        return launch_agg(q.key_bytes, pb.pkeys.p, pb.sp, pb.ap, plan.layout, g, plan.nbits, plan.max_probe,
+                          plan.workgroups, plan.block, st, 0, nullptr, nullptr, plan.fold);
       }));
-      keep.push_back(std::move(planbuf));
-      keep.push_back(std::move(cnt));
-      keep.push_back(std::move(offs));
-      keep.push_back(std::move(desc));
-      keep.push_back(std::move(pkeys));
-      for (int c = 0; c < s.ncols; ++c) keep.push_back(std::move(pcols[c]));
+      pb.keep(keep);
       if (r0 + kSuper < n) {  // reuse the buffers of this super-chunk
         TGA_HIP(ctx, hipStreamSynchronize(st));
         keep.clear();
@@ -541,43 +561,21 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
   const uint32_t P = 1u << plan.pbits;
-  const uint32_t ntiles = (uint32_t)((n + plan.tile_rows - 1) / plan.tile_rows);
-  const uint64_t cells = (uint64_t)P * ntiles;
-  DBuf cnt, offs, desc, pkeys, planbuf, counters, ovf;
+  DBuf counters, ovf;
   // at most one abandoned record per work item: partitions plus split chunks
   const uint32_t ovf_cap = P + (uint32_t)std::min<uint64_t>(1u << 24, n / std::max<uint64_t>(plan.chunk_rows, 1) + 1);
-  DBuf pcols[kMaxCols];
-  if (!cnt.alloc(ctx, cells * 4) || !offs.alloc(ctx, cells * 4) || !desc.alloc(ctx, (scan_tiles(cells) + 1) * 8 + 64) ||
-      !pkeys.alloc(ctx, n * q.key_bytes) || !planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64) ||
-      !counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(ovf_cap + 1) * 8))
+  if (!counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(ovf_cap + 1) * 8))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
-  PartOut po{};
-  for (int c = 0; c < s.ncols; ++c) {
-    if (!pcols[c].alloc(ctx, n * (uint64_t)dtype_bytes(s.col_dtype[c])))
-      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
-    po.cols[c] = pcols[c].p;
-  }
-  const uint64_t dwords = scan_tiles(cells) + 1;
-  uint32_t *ticket = (uint32_t *)(desc.as<uint64_t>() + dwords);
   uint64_t *emit_count = counters.as<uint64_t>();
-  TGA_HIP(ctx, launch_fill_u64(desc.as<uint64_t>(), dwords + 1, 0, st));
   TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
   const bool may_abandon = plan.may_replan;
   if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, (uint64_t)ovf_cap << 32, st));
-  TGA_HIP(ctx, timed(ctx, "part_count", [&] {
-    return launch_part_count(q.key_bytes, d_keys, n, plan.pbits, plan.tile_rows, ntiles, cnt.as<uint32_t>(), st);
-  }));
-  TGA_HIP(ctx, timed(ctx, "scan", [&] {
-    return launch_scan_u32(cnt.as<uint32_t>(), offs.as<uint32_t>(), cells, desc.as<uint64_t>(), ticket, st);
-  }));
-  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
-    return launch_part_scatter(q.key_bytes, d_keys, s, n, plan.pbits, plan.tile_rows, ntiles, offs.as<uint32_t>(),
-                               pkeys.p, po, st, scatter_grid(ctx, q, plan));
-  }));
-  AggPlan ap = part_plan(planbuf, P);
-  TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
-    return launch_plan_chunks(offs.as<uint32_t>(), P, ntiles, n, plan.chunk_rows, ap, st);
-  }));
+  PartBufs pb;
+  {
+    tga_status e = partition_rows(ctx, q, d_keys, s, n, plan, pb);
+    if (e) return e;
+  }
+  const AggPlan &ap = pb.ap;
   uint64_t split_rows = 0;
   TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
@@ -611,10 +609,8 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
       od.cols[a] = tmp.cols[a].p;
     }
   }
-  Schema sp = s;
-  for (int c = 0; c < s.ncols; ++c) sp.cols[c] = pcols[c].p;
   TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
-    return launch_agg(q.key_bytes, pkeys.p, sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
+    return launch_agg(q.key_bytes, pb.pkeys.p, pb.sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
                       plan.block, st, 1, &od, emit_count, plan.fold, may_abandon ? ovf.as<uint32_t>() : nullptr);
   }));
   // the global table's groups follow the emitted ones

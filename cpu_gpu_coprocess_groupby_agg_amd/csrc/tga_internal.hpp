@@ -35,18 +35,25 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
                       int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0,
                       uint32_t *ovf = nullptr);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
-hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
-                              const AggPlan &plan, hipStream_t st);
+hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
+                              hipStream_t st);
 hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
                              uint32_t ntiles, uint32_t *cnt, hipStream_t st);
 uint64_t scan_tiles(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
                            hipStream_t st);
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows);
-// grid: persistent workgroups (each takes every grid-th tile); 0 = one per tile
+// Column-wise scan of the tile-major histogram cnt[ntiles][P]: offs (same
+// shape, exclusive per partition), total[P], pbase[P+1] (partition starts).
+// desc: colscan_desc_words() zeroed words followed by a zeroed ticket word.
+uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P);
+hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
+                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st);
+// grid: persistent workgroups (each takes a contiguous range of tiles);
+// 0 = one per tile
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
-                               const PartOut &o, hipStream_t st, unsigned grid = 0);
+                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase,
+                               void *okeys, const PartOut &o, hipStream_t st, unsigned grid = 0);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,

@@ -153,17 +153,17 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
   }
 }
 
-// (c) partition p has rows [offs[p*ntiles], offs[(p+1)*ntiles]) after the
-// scan; it becomes ceil(rows/chmax) work items.
-__global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n,
-                                                      uint64_t chmax, AggPlan plan) {
+// (c) partition p has rows [pbase[p], pbase[p+1]); it becomes
+// ceil(rows/chmax) work items.
+__global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax,
+                                                      AggPlan plan) {
   __shared__ uint32_t scratch[17];
   if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
   __syncthreads();
   extern __shared__ uint32_t cnt[];  // P + 1
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-    const uint64_t b = offs[(uint64_t)p * ntiles];
-    const uint64_t e = p + 1 < P ? offs[(uint64_t)(p + 1) * ntiles] : n;
+    const uint64_t b = pbase[p];
+    const uint64_t e = p + 1 < P ? pbase[p + 1] : n;
     plan.pstart[p] = b;
     const uint64_t len = e - b;
     cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
@@ -194,9 +194,8 @@ __device__ __forceinline__ uint32_t xcd_local_index(uint32_t w, uint32_t count) 
   return (w % kXcds) * per + w / kXcds;
 }
 
-// Per-tile partition histogram, cnt[p][tile]; tiles renumbered so that the
-// counters of neighbouring tiles (adjacent words of cnt[p][*]) are written
-// from one XCD.
+// Per-tile partition histogram, tile-major: cnt[tile][p] (one contiguous
+// row of P counters per tile).
 template <typename K>
 __global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys, uint64_t n, int pbits,
                                                      uint32_t tile_rows, uint32_t ntiles, uint32_t *cnt) {
@@ -212,7 +211,64 @@ __global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys,
   for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
     atomicAdd(&hist[hash_partition(key_hash((uint64_t)keys[r]), pbits)], 1u);
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[(uint64_t)p * ntiles + tile] = hist[p];
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[(uint64_t)tile * P + p] = hist[p];
+}
+
+// Column-wise exclusive scan of the tile-major histogram: for every
+// partition p, offs[t][p] = sum of cnt[t'][p] over tiles t' < t, and
+// total[p] = the partition's row count.  Workgroups take chunks of kColTiles
+// tiles in ticket order; every column does its own decoupled look-back
+// (Merrill & Garland 2016) over the chunks before it, one lane per column,
+// so the whole scan reads and writes rows of P contiguous counters (a
+// partition-major layout would make every tile touch P scattered lines).
+constexpr uint32_t kColTiles = 32;
+__global__ void __launch_bounds__(256) k_colscan(const uint32_t *__restrict__ cnt, uint32_t ntiles, uint32_t P,
+                                                 uint64_t *desc, uint32_t *ticket, uint32_t *offs, uint32_t *total) {
+  __shared__ uint32_t s_chunk;
+  if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t c = wave_uniform(s_chunk);
+  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
+  if (c >= nchunks) return;
+  const uint32_t t0 = c * kColTiles, t1 = min(ntiles, t0 + kColTiles);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    uint64_t sum = 0;
+    for (uint32_t t = t0; t < t1; ++t) sum += cnt[(uint64_t)t * P + p];
+    uint64_t excl = 0;
+    if (c == 0) {
+      lb_store(&desc[p], kLbInc | sum);
+    } else {
+      lb_store(&desc[(uint64_t)c * P + p], kLbAgg | sum);
+      for (int64_t k = (int64_t)c - 1; k >= 0; --k) {  // every chunk k < c holds an earlier ticket
+        uint64_t d = lb_load(&desc[(uint64_t)k * P + p]);
+        while ((d >> 62) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          d = lb_load(&desc[(uint64_t)k * P + p]);
+        }
+        excl += d & kLbVal;
+        if ((d >> 62) == 2) break;  // an inclusive prefix ends the walk
+      }
+      lb_store(&desc[(uint64_t)c * P + p], kLbInc | (excl + sum));
+    }
+    uint64_t run = excl;
+    for (uint32_t t = t0; t < t1; ++t) {
+      offs[(uint64_t)t * P + p] = (uint32_t)run;
+      run += cnt[(uint64_t)t * P + p];
+    }
+    if (c == nchunks - 1) total[p] = (uint32_t)run;
+  }
+}
+
+// pbase[p] = rows of the partitions before p (pbase[P] = all rows): the
+// start of partition p in the partitioned copy.  One workgroup.
+__global__ void __launch_bounds__(1024) k_colbase(const uint32_t *total, uint32_t P, uint32_t *pbase) {
+  extern __shared__ uint32_t cnt[];  // P
+  __shared__ uint32_t scratch[17];
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = total[p];
+  __syncthreads();
+  const uint32_t all = block_exclusive_scan_lds(cnt, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) pbase[p] = cnt[p];
+  if (threadIdx.x == 0) pbase[P] = all;
 }
 
 // Exclusive scan of n u32 (total < 2^32), 4096 values per tile, tiles in
@@ -275,8 +331,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_u32(const uint32_t *__res
 // the current tile's write phase.  NC = 1: one value column.
 template <typename K, int ITEMS, int NC>
 __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
-                                                       uint32_t ntiles, const uint32_t *__restrict__ offs, K *okeys,
-                                                       PartOut o) {
+                                                       uint32_t ntiles, const uint32_t *__restrict__ offs,
+                                                       const uint32_t *__restrict__ pbase, K *okeys, PartOut o) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t scratch[17];
   const uint32_t P = 1u << pbits;
@@ -336,7 +392,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
     }
     __syncthreads();
     block_exclusive_scan_lds(hist, P, scratch);
-    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = offs[(uint64_t)p * ntiles + tile];
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = pbase[p] + offs[(uint64_t)tile * P + p];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
@@ -629,14 +685,14 @@ hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks,
   return hipGetLastError();
 }
 
-hipError_t launch_plan_chunks(const uint32_t *offs, uint32_t P, uint32_t ntiles, uint64_t n, uint64_t chmax,
-                              const AggPlan &plan, hipStream_t st) {
+hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
+                              hipStream_t st) {
   const size_t lds = (size_t)(P + 1) * 4;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)&k_plan_chunks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, offs, P, ntiles, n, chmax, plan);
+  hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, pbase, P, n, chmax, plan);
   return hipGetLastError();
 }
 
@@ -673,29 +729,48 @@ size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t til
 
 template <typename K, int NC>
 static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
-                                   uint32_t ntiles, const uint32_t *offs, void *okeys, const PartOut &o,
-                                   unsigned grid, hipStream_t st) {
+                                   uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase, void *okeys,
+                                   const PartOut &o, unsigned grid, hipStream_t st) {
   constexpr int ITEMS = 8;
   const unsigned block = tile_rows / ITEMS;
   const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
   hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC>, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n, pbits,
-                     ntiles, offs, (K *)okeys, o);
+                     ntiles, offs, pbase, (K *)okeys, o);
   return hipGetLastError();
 }
 
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, void *okeys,
-                               const PartOut &o, hipStream_t st, unsigned grid) {
+                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase,
+                               void *okeys, const PartOut &o, hipStream_t st, unsigned grid) {
   if (grid == 0 || grid > ntiles) grid = ntiles;
   grid = xcd_grid(grid);
   const bool one = s.ncols <= 1;
+#define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, offs, pbase, okeys, o, grid, st
   if (key_bytes == 4)
-    return one ? launch_scatter_t<uint32_t, 1>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st)
-               : launch_scatter_t<uint32_t, kMaxCols>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st);
-  return one ? launch_scatter_t<uint64_t, 1>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st)
-             : launch_scatter_t<uint64_t, kMaxCols>(keys, s, n, pbits, tile_rows, ntiles, offs, okeys, o, grid, st);
+    return one ? launch_scatter_t<uint32_t, 1>(TGA_SC_ARGS) : launch_scatter_t<uint32_t, kMaxCols>(TGA_SC_ARGS);
+  return one ? launch_scatter_t<uint64_t, 1>(TGA_SC_ARGS) : launch_scatter_t<uint64_t, kMaxCols>(TGA_SC_ARGS);
+#undef TGA_SC_ARGS
+}
+
+uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P) {
+  return (uint64_t)((ntiles + kColTiles - 1) / kColTiles) * P;
+}
+
+hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
+                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st) {
+  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
+  hipLaunchKernelGGL(k_colscan, dim3(nchunks), dim3(256), 0, st, cnt, ntiles, P, desc, ticket, offs, total);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t lds = (size_t)P * 4;
+  if (lds > 65536) {
+    e = hipFuncSetAttribute((const void *)&k_colbase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_colbase, dim3(1), dim3(1024), lds, st, total, P, pbase);
+  return hipGetLastError();
 }
 
 uint64_t compact_tiles(uint64_t slots_plus_one) {
