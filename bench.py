@@ -56,8 +56,8 @@ def parse():
 
 
 # profiler name of a timed step -> kernel symbol it launches
-KERNEL_SYMBOL = {"part_scatter": "k_part_scatter", "agg_partitions": "k_agg", "agg_direct": "k_agg",
-                 "part_count": "k_part_count", "scan": "k_scan_u32", "compact": "k_compact", "copy16": "k_copy16"}
+KERNEL_SYMBOL = {"part_scatter": "k_part_bucket", "agg_partitions": "k_agg", "agg_direct": "k_agg",
+                 "compact": "k_compact", "copy16": "k_copy16"}
 
 
 def pmc_pass(counter, argv, workdir):
@@ -178,7 +178,6 @@ def headline(ctx, tga, args):
         "agg_partitions": n * row_b + G * 32,
         "agg_direct": n * row_b + G * 32,
         "part_scatter": 2 * n * row_b,
-        "part_count": n * 8,
     }.get(dom_name, n * row_b)
     achieved = alg / (dom_avg_ms * 1e-3) / 1e9
     op_alg = n * row_b + G * 32

@@ -405,11 +405,13 @@ hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable
 
 static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 
-// Persistent scatter grid: as many workgroups as the LDS lets run at once.
-static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan) {
+// Persistent scatter grid: as many workgroups as the LDS lets run at once,
+// with at least 8 tiles each.
+static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan, uint64_t m) {
   const size_t lds = scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
   const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / std::max<size_t>(lds, 1)));
-  return (unsigned)(ctx->cus * per_cu);
+  const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
+  return bucket_grid(ntiles, (unsigned)(ctx->cus * per_cu));
 }
 
 // Plan buffer layout of the partitioned path: pstart[P+1] u64 | cstart[P+1]
@@ -428,55 +430,58 @@ static AggPlan part_plan(const DBuf &buf, uint32_t P) {
 // Buffers of one partitioning pass; they must outlive the work queued on
 // the stream.
 struct PartBufs {
-  DBuf cnt, offs, total, pbase, desc, pkeys, planbuf;
+  DBuf heads, meta, pkeys, planbuf;
   DBuf pcols[kMaxCols];
+  BucketBufs bb{};
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
   void keep(std::vector<DBuf> &k) {
-    for (DBuf *b : {&cnt, &offs, &total, &pbase, &desc, &pkeys, &planbuf}) k.push_back(std::move(*b));
+    for (DBuf *b : {&heads, &meta, &pkeys, &planbuf}) k.push_back(std::move(*b));
     for (auto &c : pcols)
       if (c.p) k.push_back(std::move(c));
   }
 };
 
-// (b) + (c) for rows [0, m) of keys / s.cols: tile histogram, column scan,
-// scatter into the partitioned copy, work-item plan.
+// (b) + (c) for rows [0, m) of keys / s.cols: count-free scatter into
+// buckets, then the per-partition bucket lists and the work-item plan.
 static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s, uint64_t m,
                                  const Plan &plan, PartBufs &pb) {
   hipStream_t st = ctx->stream;
   const uint32_t P = 1u << plan.pbits;
-  const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
-  const uint64_t cells = (uint64_t)P * ntiles;
-  const uint64_t dwords = colscan_desc_words(ntiles, P);
-  if (!pb.cnt.alloc(ctx, cells * 4) || !pb.offs.alloc(ctx, cells * 4) || !pb.total.alloc(ctx, (uint64_t)P * 4) ||
-      !pb.pbase.alloc(ctx, (uint64_t)(P + 1) * 4) || !pb.desc.alloc(ctx, (dwords + 1) * 8) ||
-      !pb.pkeys.alloc(ctx, m * q.key_bytes) || !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
+  const unsigned grid = scatter_grid(ctx, q, plan, m);
+  const uint64_t cap = bucket_cap(m, grid, plan.pbits, plan.tile_rows);
+  if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
+  const uint64_t slots = cap << kBucketShift;
+  // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
+  const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
+  if (!pb.heads.alloc(ctx, cap * 16) || !pb.meta.alloc(ctx, meta_words * 8) ||
+      !pb.pkeys.alloc(ctx, slots * q.key_bytes) || !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   pb.sp = s;
   for (int c = 0; c < s.ncols; ++c) {
-    if (!pb.pcols[c].alloc(ctx, m * (uint64_t)dtype_bytes(s.col_dtype[c])))
+    if (!pb.pcols[c].alloc(ctx, slots * (uint64_t)dtype_bytes(s.col_dtype[c])))
       return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
     po.cols[c] = pb.pcols[c].p;
     pb.sp.cols[c] = pb.pcols[c].p;
   }
-  TGA_HIP(ctx, launch_fill_u64(pb.desc.as<uint64_t>(), dwords + 1, 0, st));
-  TGA_HIP(ctx, timed(ctx, "part_count", [&] {
-    return launch_part_count(q.key_bytes, keys, m, plan.pbits, plan.tile_rows, ntiles, pb.cnt.as<uint32_t>(), st);
-  }));
-  TGA_HIP(ctx, timed(ctx, "scan", [&] {
-    return launch_colscan(pb.cnt.as<uint32_t>(), ntiles, P, pb.desc.as<uint64_t>(),
-                          (uint32_t *)(pb.desc.as<uint64_t>() + dwords), pb.offs.as<uint32_t>(),
-                          pb.total.as<uint32_t>(), pb.pbase.as<uint32_t>(), st);
-  }));
+  BucketBufs &bb = pb.bb;
+  bb.blist = pb.heads.as<uint64_t>();
+  bb.bpart = (uint32_t *)(bb.blist + cap);
+  bb.bcnt = bb.bpart + cap;
+  uint64_t *meta = pb.meta.as<uint64_t>();
+  bb.pool = (uint32_t *)meta;
+  bb.pbk = (uint32_t *)(meta + 1);
+  bb.bcur = bb.pbk + P;
+  bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
+  bb.cap = (uint32_t)cap;
+  TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, st));
   TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
-    return launch_part_scatter(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, ntiles, pb.offs.as<uint32_t>(),
-                               pb.pbase.as<uint32_t>(), pb.pkeys.p, po, st, scatter_grid(ctx, q, plan));
+    return launch_part_bucket(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, pb.pkeys.p, po, bb, grid, st);
   }));
   pb.ap = part_plan(pb.planbuf, P);
-  TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
-    return launch_plan_chunks(pb.pbase.as<uint32_t>(), P, m, plan.chunk_rows, pb.ap, st);
-  }));
+  pb.ap.blist = bb.blist;
+  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st); }));
   return TGA_OK;
 }
 
@@ -577,8 +582,13 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   }
   const AggPlan &ap = pb.ap;
   uint64_t split_rows = 0;
+  uint32_t used = 0;
   TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
+  // the scatter never takes more buckets than bucket_cap(); a larger count
+  // would mean dropped rows
+  if (used > pb.bb.cap) return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
   // global table: split partitions and spills only
   uint64_t S = plan.gslots;
   if (!full_table) {

@@ -17,6 +17,10 @@ namespace tga {
 
 constexpr int kMaxCols = 4;
 constexpr int kMaxAccs = 8;
+// rows per bucket of the partitioned copy (one wave step of the aggregation:
+// 64 lanes x 4 rows)
+constexpr int kBucketShift = 8;
+constexpr uint32_t kBucketRows = 1u << kBucketShift;
 constexpr int kMaxFx = 2;  // exact fp64 SUM accumulators per query
 
 // Exact fp64 SUM: a double is split into 32-bit chunks of the fixed-point
@@ -136,6 +140,14 @@ TGA_HD uint32_t fmix32(uint32_t h) {
 //   global slot= bits 16..47
 TGA_HD uint64_t key_hash(uint64_t k) { return mix64(k ^ 0x5851f42d4c957f2dULL); }
 TGA_HD uint32_t hash_partition(uint64_t h, int pbits) { return pbits ? (uint32_t)(h >> (64 - pbits)) : 0u; }
+// Partition of a key in the plain operator's partitioned path (pbits <= 32):
+// the top bits of k * 2^64/phi mod 2^64 (Fibonacci hashing, Knuth TAOCP
+// vol. 3, 6.4).  Only the high product word is formed: three 32-bit
+// multiplies instead of key_hash's two 64-bit ones, and a function unrelated
+// to the key_hash bits that place a key inside its partition's tables.
+TGA_HD uint32_t key_partition(uint64_t k, int pbits) {
+  return pbits ? (uint32_t)((k * 0x9e3779b97f4a7c15ULL) >> 32) >> (32 - pbits) : 0u;
+}
 TGA_HD uint32_t hash_lds_slot(uint64_t h, uint32_t slots) {
   return (uint32_t)(((uint64_t)(uint32_t)h * slots) >> 32);
 }

@@ -379,20 +379,59 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint
   return total;
 }
 
-// Aggregates rows [b, e) into the LDS table.  Rows whose key finds no free
-// slot within max_probe go straight to the global table.  FX: the schema has
-// exact fp64 SUM accumulators (their limb vectors cost registers, so the
-// common integer case is compiled without them).
+// ---------------------------------------------------------------------------
+// rows of a work item
+//
+// Contiguous mode: rows [b, e), taken by the waves in blocks of kBucketRows
+// rows (wave w takes blocks w, w+nw, ...).  Bucket mode (ents != null): b..e
+// index a list of bucket entries, rows << 32 | id; bucket `id` holds `rows`
+// (1..kBucketRows) rows at [id*kBucketRows, ...) of the partitioned copy, and
+// one bucket is one wave block.
+// ---------------------------------------------------------------------------
+
+struct BucketList {
+  const uint64_t *ents;  // null: contiguous mode
+};
+
+// List entry of block i of wave `wave` (0 past the end; 0 in contiguous mode).
+__device__ __forceinline__ uint64_t list_entry(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
+                                               uint32_t nw) {
+  if (!bl.ents) return 0;
+  const uint64_t k = b + wave + i * nw;
+  return k < e ? bl.ents[k] : 0;
+}
+
+// Block i of wave `wave` (wave-uniform), given its list entry: rows
+// [bb, be); false past the end.
+__device__ __forceinline__ bool wave_block(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
+                                           uint32_t nw, uint64_t ent, uint64_t &bb, uint64_t &be) {
+  if (bl.ents) {
+    if ((ent >> 32) == 0) return false;
+    bb = (ent & 0xffffffffULL) << kBucketShift;
+    be = bb + (ent >> 32);
+    return true;
+  }
+  bb = b + (wave + i * nw) * (uint64_t)kBucketRows;
+  be = e;
+  return bb < e;
+}
+
+// Aggregates the item's rows (see wave_block) into the LDS table.  Rows whose
+// key finds no free slot within max_probe go straight to the global table.
+// FX: the schema has exact fp64 SUM accumulators (their limb vectors cost
+// registers, so the common integer case is compiled without them).
 template <typename K, bool FX, int NC = kMaxCols>
-__device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b, uint64_t e,
-                         const LdsTable<K> &t, const GTable &g, int nbits, int max_probe) {
+__device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b0, uint64_t e0,
+                         const LdsTable<K> &t, const GTable &g, int nbits, int max_probe,
+                         const BucketList bl = BucketList{nullptr}) {
   constexpr int U = 4;
   constexpr int NF = FX ? kMaxFx : 1;
   const int lane = (int)__lane_id();
-  const int wave = (int)(threadIdx.x >> 6);
-  const int nw = (int)(blockDim.x >> 6);
+  const uint32_t wave = wave_uniform(threadIdx.x >> 6);
+  const uint32_t nw = blockDim.x >> 6;
   const int ncol = s.ncols, nacc = s.nacc;
-  for (uint64_t base = b + (uint64_t)wave * kWave * U; base < e; base += (uint64_t)nw * kWave * U) {
+  uint64_t base = 0, e = 0;
+  for (uint64_t it = 0; wave_block(bl, b0, e0, it, wave, nw, list_entry(bl, b0, e0, it, wave, nw), base, e); ++it) {
     K kk[U];
     uint64_t cell[U][kMaxCols];
 #pragma unroll
@@ -560,38 +599,62 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 // had consumed, from which the host sizes a finer partitioning.
 // spilled (LDS word, optional, spill mode): set when a row went to the
 // global table.
+//
+// The loop is software-pipelined: the next block's key and value loads are
+// issued before the current block's LDS work, so every wave keeps its HBM
+// reads in flight while it probes and updates the table.
+template <typename K, int NC, int VW, int U>
+__device__ __forceinline__ void load_block(const K *__restrict__ keys, const void *const *cols, uint32_t ncols,
+                                           uint32_t coldt, uint64_t base, uint64_t e, int lane, K *kk,
+                                           uint64_t (*raw)[NC]) {
+  // rows past the end re-read the block's last row (in bounds, no branch),
+  // so all U key and value loads are in flight together
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t row = base + (uint64_t)u * kWave + lane;
+    const uint64_t r = row < e ? row : e - 1;
+    kk[u] = keys[r];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
+  }
+}
+
 template <typename K, int NC, int VW, bool FOLD>
-__device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b,
-                                             uint64_t e, const LdsTable<K> &t, const AccProg &ap, int nbits,
+__device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b0,
+                                             uint64_t e0, const LdsTable<K> &t, const AccProg &ap, int nbits,
                                              int max_probe, const GTable *gsp, const Schema *ssp,
-                                             uint32_t *giveup = nullptr, uint32_t *spilled = nullptr) {
+                                             uint32_t *giveup = nullptr, uint32_t *spilled = nullptr,
+                                             const BucketList bl = BucketList{nullptr}) {
   constexpr int U = 4;
+  static_assert(kWave * U == (int)kBucketRows, "one bucket is one wave block");
   const int lane = (int)__lane_id();
-  const int wave = (int)(threadIdx.x >> 6);
-  const int nw = (int)(blockDim.x >> 6);
+  const uint32_t wave = wave_uniform(threadIdx.x >> 6);
+  const uint32_t nw = blockDim.x >> 6;
   const uint32_t S = t.L.slots;
   const uint32_t nb = S >> 2;
   K *tk = t.keys();
   char *tb = t.base;
   const uint32_t ncols = ap.ncols, coldt = ap.coldt;
   uint32_t consumed = 0;
-  for (uint64_t base = b + (uint64_t)wave * kWave * U; base < e; base += (uint64_t)nw * kWave * U) {
+  // bucket mode: the list entry of a block is read two blocks ahead, so no
+  // wave waits on it before issuing that block's row loads
+  uint64_t base = 0, e = 0;
+  bool have = wave_block(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, e);
+  uint64_t ent_next = list_entry(bl, b0, e0, 1, wave, nw);
+  K kk[U];
+  uint64_t raw[U][NC];
+  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, e, lane, kk, raw);
+  for (uint64_t it = 1; have; ++it) {
     if (giveup && wave_uniform(*(volatile uint32_t *)giveup)) break;
     consumed += kWave * U;
-    K kk[U];
-    uint64_t raw[U][NC];
+    uint64_t nbase = 0, ne = 0;
+    const bool next = wave_block(bl, b0, e0, it, wave, nw, ent_next, nbase, ne);
+    K nkk[U];
+    uint64_t nraw[U][NC];
+    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ne, lane, nkk, nraw);
+    ent_next = list_entry(bl, b0, e0, it + 1, wave, nw);
     uint64_t hh[U];
     int slot[U];
-    // loads: rows past the end re-read the last row (in bounds, no branch),
-    // so all U key and value loads are in flight together
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t row = base + (uint64_t)u * kWave + lane;
-      const uint64_t r = row < e ? row : e - 1;
-      kk[u] = keys[r];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
-    }
     // first buckets of all U rows, read before any is resolved
     uint32_t s0[U];
     K k4[U][4];
@@ -724,6 +787,17 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
             break;
           default: break;
         }
+      }
+    }
+    have = next;
+    if (next) {
+      base = nbase;
+      e = ne;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        kk[u] = nkk[u];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) raw[u][c] = nraw[u][c];
       }
     }
   }

@@ -9,13 +9,17 @@
 namespace tga {
 
 // Work plan of k_agg: segment p covers rows [pstart[p], pstart[p+1]) and is
-// cut into cstart[p+1]-cstart[p] slices; ticket hands out slice ids.
+// cut into cstart[p+1]-cstart[p] slices; ticket hands out slice ids.  With a
+// bucket list (blist != null) pstart indexes blist instead: segment p is the
+// buckets blist[pstart[p] .. pstart[p+1]), each entry rows << 32 | id for the
+// bucket holding `rows` rows at rows [id * kBucketRows, ...).
 struct AggPlan {
   uint64_t *pstart;  // [P+1]
   uint32_t *cstart;  // [P+1]
   uint32_t P;
   uint32_t *ticket;
-  uint64_t *split_rows;  // rows of partitions cut into more than one item (k_plan_chunks)
+  uint64_t *split_rows;  // rows of partitions cut into more than one item
+  const uint64_t *blist;
 };
 
 struct PartOut {
@@ -35,25 +39,32 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
                       int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0,
                       uint32_t *ovf = nullptr);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
-hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
-                              hipStream_t st);
-hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
-                             uint32_t ntiles, uint32_t *cnt, hipStream_t st);
 uint64_t scan_tiles(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
                            hipStream_t st);
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows);
-// Column-wise scan of the tile-major histogram cnt[ntiles][P]: offs (same
-// shape, exclusive per partition), total[P], pbase[P+1] (partition starts).
-// desc: colscan_desc_words() zeroed words followed by a zeroed ticket word.
-uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P);
-hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
-                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st);
-// grid: persistent workgroups (each takes a contiguous range of tiles);
-// 0 = one per tile
-hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase,
-                               void *okeys, const PartOut &o, hipStream_t st, unsigned grid = 0);
+
+// Buckets of the count-free partitioning: bpart/bcnt [cap] u32, blist [cap]
+// u64 entries; pool (the allocation counter), pbk [P] u32, prow [P] u64 and
+// bcur [P] u32 start zeroed.
+struct BucketBufs {
+  uint32_t *bpart, *bcnt, *pool, *pbk, *bcur;
+  uint64_t *blist, *prow;
+  uint32_t cap;
+};
+// persistent workgroups of the scatter for ntiles tiles (at most max_grid)
+unsigned bucket_grid(uint32_t ntiles, unsigned max_grid);
+// buckets the scatter can take: ceil(n / kBucketRows) + grid * (2^pbits +
+// its workgroup size)
+uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows);
+// rows [0, n) into buckets of the partitioned copy okeys / o.cols
+hipError_t launch_part_bucket(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                              uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
+                              hipStream_t st);
+// bucket headers -> per-partition bucket lists and the k_agg work plan
+// (items of at most about chmax rows); plan.blist/bcnt must point at bb's
+hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
+                               hipStream_t st);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,

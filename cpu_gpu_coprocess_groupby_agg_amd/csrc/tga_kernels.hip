@@ -1,19 +1,23 @@
 // tga_kernels.hip -- gfx950 kernels of the plain group-by operator.
 //
-//   (a) k_agg          LDS open-addressing pre-aggregation with wave64 match
-//                      folding; flushes each workgroup table into the global
-//                      table.  Used directly on the input when the groups fit
-//                      one LDS table, and per partition after (b).
-//   (b) k_part_count   per-tile partition histogram
-//       k_scan_u32     decoupled-look-back exclusive scan of the counts
-//       k_part_scatter LDS counting sort of a tile, then coalesced runs
-//   (c) k_plan_chunks  partition -> work-item plan for k_agg
-//   (d) k_compact      one-pass compaction of the global table (own look-back)
+//   (a) k_agg           LDS open-addressing pre-aggregation with wave64 match
+//                       folding; flushes each workgroup table into the global
+//                       table or emits whole partitions.  Used directly on the
+//                       input when the groups fit one LDS table, and per
+//                       partition after (b).
+//   (b) k_part_bucket   count-free partitioning: LDS counting sort of a tile,
+//                       then each partition's run appended to the workgroup's
+//                       open 256-row bucket of that partition
+//   (c) k_bucket_count, k_bucket_plan, k_bucket_list
+//                       bucket headers -> per-partition bucket lists and the
+//                       work-item plan of k_agg
+//   (d) k_compact       one-pass compaction of the global table (own look-back)
+//       k_scan_u32      decoupled-look-back exclusive scan (the scan entry point)
 //
 // Reference counterparts (T/ = tiled_groupby_agg_partition/):
 //   K1-K5 pre-aggregation      T/src/kernel.cu:9-186     -> k_agg
 //   K6/K7 + CUB scan            T/src/kernel.cu:189-291, T/src/util.cu:68-88
-//                                                        -> k_part_count/k_scan_u32/k_part_scatter
+//                                                        -> k_part_bucket + (c)
 //   K8/K9 intra-partition agg   T/src/kernel.cu:294-378  -> k_agg over partitions
 //   S5 + K10 compaction         T/src/kernel.cu:380-406  -> k_compact
 #include "tga_device.hpp"
@@ -115,22 +119,25 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
       s_abort[1] = 0;
     }
     __syncthreads();
+    const BucketList bl{plan.blist};
     if constexpr (FX) {
-      agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe);
+      agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe, bl);
     } else {
       const void *cols[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
       agg_rows_int<K, NC, VW, FOLD>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
-                                    abandon ? s_abort : nullptr);
+                                    abandon ? s_abort : nullptr, nullptr, bl);
     }
     __syncthreads();
     if (abandon && s_abort[0]) {
       if (threadIdx.x == 0) {
         const uint32_t i = atomicAdd(ovf, 1u);
         if (i < ovf[1]) {
+          // rows of the item (buckets counted full)
+          const uint64_t rows = plan.blist ? (e - b) << kBucketShift : e - b;
           ovf[2 + 2 * i] = s_abort[1];
-          ovf[3 + 2 * i] = (uint32_t)(e - b);
+          ovf[3 + 2 * i] = (uint32_t)min(rows, (uint64_t)UINT32_MAX);
         }
       }
     } else if (whole) {
@@ -153,29 +160,6 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
   }
 }
 
-// (c) partition p has rows [pbase[p], pbase[p+1]); it becomes
-// ceil(rows/chmax) work items.
-__global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax,
-                                                      AggPlan plan) {
-  __shared__ uint32_t scratch[17];
-  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
-  __syncthreads();
-  extern __shared__ uint32_t cnt[];  // P + 1
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-    const uint64_t b = pbase[p];
-    const uint64_t e = p + 1 < P ? pbase[p + 1] : n;
-    plan.pstart[p] = b;
-    const uint64_t len = e - b;
-    cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
-    if (cnt[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
-  }
-  if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
-  __syncthreads();
-  const uint32_t total = block_exclusive_scan_lds(cnt, P, scratch);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) plan.cstart[p] = cnt[p];
-  if (threadIdx.x == 0) plan.cstart[P] = total;
-}
-
 // ---------------------------------------------------------------------------
 // (b) partitioning
 // ---------------------------------------------------------------------------
@@ -192,83 +176,6 @@ __host__ __device__ inline uint32_t xcd_grid(uint32_t count) { return (count + k
 __device__ __forceinline__ uint32_t xcd_local_index(uint32_t w, uint32_t count) {
   const uint32_t per = (count + kXcds - 1) / kXcds;
   return (w % kXcds) * per + w / kXcds;
-}
-
-// Per-tile partition histogram, tile-major: cnt[tile][p] (one contiguous
-// row of P counters per tile).
-template <typename K>
-__global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys, uint64_t n, int pbits,
-                                                     uint32_t tile_rows, uint32_t ntiles, uint32_t *cnt) {
-  extern __shared__ uint32_t hist[];
-  const uint32_t P = 1u << pbits;
-  const uint32_t tile = xcd_local_index(blockIdx.x, ntiles);
-  if (tile >= ntiles) return;  // uniform per workgroup
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-  __syncthreads();
-  const uint64_t b = (uint64_t)tile * tile_rows;
-  const uint64_t e = min(n, b + tile_rows);
-#pragma unroll 8
-  for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
-    atomicAdd(&hist[hash_partition(key_hash((uint64_t)keys[r]), pbits)], 1u);
-  __syncthreads();
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[(uint64_t)tile * P + p] = hist[p];
-}
-
-// Column-wise exclusive scan of the tile-major histogram: for every
-// partition p, offs[t][p] = sum of cnt[t'][p] over tiles t' < t, and
-// total[p] = the partition's row count.  Workgroups take chunks of kColTiles
-// tiles in ticket order; every column does its own decoupled look-back
-// (Merrill & Garland 2016) over the chunks before it, one lane per column,
-// so the whole scan reads and writes rows of P contiguous counters (a
-// partition-major layout would make every tile touch P scattered lines).
-constexpr uint32_t kColTiles = 32;
-__global__ void __launch_bounds__(256) k_colscan(const uint32_t *__restrict__ cnt, uint32_t ntiles, uint32_t P,
-                                                 uint64_t *desc, uint32_t *ticket, uint32_t *offs, uint32_t *total) {
-  __shared__ uint32_t s_chunk;
-  if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t c = wave_uniform(s_chunk);
-  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
-  if (c >= nchunks) return;
-  const uint32_t t0 = c * kColTiles, t1 = min(ntiles, t0 + kColTiles);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-    uint64_t sum = 0;
-    for (uint32_t t = t0; t < t1; ++t) sum += cnt[(uint64_t)t * P + p];
-    uint64_t excl = 0;
-    if (c == 0) {
-      lb_store(&desc[p], kLbInc | sum);
-    } else {
-      lb_store(&desc[(uint64_t)c * P + p], kLbAgg | sum);
-      for (int64_t k = (int64_t)c - 1; k >= 0; --k) {  // every chunk k < c holds an earlier ticket
-        uint64_t d = lb_load(&desc[(uint64_t)k * P + p]);
-        while ((d >> 62) == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          d = lb_load(&desc[(uint64_t)k * P + p]);
-        }
-        excl += d & kLbVal;
-        if ((d >> 62) == 2) break;  // an inclusive prefix ends the walk
-      }
-      lb_store(&desc[(uint64_t)c * P + p], kLbInc | (excl + sum));
-    }
-    uint64_t run = excl;
-    for (uint32_t t = t0; t < t1; ++t) {
-      offs[(uint64_t)t * P + p] = (uint32_t)run;
-      run += cnt[(uint64_t)t * P + p];
-    }
-    if (c == nchunks - 1) total[p] = (uint32_t)run;
-  }
-}
-
-// pbase[p] = rows of the partitions before p (pbase[P] = all rows): the
-// start of partition p in the partitioned copy.  One workgroup.
-__global__ void __launch_bounds__(1024) k_colbase(const uint32_t *total, uint32_t P, uint32_t *pbase) {
-  extern __shared__ uint32_t cnt[];  // P
-  __shared__ uint32_t scratch[17];
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = total[p];
-  __syncthreads();
-  const uint32_t all = block_exclusive_scan_lds(cnt, P, scratch);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) pbase[p] = cnt[p];
-  if (threadIdx.x == 0) pbase[P] = all;
 }
 
 // Exclusive scan of n u32 (total < 2^32), 4096 values per tile, tiles in
@@ -317,29 +224,66 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_u32(const uint32_t *__res
   }
 }
 
-// Tiles of blockDim*ITEMS rows (the tiles of k_part_count): rank each row
-// inside its partition with an LDS histogram atomic, counting-sort the tile
-// into LDS, then write every partition's run contiguously at its scanned
-// offset (the LDS acts as the write-combining buffer).  Persistent and
-// XCD-aware: a workgroup takes one contiguous range of tiles.  The run of
-// partition p from tile t ends where the run from tile t+1 begins, so the
-// partial cache lines at the run ends are completed by the same workgroup a
-// moment later, in the same XCD's L2, and reach HBM as whole lines (with one
-// tile per workgroup, neighbouring tiles run on different XCDs, whose L2s
-// cannot merge them).  The next tile's rows are loaded into registers as
-// soon as the current tile is staged, so those reads are in flight during
-// the current tile's write phase.  NC = 1: one value column.
-template <typename K, int ITEMS, int NC>
-__global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
-                                                       uint32_t ntiles, const uint32_t *__restrict__ offs,
-                                                       const uint32_t *__restrict__ pbase, K *okeys, PartOut o) {
+// Count-free partitioning into buckets (replaces the reference's histogram
+// K6, CUB scan and scatter K7 with one pass over the rows).
+//
+// The rows are cut into tiles of blockDim*ITEMS rows and a persistent
+// workgroup takes one contiguous range of tiles.  For each tile it ranks
+// every row inside its partition with an LDS histogram atomic and
+// counting-sorts the tile into LDS, which serves as the write-combining
+// buffer.  Each partition's run is then appended to that partition's open
+// bucket of this workgroup: a block of kBucketRows rows of the partitioned
+// copy.  A workgroup keeps one open bucket per partition, so the run of a
+// partition from its next tile continues the same bucket and completes the
+// cache lines this tile left partial, in the same XCD's L2; buckets are
+// line-aligned, so no line is shared between workgroups.  No histogram pass
+// has to run first: a partition's rows need not be contiguous, only grouped
+// in buckets, which k_bucket_count / k_bucket_plan / k_bucket_list turn into
+// one bucket list per partition for k_agg.
+//
+// Bucket ids come from an atomic pool counter.  A run that needs one fresh
+// bucket (the common case: runs are much shorter than a bucket) takes it
+// from its wave's reserve, refilled kReserve ids at a time with one atomic,
+// so most tiles need no global atomic at all; a run needing several takes
+// them, contiguous, with its own atomic.  Ids left in the reserves at the end
+// are marked unused (bpart = ~0).
+//
+// Bucket headers: bpart[id] = partition (~0: unused); bcnt[id] = rows
+// (kBucketRows for every bucket except the last one of each (workgroup,
+// partition)).  At most ceil(n / kBucketRows) + gridDim.x * (P + blockDim.x)
+// ids are taken (full buckets, one open bucket per workgroup and partition,
+// and a reserve of < kReserve per wave); the host sizes `cap` to that bound,
+// and no id >= cap is ever written.
+//
+// With one value column the next tile's rows are loaded into a second
+// register set right after this tile is ranked, so they have this tile's
+// remaining phases to arrive; with more columns (whose second register set
+// would spill) they are loaded after this tile is staged, during its writes.
+// VW: width of every value cell when NC == 1 (4 or 8), 0 = per column.
+constexpr uint32_t kReserve = 64;
+
+__host__ __device__ inline uint32_t bucket_of(uint32_t k, uint32_t first, uint32_t delta) {
+  return k == 0 ? first : delta + k;
+}
+
+template <typename K, int ITEMS, int NC, int VW>
+__global__ void __launch_bounds__(1024) k_part_bucket(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
+                                                      uint32_t ntiles, K *okeys, PartOut o, uint32_t *bpart,
+                                                      uint32_t *bcnt, uint32_t *pool, uint32_t cap) {
+  using CellT = typename std::conditional<VW == 4, uint32_t, uint64_t>::type;
+  constexpr bool kEarly = NC == 1;
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t scratch[17];
   const uint32_t P = 1u << pbits;
   const uint32_t TR = blockDim.x * ITEMS;
-  uint32_t *hist = (uint32_t *)smem;
-  uint32_t *goff = hist + P;
-  char *stage = (char *)(goff + P);
+  const int lane = (int)__lane_id();
+  // wdesc[p] = {q0, first, delta, -}: row j of the tile's run of p goes to
+  // position q = j + q0 of the bucket chain bucket_of(q >> shift, first, delta)
+  uint4 *wdesc = (uint4 *)smem;
+  uint32_t *hist = (uint32_t *)(wdesc + P);  // tile counts, then exclusive offsets
+  uint32_t *fill = hist + P;                 // rows in the open bucket (0: none open)
+  uint32_t *cur = fill + P;                  // the open bucket
+  char *stage = (char *)(cur + P);
   // stage layout: K keys[TR] | per column [TR] cells | u16 pid[TR]
   K *skey = (K *)stage;
   char *scol = stage + (size_t)TR * sizeof(K);
@@ -349,36 +293,41 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
   size_t acc_off = 0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    colb[c] = c < ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
+    colb[c] = VW ? (uint32_t)VW : c < ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
     coff[c] = acc_off;
     acc_off += (size_t)TR * colb[c];
   }
   uint16_t *spid = (uint16_t *)(scol + acc_off);
 
-  K kk[ITEMS];
-  uint64_t cell[ITEMS][NC];
+  K kk[ITEMS], nkk[ITEMS];
+  CellT cell[ITEMS][NC], ncell[ITEMS][NC];
+  uint32_t rlo = 0, rhi = 0;  // this wave's reserve of bucket ids (wave-uniform)
   // chunk of consecutive tiles; chunks renumbered so the chunks of one XCD
   // are neighbours too (the grid is a multiple of the XCD count)
   const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
   uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
   const uint32_t tend = min(ntiles, tile + per);
-  if (tile >= tend) return;
+  if (tile >= tend) return;  // uniform per workgroup
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    fill[p] = 0;
+    cur[p] = 0;
+  }
   // rows past the end re-read the last row (in bounds); they are not staged
-#define TGA_SCATTER_LOAD(T)                                                                     \
+#define TGA_SCATTER_LOAD(T, KK, CELL)                                                           \
   {                                                                                             \
     const uint64_t b0 = (uint64_t)(T) * TR;                                                     \
     _Pragma("unroll") for (int i = 0; i < ITEMS; ++i) {                                         \
       const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;                         \
       const uint64_t r = row < n ? row : n - 1;                                                 \
-      kk[i] = keys[r];                                                                          \
+      KK[i] = keys[r];                                                                          \
       _Pragma("unroll") for (int c = 0; c < NC; ++c) {                                          \
-        if (c >= ncol) { cell[i][c] = 0; continue; }                                            \
-        cell[i][c] = colb[c] == 4 ? (uint64_t)((const uint32_t *)s.cols[c])[r]                  \
-                                  : ((const uint64_t *)s.cols[c])[r];                           \
+        if (c >= ncol) { CELL[i][c] = 0; continue; }                                            \
+        CELL[i][c] = colb[c] == 4 ? (CellT)((const uint32_t *)s.cols[c])[r]                     \
+                                  : (CellT)((const uint64_t *)s.cols[c])[r];                    \
       }                                                                                         \
     }                                                                                           \
   }
-  TGA_SCATTER_LOAD(tile)
+  TGA_SCATTER_LOAD(tile, kk, cell)
   for (;;) {
     const uint64_t b = (uint64_t)tile * TR;
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
@@ -387,12 +336,66 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-      pp[i] = hash_partition(key_hash((uint64_t)kk[i]), pbits);
+      pp[i] = key_partition((uint64_t)kk[i], pbits);
       rk[i] = row < n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
     }
+    const uint32_t next = tile + 1;
+    if constexpr (kEarly) {
+      if (next < tend) TGA_SCATTER_LOAD(next, nkk, ncell)
+    }
     __syncthreads();
-    block_exclusive_scan_lds(hist, P, scratch);
-    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = pbase[p] + offs[(uint64_t)tile * P + p];
+    const uint32_t rows = block_exclusive_scan_lds(hist, P, scratch);
+    // extend every partition's bucket chain by its run of this tile
+    for (uint32_t p0 = 0; p0 < P; p0 += blockDim.x) {
+      const uint32_t p = p0 + threadIdx.x;
+      const bool on = p < P;
+      uint32_t off = 0, c = 0, f = 0, first = 0, fresh = 0;
+      if (on) {
+        off = hist[p];
+        c = (p + 1 < P ? hist[p + 1] : rows) - off;
+        f = fill[p];
+        first = cur[p];
+        if (c) fresh = ((f + c + kBucketRows - 1) >> kBucketShift) - (f ? 1u : 0u);
+      }
+      // single fresh buckets: from the wave's reserve
+      const bool one = fresh == 1;
+      const uint64_t m1 = __ballot(one);
+      const uint32_t n1 = (uint32_t)__builtin_popcountll(m1);
+      const uint32_t j = (uint32_t)__builtin_popcountll(m1 & lanemask_lt());
+      uint32_t base = 0;
+      if (n1) {
+        const uint32_t have = rhi - rlo;
+        uint32_t got = 0;
+        if (have < n1) {  // refill (wave-uniform branch)
+          if (lane == 0) got = atomicAdd(pool, kReserve);
+          got = wave_uniform((uint32_t)__shfl((int)got, 0, kWave));
+        }
+        if (one) base = j < have ? rlo + j : got + (j - have);
+        if (have < n1) {
+          rlo = got + (n1 - have);
+          rhi = got + kReserve;
+        } else {
+          rlo += n1;
+        }
+      }
+      if (fresh > 1) base = atomicAdd(pool, fresh);  // several: contiguous, own atomic
+      uint32_t delta = 0;
+      if (on && c) {
+        for (uint32_t k = 0; k < fresh; ++k)
+          if (base + k < cap) bpart[base + k] = p;
+        delta = f ? base - 1 : base;
+        if (!f) first = base;
+        const uint32_t end = f + c;
+        const uint32_t full = end >> kBucketShift;  // buckets this run completes
+        for (uint32_t k = 0; k < full; ++k) {
+          const uint32_t id = bucket_of(k, first, delta);
+          if (id < cap) bcnt[id] = kBucketRows;
+        }
+        fill[p] = end & (kBucketRows - 1);
+        cur[p] = bucket_of((end - 1) >> kBucketShift, first, delta);
+      }
+      if (on) wdesc[p] = make_uint4(f - off, first, delta, 0u);
+    }
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
@@ -403,32 +406,140 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
         for (int c = 0; c < NC; ++c) {
           if (c >= ncol) break;
           if (colb[c] == 4) ((uint32_t *)(scol + coff[c]))[lp] = (uint32_t)cell[i][c];
-          else ((uint64_t *)(scol + coff[c]))[lp] = cell[i][c];
+          else ((uint64_t *)(scol + coff[c]))[lp] = (uint64_t)cell[i][c];
         }
         spid[lp] = (uint16_t)pp[i];
       }
     }
     __syncthreads();
-    // the next tile's reads go out before this tile's writes
-    const uint32_t next = tile + 1;
-    if (next < tend) TGA_SCATTER_LOAD(next)
-    const uint32_t rows = (uint32_t)min((uint64_t)TR, n - b);
-    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
-      const uint32_t p = spid[j];
-      const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
-      okeys[dst] = skey[j];
+    if constexpr (!kEarly) {
+      if (next < tend) TGA_SCATTER_LOAD(next, kk, cell)
+    }
+    for (uint32_t jr = threadIdx.x; jr < rows; jr += blockDim.x) {
+      const uint4 d = wdesc[spid[jr]];
+      const uint32_t q = jr + d.x;
+      const uint32_t id = bucket_of(q >> kBucketShift, d.y, d.z);
+      if (id >= cap) continue;
+      const uint64_t dst = ((uint64_t)id << kBucketShift) | (q & (kBucketRows - 1));
+      okeys[dst] = skey[jr];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         if (c >= ncol) break;
-        if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[j];
-        else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[j];
+        if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[jr];
+        else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[jr];
       }
     }
     __syncthreads();
     if (next >= tend) break;
     tile = next;
+    if constexpr (kEarly) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        kk[i] = nkk[i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cell[i][c] = ncell[i][c];
+      }
+    }
   }
 #undef TGA_SCATTER_LOAD
+  // the open buckets are the last of their chains
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x)
+    if (fill[p] && cur[p] < cap) bcnt[cur[p]] = fill[p];
+  // unused reserve ids
+  for (uint32_t id = rlo + lane; id < rhi; id += kWave)
+    if (id < cap) bpart[id] = ~0u;
+}
+
+// Per-partition bucket and row totals from the bucket headers (pbk, prow
+// zeroed by the caller).
+__global__ void __launch_bounds__(1024) k_bucket_count(const uint32_t *__restrict__ bpart,
+                                                       const uint32_t *__restrict__ bcnt, const uint32_t *pool,
+                                                       uint32_t cap, uint32_t P, uint32_t *pbk, uint64_t *prow) {
+  extern __shared__ __align__(16) char smem[];
+  uint64_t *lrow = (uint64_t *)smem;
+  uint32_t *lbk = (uint32_t *)(lrow + P);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    lrow[p] = 0;
+    lbk[p] = 0;
+  }
+  __syncthreads();
+  const uint32_t nb = min(*pool, cap);
+  const uint32_t per = (nb + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = blockIdx.x * per, hi = min(nb, lo + per);
+  for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
+    const uint32_t p = bpart[id];
+    if (p >= P) continue;
+    atomicAdd(&lbk[p], 1u);
+    atomicAdd((unsigned long long *)&lrow[p], (unsigned long long)bcnt[id]);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    if (!lbk[p]) continue;
+    atomicAdd(&pbk[p], lbk[p]);
+    atomicAdd((unsigned long long *)&prow[p], (unsigned long long)lrow[p]);
+  }
+}
+
+// (c) Work plan over the bucket lists: partition p owns list entries
+// [pstart[p], pstart[p+1]) and is cut into ceil(rows / chmax) work items;
+// bcur[p] = pstart[p] (the fill cursor of k_bucket_list).  One workgroup.
+__global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const uint64_t *prow, uint32_t P,
+                                                      uint64_t chmax, uint32_t *bcur, AggPlan plan) {
+  extern __shared__ uint32_t cnt[];  // [2P]
+  __shared__ uint32_t scratch[17];
+  uint32_t *nbk = cnt, *nit = cnt + P;
+  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    const uint64_t rows = prow[p];
+    nbk[p] = pbk[p];
+    nit[p] = (uint32_t)((rows + chmax - 1) / chmax);
+    if (nit[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)rows);
+  }
+  __syncthreads();
+  const uint32_t buckets = block_exclusive_scan_lds(nbk, P, scratch);
+  const uint32_t items = block_exclusive_scan_lds(nit, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    plan.pstart[p] = nbk[p];
+    bcur[p] = nbk[p];
+    plan.cstart[p] = nit[p];
+  }
+  if (threadIdx.x == 0) {
+    plan.pstart[P] = buckets;
+    plan.cstart[P] = items;
+    *plan.ticket = 0;
+  }
+}
+
+// Buckets grouped by partition: blist[pstart[p] ..] = the entries
+// rows << 32 | id of the buckets of p (any order).  Each workgroup counts its
+// range of ids per partition in LDS, reserves its share of every list with
+// one atomic, then places its entries.
+__global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict__ bpart,
+                                                      const uint32_t *__restrict__ bcnt, const uint32_t *pool,
+                                                      uint32_t cap, uint32_t P, uint32_t *bcur, uint64_t *blist) {
+  extern __shared__ uint32_t cnt[];  // [2P]
+  uint32_t *lc = cnt, *lb = cnt + P;
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) lc[p] = 0;
+  __syncthreads();
+  const uint32_t nb = min(*pool, cap);
+  const uint32_t per = (nb + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = blockIdx.x * per, hi = min(nb, lo + per);
+  for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
+    const uint32_t p = bpart[id];
+    if (p < P) atomicAdd(&lc[p], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    lb[p] = lc[p] ? atomicAdd(&bcur[p], lc[p]) : 0u;
+    lc[p] = 0;
+  }
+  __syncthreads();
+  for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
+    const uint32_t p = bpart[id];
+    if (p >= P) continue;
+    blist[lb[p] + atomicAdd(&lc[p], 1u)] = ((uint64_t)bcnt[id] << 32) | id;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -685,31 +796,6 @@ hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks,
   return hipGetLastError();
 }
 
-hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
-                              hipStream_t st) {
-  const size_t lds = (size_t)(P + 1) * 4;
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)&k_plan_chunks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, pbase, P, n, chmax, plan);
-  return hipGetLastError();
-}
-
-hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
-                             uint32_t ntiles, uint32_t *cnt, hipStream_t st) {
-  const size_t lds = (size_t)4 << pbits;
-  if (key_bytes == 4)
-    hipLaunchKernelGGL(k_part_count<uint32_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint32_t *)keys, n,
-                       pbits,
-                       tile_rows, ntiles, cnt);
-  else
-    hipLaunchKernelGGL(k_part_count<uint64_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint64_t *)keys, n,
-                       pbits,
-                       tile_rows, ntiles, cnt);
-  return hipGetLastError();
-}
-
 uint64_t scan_tiles(uint64_t n) { return (n + kScanThreads * kScanItems - 1) / (kScanThreads * kScanItems); }
 
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
@@ -721,55 +807,75 @@ hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
 }
 
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows) {
-  size_t b = (size_t)8 << pbits;  // hist + goff
+  size_t b = (size_t)28 << pbits;  // wdesc + hist + fill + cur
   size_t row = (size_t)key_bytes + 2;
   for (int c = 0; c < s.ncols; ++c) row += (size_t)dtype_bytes(s.col_dtype[c]);
   return b + row * tile_rows;
 }
 
-template <typename K, int NC>
-static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
-                                   uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase, void *okeys,
-                                   const PartOut &o, unsigned grid, hipStream_t st) {
+template <typename K, int NC, int VW>
+static hipError_t launch_bucket_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                  uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
+                                  hipStream_t st) {
   constexpr int ITEMS = 8;
   const unsigned block = tile_rows / ITEMS;
   const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
-  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_bucket<K, ITEMS, NC, VW>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n, pbits,
-                     ntiles, offs, pbase, (K *)okeys, o);
+  hipLaunchKernelGGL((k_part_bucket<K, ITEMS, NC, VW>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n,
+                     pbits, ntiles, (K *)okeys, o, bb.bpart, bb.bcnt, bb.pool, bb.cap);
   return hipGetLastError();
 }
 
-hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                               uint32_t tile_rows, uint32_t ntiles, const uint32_t *offs, const uint32_t *pbase,
-                               void *okeys, const PartOut &o, hipStream_t st, unsigned grid) {
-  if (grid == 0 || grid > ntiles) grid = ntiles;
-  grid = xcd_grid(grid);
+unsigned bucket_grid(uint32_t ntiles, unsigned max_grid) {
+  // at least 8 tiles per workgroup, so its buckets fill up
+  unsigned g = std::max<unsigned>(1, std::min<unsigned>(max_grid, ntiles / 8));
+  return xcd_grid(g);
+}
+
+uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows) {
+  const uint64_t block = tile_rows / 8;  // the scatter's ITEMS
+  return (n + kBucketRows - 1) / kBucketRows + (uint64_t)grid * ((1ULL << pbits) + block) + 1;
+}
+
+hipError_t launch_part_bucket(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                              uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
+                              hipStream_t st) {
+  const uint32_t ntiles = (uint32_t)((n + tile_rows - 1) / tile_rows);
+  if (ntiles == 0) return hipSuccess;
   const bool one = s.ncols <= 1;
-#define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, offs, pbase, okeys, o, grid, st
-  if (key_bytes == 4)
-    return one ? launch_scatter_t<uint32_t, 1>(TGA_SC_ARGS) : launch_scatter_t<uint32_t, kMaxCols>(TGA_SC_ARGS);
-  return one ? launch_scatter_t<uint64_t, 1>(TGA_SC_ARGS) : launch_scatter_t<uint64_t, kMaxCols>(TGA_SC_ARGS);
+  const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
+#define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, grid, st
+#define TGA_SC_CASE(K)                                                                              \
+  if (one) return w8 ? launch_bucket_t<K, 1, 8>(TGA_SC_ARGS) : launch_bucket_t<K, 1, 4>(TGA_SC_ARGS); \
+  return launch_bucket_t<K, kMaxCols, 0>(TGA_SC_ARGS);
+  if (key_bytes == 4) { TGA_SC_CASE(uint32_t) }
+  TGA_SC_CASE(uint64_t)
+#undef TGA_SC_CASE
 #undef TGA_SC_ARGS
 }
 
-uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P) {
-  return (uint64_t)((ntiles + kColTiles - 1) / kColTiles) * P;
+static hipError_t lds_attr(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
-                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st) {
-  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
-  hipLaunchKernelGGL(k_colscan, dim3(nchunks), dim3(256), 0, st, cnt, ntiles, P, desc, ticket, offs, total);
-  hipError_t e = hipGetLastError();
+hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
+                               hipStream_t st) {
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(256, bb.cap / 4096));
+  size_t lds = (size_t)P * 12;
+  hipError_t e = lds_attr((const void *)&k_bucket_count, lds);
   if (e != hipSuccess) return e;
-  const size_t lds = (size_t)P * 4;
-  if (lds > 65536) {
-    e = hipFuncSetAttribute((const void *)&k_colbase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_colbase, dim3(1), dim3(1024), lds, st, total, P, pbase);
+  hipLaunchKernelGGL(k_bucket_count, dim3(grid), dim3(1024), lds, st, bb.bpart, bb.bcnt, bb.pool, bb.cap, P, bb.pbk,
+                     bb.prow);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  lds = (size_t)P * 8;
+  if ((e = lds_attr((const void *)&k_bucket_plan, lds)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bucket_plan, dim3(1), dim3(1024), lds, st, bb.pbk, bb.prow, P, chmax, bb.bcur, plan);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = lds_attr((const void *)&k_bucket_list, lds)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bucket_list, dim3(grid), dim3(1024), lds, st, bb.bpart, bb.bcnt, bb.pool, bb.cap, P, bb.bcur,
+                     bb.blist);
   return hipGetLastError();
 }
 

@@ -36,6 +36,7 @@ struct dim3 {
   constexpr dim3(uint32_t x_ = 1, uint32_t y_ = 1, uint32_t z_ = 1) : x(x_), y(y_), z(z_) {}
 };
 struct __attribute__((aligned(16))) uint4 { uint32_t x, y, z, w; };
+inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return uint4{x, y, z, w}; }
 struct __attribute__((aligned(8))) uint2 { uint32_t x, y; };
 
 namespace emu {

@@ -236,9 +236,9 @@ int main(int argc, char **argv) {
     cases.push_back({"replan_skew_split", 64, k, {col_i32(N, -9, 9, 32)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
                      opts(TGA_PATH_PARTITIONED, 2, 0, 1024, 1000)});
   }
-  // enough tiles (8192 rows each) for several column-scan chunks (32 tiles
-  // each), so the per-column look-back runs across chunks
-  cases.push_back({"colscan_chunks_p1024", 64, keys_u64(3 * N, 200000, 33), {col_i32(3 * N, -99, 99, 34)}, {TGA_I32},
+  // 2^10 partitions (more than one per scatter thread) over enough tiles
+  // that the waves refill their bucket reserves several times
+  cases.push_back({"buckets_p1024", 64, keys_u64(3 * N, 200000, 33), {col_i32(3 * N, -99, 99, 34)}, {TGA_I32},
                    {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 10, 0, 200000)});
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
