@@ -66,7 +66,8 @@ class Options(ctypes.Structure):
     _fields_ = [("est_groups", ctypes.c_int64), ("path", ctypes.c_int32), ("partition_bits", ctypes.c_int32),
                 ("table_slots", ctypes.c_int32), ("tile_rows", ctypes.c_int32), ("workgroups", ctypes.c_int32),
                 ("block_threads", ctypes.c_int32), ("chunk_rows", ctypes.c_int64),
-                ("wave_fold", ctypes.c_int32), ("attempts", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("wave_fold", ctypes.c_int32), ("attempts", ctypes.c_int32), ("scatter_layout", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
 
 
 class Query(ctypes.Structure):

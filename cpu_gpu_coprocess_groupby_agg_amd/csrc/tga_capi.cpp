@@ -316,6 +316,11 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   // key (2^26 rows, one key: 0.52 ms without, 6.9 ms with; Zipf(1.1) keys:
   // 2x slower with).
   p.fold = o.wave_fold == 1 ? 1 : 0;
+  // Per-workgroup buckets by default: no histogram pass over the keys.  The
+  // partition-contiguous copy (histogram + column scan first) makes the
+  // aggregation ~12% faster (no partially filled buckets) but costs more in
+  // the extra pass (2^28 rows, 2^20 groups on MI355X: 4.24 vs 4.68 ms).
+  p.scatter_layout = o.scatter_layout == 1 ? 1 : 2;
   p.opt = o;
   p.top_share = top_share;
   p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < 12;
@@ -337,6 +342,7 @@ static void plan_to_options(const Plan &p, tga_options &o) {
   o.wave_fold = p.fold ? 1 : 2;
   o.chunk_rows = (int64_t)p.chunk_rows;
   o.est_groups = (int64_t)(p.gslots / 2);
+  o.scatter_layout = p.scatter_layout;
 }
 
 // ---------------------------------------------------------------------------
@@ -430,58 +436,96 @@ static AggPlan part_plan(const DBuf &buf, uint32_t P) {
 // Buffers of one partitioning pass; they must outlive the work queued on
 // the stream.
 struct PartBufs {
-  DBuf heads, meta, pkeys, planbuf;
+  DBuf heads, meta, pkeys, planbuf;  // bucket layout: headers/lists, counters
+  DBuf cnt, offs, total, pbase, desc;  // partition-contiguous layout
   DBuf pcols[kMaxCols];
   BucketBufs bb{};
+  bool buckets = false;
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
   void keep(std::vector<DBuf> &k) {
-    for (DBuf *b : {&heads, &meta, &pkeys, &planbuf}) k.push_back(std::move(*b));
+    for (DBuf *b : {&heads, &meta, &pkeys, &planbuf, &cnt, &offs, &total, &pbase, &desc})
+      if (b->p) k.push_back(std::move(*b));
     for (auto &c : pcols)
       if (c.p) k.push_back(std::move(c));
   }
 };
 
-// (b) + (c) for rows [0, m) of keys / s.cols: count-free scatter into
-// buckets, then the per-partition bucket lists and the work-item plan.
+// (b) + (c) for rows [0, m) of keys / s.cols: the partitioned copy and the
+// work-item plan of k_agg.  Partition-contiguous layout: per-tile histogram,
+// column scan, scatter to the scanned offsets.  Bucket layout: count-free
+// scatter into buckets, then the per-partition bucket lists.
 static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s, uint64_t m,
                                  const Plan &plan, PartBufs &pb) {
   hipStream_t st = ctx->stream;
   const uint32_t P = 1u << plan.pbits;
   const unsigned grid = scatter_grid(ctx, q, plan, m);
-  const uint64_t cap = bucket_cap(m, grid, plan.pbits, plan.tile_rows);
-  if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
-  const uint64_t slots = cap << kBucketShift;
-  // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
-  const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
-  if (!pb.heads.alloc(ctx, cap * 16) || !pb.meta.alloc(ctx, meta_words * 8) ||
-      !pb.pkeys.alloc(ctx, slots * q.key_bytes) || !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
+  pb.buckets = plan.scatter_layout == 2;
+  uint64_t slots = m;
+  if (pb.buckets) {
+    const uint64_t cap = bucket_cap(m, grid, plan.pbits, plan.tile_rows);
+    if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
+    slots = cap << kBucketShift;
+    // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
+    const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
+    if (!pb.heads.alloc(ctx, cap * 16) || !pb.meta.alloc(ctx, meta_words * 8))
+      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+    BucketBufs &bb = pb.bb;
+    bb.blist = pb.heads.as<uint64_t>();
+    bb.bpart = (uint32_t *)(bb.blist + cap);
+    bb.bcnt = bb.bpart + cap;
+    uint64_t *meta = pb.meta.as<uint64_t>();
+    bb.pool = (uint32_t *)meta;
+    bb.pbk = (uint32_t *)(meta + 1);
+    bb.bcur = bb.pbk + P;
+    bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
+    bb.cap = (uint32_t)cap;
+    TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, st));
+  }
+  if (!pb.pkeys.alloc(ctx, std::max<uint64_t>(slots, 1) * q.key_bytes) ||
+      !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   pb.sp = s;
   for (int c = 0; c < s.ncols; ++c) {
-    if (!pb.pcols[c].alloc(ctx, slots * (uint64_t)dtype_bytes(s.col_dtype[c])))
+    if (!pb.pcols[c].alloc(ctx, std::max<uint64_t>(slots, 1) * (uint64_t)dtype_bytes(s.col_dtype[c])))
       return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
     po.cols[c] = pb.pcols[c].p;
     pb.sp.cols[c] = pb.pcols[c].p;
   }
-  BucketBufs &bb = pb.bb;
-  bb.blist = pb.heads.as<uint64_t>();
-  bb.bpart = (uint32_t *)(bb.blist + cap);
-  bb.bcnt = bb.bpart + cap;
-  uint64_t *meta = pb.meta.as<uint64_t>();
-  bb.pool = (uint32_t *)meta;
-  bb.pbk = (uint32_t *)(meta + 1);
-  bb.bcur = bb.pbk + P;
-  bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
-  bb.cap = (uint32_t)cap;
-  TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, st));
-  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
-    return launch_part_bucket(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, pb.pkeys.p, po, bb, grid, st);
-  }));
   pb.ap = part_plan(pb.planbuf, P);
-  pb.ap.blist = bb.blist;
-  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st); }));
+  if (pb.buckets) {
+    BucketBufs &bb = pb.bb;
+    TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+      return launch_part_scatter(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, pb.pkeys.p, po, &bb, nullptr,
+                                 nullptr, grid, st);
+    }));
+    pb.ap.blist = bb.blist;
+    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st); }));
+    return TGA_OK;
+  }
+  const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
+  const uint64_t cells = (uint64_t)P * ntiles;
+  const uint64_t dwords = colscan_desc_words(ntiles, P);
+  if (!pb.cnt.alloc(ctx, cells * 4) || !pb.offs.alloc(ctx, cells * 4) || !pb.total.alloc(ctx, (uint64_t)P * 4) ||
+      !pb.pbase.alloc(ctx, (uint64_t)(P + 1) * 4) || !pb.desc.alloc(ctx, (dwords + 1) * 8))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  TGA_HIP(ctx, launch_fill_u64(pb.desc.as<uint64_t>(), dwords + 1, 0, st));
+  TGA_HIP(ctx, timed(ctx, "part_count", [&] {
+    return launch_part_count(q.key_bytes, keys, m, plan.pbits, plan.tile_rows, ntiles, pb.cnt.as<uint32_t>(), st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "scan", [&] {
+    return launch_colscan(pb.cnt.as<uint32_t>(), ntiles, P, pb.desc.as<uint64_t>(),
+                          (uint32_t *)(pb.desc.as<uint64_t>() + dwords), pb.offs.as<uint32_t>(),
+                          pb.total.as<uint32_t>(), pb.pbase.as<uint32_t>(), st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+    return launch_part_scatter(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, pb.pkeys.p, po, nullptr,
+                               pb.offs.as<uint32_t>(), pb.pbase.as<uint32_t>(), grid, st);
+  }));
+  TGA_HIP(ctx, timed(ctx, "plan_chunks", [&] {
+    return launch_plan_chunks(pb.pbase.as<uint32_t>(), P, m, plan.chunk_rows, pb.ap, st);
+  }));
   return TGA_OK;
 }
 
@@ -584,11 +628,12 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   uint64_t split_rows = 0;
   uint32_t used = 0;
   TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
-  TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
+  if (pb.buckets) TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
   // the scatter never takes more buckets than bucket_cap(); a larger count
   // would mean dropped rows
-  if (used > pb.bb.cap) return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
+  if (pb.buckets && used > pb.bb.cap)
+    return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
   // global table: split partitions and spills only
   uint64_t S = plan.gslots;
   if (!full_table) {

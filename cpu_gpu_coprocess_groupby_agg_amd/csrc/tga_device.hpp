@@ -348,9 +348,50 @@ __device__ inline uint64_t lookback_exclusive(uint64_t *desc, uint64_t tile, uin
 // block-wide helpers and the row -> LDS table -> global table pipeline
 // ---------------------------------------------------------------------------
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is also a
+// workgroup-scope release fence, for which the compiler first waits for every
+// outstanding global load and store of the wave (vmcnt(0)); a kernel that
+// keeps loads in flight across its barriers (the next tile's rows) would wait
+// for them at each one.  Here a wave waits for its own LDS accesses only
+// (lgkmcnt(0) = s_waitcnt 0xc07f on gfx9-family targets), then meets the
+// others at s_barrier; the signal fences keep the compiler from moving memory
+// accesses across it.  Only for barriers that order LDS data.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Marks a register value as defined here: an empty asm statement that reads
+// and writes it.  After an explicit s_waitcnt has completed the loads of a
+// register set, pinning the set tells the compiler no load is outstanding on
+// it any more, so it inserts no further waits for those registers (its own
+// tracking loses that across loop back edges and then waits there, behind
+// whatever was issued since).  A no-op outside the device compilation.
+template <typename T>
+__device__ __forceinline__ void pin_reg(T &x) {
+#if defined(__AMDGCN__)
+  asm volatile("" : "+v"(x));
+#else
+  (void)x;
+#endif
+}
+
+struct SyncBarrier {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+struct LdsBarrier {
+  __device__ __forceinline__ void operator()() const { lds_barrier(); }
+};
+
 // Exclusive scan of n u32 values held in LDS, in place; returns the total.
-// scratch needs blockDim/64 + 1 words.  All threads must call it.
+// scratch needs blockDim/64 + 1 words.  All threads must call it.  Bar: the
+// barrier used between the phases (LdsBarrier when global accesses must stay
+// in flight).
+template <typename Bar = SyncBarrier>
 __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
+  const Bar bar{};
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t per = (n + nt - 1) / nt;
   const uint32_t b = tid * per, e = min(n, b + per);
@@ -364,18 +405,18 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint
     if (lane >= o) x += y;
   }
   if (lane == kWave - 1) scratch[wave] = x;
-  __syncthreads();
+  bar();
   if (tid == 0) {
     uint32_t run = 0;
     const uint32_t nw = nt >> 6;
     for (uint32_t w = 0; w < nw; ++w) { const uint32_t v = scratch[w]; scratch[w] = run; run += v; }
     scratch[nw] = run;
   }
-  __syncthreads();
+  bar();
   uint32_t run = scratch[wave] + x - local;
   for (uint32_t i = b; i < e; ++i) { const uint32_t v = arr[i]; arr[i] = run; run += v; }
   const uint32_t total = scratch[nt >> 6];
-  __syncthreads();
+  bar();
   return total;
 }
 
@@ -619,7 +660,41 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
   }
 }
 
-template <typename K, int NC, int VW, bool FOLD>
+// Accumulator program of the row loop.  SIG == 0: decoded from the AccProg
+// at run time (any schema).  SIG != 0: fixed at compile time, so the per-row
+// dispatch over accumulator kinds folds away -- accumulator a has kind
+// ((SIG >> 4a) & 15) - 1, on value column 0 with the raw operand, for every
+// nonzero nibble (the common one-column integer schemas).
+constexpr uint32_t kSigCountSum = (ACC_COUNT + 1) | (ACC_SUM_I + 1) << 4;
+constexpr uint32_t kSigCountSumMinMax = kSigCountSum | (ACC_MIN_I32 + 1) << 8 | (ACC_MAX_I32 + 1) << 12;
+
+template <uint32_t SIG>
+__device__ __forceinline__ uint32_t prog_nacc(const AccProg &ap) {
+  if constexpr (SIG != 0) {
+    uint32_t n = 0;
+    while (n < 8 && ((SIG >> (4 * n)) & 15u)) ++n;
+    return n;
+  } else {
+    return ap.nacc;
+  }
+}
+template <uint32_t SIG>
+__device__ __forceinline__ uint32_t prog_kind(const AccProg &ap, int a) {
+  if constexpr (SIG != 0) return ((SIG >> (4 * a)) & 15u) - 1u;
+  else return (ap.kinds >> (4 * a)) & 15u;
+}
+template <uint32_t SIG>
+__device__ __forceinline__ uint32_t prog_col(const AccProg &ap, int a) {
+  if constexpr (SIG != 0) return 0u;
+  else return (ap.cols >> (2 * a)) & 3u;
+}
+template <uint32_t SIG>
+__device__ __forceinline__ uint32_t prog_map(const AccProg &ap, int a) {
+  if constexpr (SIG != 0) return MAP_RAW;
+  else return (ap.maps >> (2 * a)) & 3u;
+}
+
+template <typename K, int NC, int VW, bool FOLD, uint32_t SIG = 0>
 __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b0,
                                              uint64_t e0, const LdsTable<K> &t, const AccProg &ap, int nbits,
                                              int max_probe, const GTable *gsp, const Schema *ssp,
@@ -711,17 +786,17 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
 #pragma unroll
         for (int a = 0; a < kMaxAccs; ++a) {
           ops[a] = 0;
-          if ((uint32_t)a < ap.nacc)
-            ops[a] = prog_operand((ap.kinds >> (4 * a)) & 15u, (ap.maps >> (2 * a)) & 3u,
-                                  pick_col<NC>(raw[u], (ap.cols >> (2 * a)) & 3u));
+          if ((uint32_t)a < prog_nacc<SIG>(ap))
+            ops[a] = prog_operand(prog_kind<SIG>(ap, a), prog_map<SIG>(ap, a),
+                                  pick_col<NC>(raw[u], prog_col<SIG>(ap, a)));
         }
         while (__ballot(rest != 0)) {
           const int src = rest ? (int)__builtin_ctzll(rest) : lane;
 #pragma unroll
           for (int a = 0; a < kMaxAccs; ++a) {
-            if ((uint32_t)a >= ap.nacc) break;
+            if ((uint32_t)a >= prog_nacc<SIG>(ap)) break;
             const uint64_t o = shfl64(ops[a], src);
-            if (rest) ops[a] = acc_fold((int)((ap.kinds >> (4 * a)) & 15u), ops[a], o);
+            if (rest) ops[a] = acc_fold((int)prog_kind<SIG>(ap, a), ops[a], o);
           }
           if (rest) rest &= rest - 1;
         }
@@ -731,8 +806,8 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
           const int sl = leader ? slot[u] : (int)S + 1;
 #pragma unroll
           for (int a = 0; a < kMaxAccs; ++a) {
-            if ((uint32_t)a >= ap.nacc) break;
-            const uint32_t kind = (ap.kinds >> (4 * a)) & 15u;
+            if ((uint32_t)a >= prog_nacc<SIG>(ap)) break;
+            const uint32_t kind = prog_kind<SIG>(ap, a);
             char *arr = tb + ap.off[a];
             switch (kind) {
               case ACC_COUNT: atomicAdd((uint32_t *)arr + sl, (uint32_t)ops[a]); break;
@@ -753,10 +828,10 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
       for (int u = 0; u < U; ++u) sl[u] = slot[u] >= 0 ? slot[u] : (int)S + 1;
 #pragma unroll
       for (int a = 0; a < kMaxAccs; ++a) {
-        if ((uint32_t)a >= ap.nacc) break;
-        const uint32_t kind = (ap.kinds >> (4 * a)) & 15u;
-        const uint32_t col = (ap.cols >> (2 * a)) & 3u;
-        const uint32_t map = (ap.maps >> (2 * a)) & 3u;
+        if ((uint32_t)a >= prog_nacc<SIG>(ap)) break;
+        const uint32_t kind = prog_kind<SIG>(ap, a);
+        const uint32_t col = prog_col<SIG>(ap, a);
+        const uint32_t map = prog_map<SIG>(ap, a);
         char *arr = tb + ap.off[a];
         switch (kind) {
           case ACC_COUNT:

@@ -146,6 +146,7 @@ struct Plan {
   int nbits = 0;
   int max_probe = 0;
   int fold = 0;  // fold equal keys of a wave before the LDS atomics
+  int scatter_layout = 1;  // 1 = partition-contiguous, 2 = per-workgroup buckets
   // re-planning: the options and skew the plan came from, and whether the
   // fanout was the tuner's (so a too-small estimate may raise it)
   tga_options opt{};

@@ -57,9 +57,22 @@ unsigned bucket_grid(uint32_t ntiles, unsigned max_grid);
 // buckets the scatter can take: ceil(n / kBucketRows) + grid * (2^pbits +
 // its workgroup size)
 uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows);
-// rows [0, n) into buckets of the partitioned copy okeys / o.cols
-hipError_t launch_part_bucket(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                              uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
+// rows [0, n) into the partitioned copy okeys / o.cols: into buckets with
+// bb, else partition-contiguous at pbase[p] + offs[tile][p] (from
+// launch_part_count + launch_colscan over the same tiles)
+hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                               uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs *bb,
+                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st);
+// Partition-contiguous layout: per-tile histogram cnt[ntiles][P]; its
+// column-wise scan offs (same shape, exclusive per partition), total[P] and
+// pbase[P+1] (partition starts); desc: colscan_desc_words() zeroed words
+// followed by a zeroed ticket word; work plan over pbase.
+hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
+                             uint32_t ntiles, uint32_t *cnt, hipStream_t st);
+uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P);
+hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
+                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st);
+hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
                               hipStream_t st);
 // bucket headers -> per-partition bucket lists and the k_agg work plan
 // (items of at most about chmax rows); plan.blist/bcnt must point at bb's

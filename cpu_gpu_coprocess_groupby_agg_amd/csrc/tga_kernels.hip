@@ -76,7 +76,7 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 // Persistent: workgroups pull work items from plan.ticket until none remain.
 // Item -> (segment p, slice j of nch) through plan.cstart; rows of segment p
 // are [pstart[p], pstart[p+1]).
-template <typename K, bool FX, int NC, int VW, bool FOLD>
+template <typename K, bool FX, int NC, int VW, bool FOLD, uint32_t SIG>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
                                               GTable g, AccProg ap, int nbits, int max_probe, int emit, OutDesc od,
                                               uint64_t *emit_count, uint32_t *ovf) {
@@ -126,8 +126,8 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
       const void *cols[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
-      agg_rows_int<K, NC, VW, FOLD>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
-                                    abandon ? s_abort : nullptr, nullptr, bl);
+      agg_rows_int<K, NC, VW, FOLD, SIG>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
+                                         abandon ? s_abort : nullptr, nullptr, bl);
     }
     __syncthreads();
     if (abandon && s_abort[0]) {
@@ -160,6 +160,29 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
   }
 }
 
+// (c) Partition-contiguous layout: partition p has rows [pbase[p], pbase[p+1]); it becomes
+// ceil(rows/chmax) work items.
+__global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax,
+                                                      AggPlan plan) {
+  __shared__ uint32_t scratch[17];
+  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
+  __syncthreads();
+  extern __shared__ uint32_t cnt[];  // P + 1
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    const uint64_t b = pbase[p];
+    const uint64_t e = p + 1 < P ? pbase[p + 1] : n;
+    plan.pstart[p] = b;
+    const uint64_t len = e - b;
+    cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
+    if (cnt[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
+  }
+  if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
+  __syncthreads();
+  const uint32_t total = block_exclusive_scan_lds(cnt, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) plan.cstart[p] = cnt[p];
+  if (threadIdx.x == 0) plan.cstart[P] = total;
+}
+
 // ---------------------------------------------------------------------------
 // (b) partitioning
 // ---------------------------------------------------------------------------
@@ -176,6 +199,83 @@ __host__ __device__ inline uint32_t xcd_grid(uint32_t count) { return (count + k
 __device__ __forceinline__ uint32_t xcd_local_index(uint32_t w, uint32_t count) {
   const uint32_t per = (count + kXcds - 1) / kXcds;
   return (w % kXcds) * per + w / kXcds;
+}
+
+// Per-tile partition histogram, tile-major: cnt[tile][p] (one contiguous
+// row of P counters per tile).
+template <typename K>
+__global__ void __launch_bounds__(1024) k_part_count(const K *__restrict__ keys, uint64_t n, int pbits,
+                                                     uint32_t tile_rows, uint32_t ntiles, uint32_t *cnt) {
+  extern __shared__ uint32_t hist[];
+  const uint32_t P = 1u << pbits;
+  const uint32_t tile = xcd_local_index(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;  // uniform per workgroup
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+  __syncthreads();
+  const uint64_t b = (uint64_t)tile * tile_rows;
+  const uint64_t e = min(n, b + tile_rows);
+#pragma unroll 8
+  for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
+    atomicAdd(&hist[key_partition((uint64_t)keys[r], pbits)], 1u);
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[(uint64_t)tile * P + p] = hist[p];
+}
+
+// Column-wise exclusive scan of the tile-major histogram: for every
+// partition p, offs[t][p] = sum of cnt[t'][p] over tiles t' < t, and
+// total[p] = the partition's row count.  Workgroups take chunks of kColTiles
+// tiles in ticket order; every column does its own decoupled look-back
+// (Merrill & Garland 2016) over the chunks before it, one lane per column,
+// so the whole scan reads and writes rows of P contiguous counters (a
+// partition-major layout would make every tile touch P scattered lines).
+constexpr uint32_t kColTiles = 32;
+__global__ void __launch_bounds__(256) k_colscan(const uint32_t *__restrict__ cnt, uint32_t ntiles, uint32_t P,
+                                                 uint64_t *desc, uint32_t *ticket, uint32_t *offs, uint32_t *total) {
+  __shared__ uint32_t s_chunk;
+  if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t c = wave_uniform(s_chunk);
+  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
+  if (c >= nchunks) return;
+  const uint32_t t0 = c * kColTiles, t1 = min(ntiles, t0 + kColTiles);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+    uint64_t sum = 0;
+    for (uint32_t t = t0; t < t1; ++t) sum += cnt[(uint64_t)t * P + p];
+    uint64_t excl = 0;
+    if (c == 0) {
+      lb_store(&desc[p], kLbInc | sum);
+    } else {
+      lb_store(&desc[(uint64_t)c * P + p], kLbAgg | sum);
+      for (int64_t k = (int64_t)c - 1; k >= 0; --k) {  // every chunk k < c holds an earlier ticket
+        uint64_t d = lb_load(&desc[(uint64_t)k * P + p]);
+        while ((d >> 62) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          d = lb_load(&desc[(uint64_t)k * P + p]);
+        }
+        excl += d & kLbVal;
+        if ((d >> 62) == 2) break;  // an inclusive prefix ends the walk
+      }
+      lb_store(&desc[(uint64_t)c * P + p], kLbInc | (excl + sum));
+    }
+    uint64_t run = excl;
+    for (uint32_t t = t0; t < t1; ++t) {
+      offs[(uint64_t)t * P + p] = (uint32_t)run;
+      run += cnt[(uint64_t)t * P + p];
+    }
+    if (c == nchunks - 1) total[p] = (uint32_t)run;
+  }
+}
+
+// pbase[p] = rows of the partitions before p (pbase[P] = all rows): the
+// start of partition p in the partitioned copy.  One workgroup.
+__global__ void __launch_bounds__(1024) k_colbase(const uint32_t *total, uint32_t P, uint32_t *pbase) {
+  extern __shared__ uint32_t cnt[];  // P
+  __shared__ uint32_t scratch[17];
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) cnt[p] = total[p];
+  __syncthreads();
+  const uint32_t all = block_exclusive_scan_lds(cnt, P, scratch);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) pbase[p] = cnt[p];
+  if (threadIdx.x == 0) pbase[P] = all;
 }
 
 // Exclusive scan of n u32 (total < 2^32), 4096 values per tile, tiles in
@@ -255,10 +355,18 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_u32(const uint32_t *__res
 // and a reserve of < kReserve per wave); the host sizes `cap` to that bound,
 // and no id >= cap is ever written.
 //
-// With one value column the next tile's rows are loaded into a second
-// register set right after this tile is ranked, so they have this tile's
-// remaining phases to arrive; with more columns (whose second register set
-// would spill) they are loaded after this tile is staged, during its writes.
+// Per tile the workgroup goes through two barrier intervals:
+//   [scan the tile's counts, extend the bucket chains, stage the rows]
+//   [write the staged rows out, then rank the next tile's rows]
+// so the stores of one tile overlap the hashing and LDS atomics of the next
+// (the counts go to the other of two histograms).  The barriers order LDS
+// only (lds_barrier), so loads stay in flight across them.  With one value
+// column the rows of the tile after the next are loaded as soon as the next
+// one is ranked, into the register set the current tile was staged from:
+// two register sets alternate, in a loop unrolled twice, so no register
+// move (and no wait for a load in flight) is ever needed between them.
+// With more columns (whose second register set would spill) the next tile is
+// loaded after the staging, during the writes.
 // VW: width of every value cell when NC == 1 (4 or 8), 0 = per column.
 constexpr uint32_t kReserve = 64;
 
@@ -266,188 +374,305 @@ __host__ __device__ inline uint32_t bucket_of(uint32_t k, uint32_t first, uint32
   return k == 0 ? first : delta + k;
 }
 
-template <typename K, int ITEMS, int NC, int VW>
-__global__ void __launch_bounds__(1024) k_part_bucket(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
-                                                      uint32_t ntiles, K *okeys, PartOut o, uint32_t *bpart,
-                                                      uint32_t *bcnt, uint32_t *pool, uint32_t cap) {
+// LDS and register state of one k_part_bucket workgroup.
+template <typename K, int NC>
+struct ScatterState {
+  const K *keys;
+  const Schema *s;
+  uint64_t n;
+  uint32_t P, TR, cap;
+  int pbits, ncol;
+  uint4 *wdesc;     // {q0, first, delta, -}: row j of p's run -> position j + q0 of its chain
+  uint32_t *hist;   // the tile's counts, then exclusive offsets
+  uint32_t *hnext;  // the next tile's counts
+  uint32_t *fill;   // rows in the open bucket (0: none open)
+  uint32_t *cur;    // the open bucket
+  K *skey;
+  char *scol;
+  uint16_t *spid;
+  uint32_t colb[NC];
+  size_t coff[NC];
+  uint32_t *scratch;
+  uint32_t *bpart, *bcnt, *pool;  // bucket layout
+  const uint32_t *offs, *pbase;     // partition-contiguous layout
+  K *okeys;
+  PartOut o;
+  uint32_t rlo, rhi;  // this wave's reserve of bucket ids (wave-uniform)
+};
+
+// Rows of tile t into registers (rows past the end re-read the last row).
+template <typename K, int ITEMS, int NC, typename CellT>
+__device__ __forceinline__ void scatter_load(const ScatterState<K, NC> &S, uint32_t t, K (&kk)[ITEMS],
+                                             CellT (&cell)[ITEMS][NC]) {
+  const uint64_t b0 = (uint64_t)t * S.TR;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;
+    const uint64_t r = row < S.n ? row : S.n - 1;
+    kk[i] = S.keys[r];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= S.ncol) { cell[i][c] = 0; continue; }
+      cell[i][c] = S.colb[c] == 4 ? (CellT)((const uint32_t *)S.s->cols[c])[r]
+                                  : (CellT)((const uint64_t *)S.s->cols[c])[r];
+    }
+  }
+}
+
+// The registers of one tile, pinned (see pin_reg) once their loads are done.
+template <typename K, int ITEMS, int NC, typename CellT>
+__device__ __forceinline__ void scatter_pin(K (&kk)[ITEMS], CellT (&cell)[ITEMS][NC]) {
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    pin_reg(kk[i]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) pin_reg(cell[i][c]);
+  }
+}
+
+// Rank of each row of tile t inside its partition, counted into h.
+template <typename K, int ITEMS, int NC>
+__device__ __forceinline__ void scatter_rank(const ScatterState<K, NC> &S, uint32_t t, const K (&kk)[ITEMS],
+                                             uint32_t *h, uint32_t (&pp)[ITEMS], uint32_t (&rk)[ITEMS]) {
+  const uint64_t b0 = (uint64_t)t * S.TR;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;
+    pp[i] = key_partition((uint64_t)kk[i], S.pbits);
+    rk[i] = row < S.n ? atomicAdd(&h[pp[i]], 1u) : 0u;
+  }
+}
+
+// Write descriptors of tile t's runs (rows in all), and the next histogram
+// zeroed.  Partition-contiguous layout (!BK): partition p's run of tile t
+// starts at pbase[p] + offs[t][p].  Bucket layout (BK): every partition's
+// bucket chain is extended by its run; runs needing one fresh bucket take it
+// from the wave's reserve, runs needing several take them, contiguous, with
+// their own atomic.
+template <bool BK, typename K, int NC>
+__device__ __forceinline__ void scatter_extend(ScatterState<K, NC> &S, uint32_t rows, uint32_t t) {
+  if constexpr (!BK) {
+    for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {
+      S.wdesc[p] = make_uint4(S.pbase[p] + S.offs[(uint64_t)t * S.P + p] - S.hist[p], 0u, 0u, 0u);
+      S.hnext[p] = 0;
+    }
+    return;
+  }
+  const int lane = (int)__lane_id();
+  for (uint32_t p0 = 0; p0 < S.P; p0 += blockDim.x) {
+    const uint32_t p = p0 + threadIdx.x;
+    const bool on = p < S.P;
+    uint32_t off = 0, c = 0, f = 0, first = 0, fresh = 0;
+    if (on) {
+      off = S.hist[p];
+      c = (p + 1 < S.P ? S.hist[p + 1] : rows) - off;
+      f = S.fill[p];
+      first = S.cur[p];
+      if (c) fresh = ((f + c + kBucketRows - 1) >> kBucketShift) - (f ? 1u : 0u);
+    }
+    const bool one = fresh == 1;
+    const uint64_t m1 = __ballot(one);
+    const uint32_t n1 = (uint32_t)__builtin_popcountll(m1);
+    const uint32_t j = (uint32_t)__builtin_popcountll(m1 & lanemask_lt());
+    uint32_t base = 0;
+    if (n1) {
+      const uint32_t have = S.rhi - S.rlo;
+      uint32_t got = 0;
+      if (have < n1) {  // refill (wave-uniform branch)
+        if (lane == 0) got = atomicAdd(S.pool, kReserve);
+        got = wave_uniform((uint32_t)__shfl((int)got, 0, kWave));
+      }
+      if (one) base = j < have ? S.rlo + j : got + (j - have);
+      if (have < n1) {
+        S.rlo = got + (n1 - have);
+        S.rhi = got + kReserve;
+      } else {
+        S.rlo += n1;
+      }
+    }
+    if (fresh > 1) base = atomicAdd(S.pool, fresh);
+    uint32_t delta = 0;
+    if (on && c) {
+      for (uint32_t k = 0; k < fresh; ++k)
+        if (base + k < S.cap) S.bpart[base + k] = p;
+      delta = f ? base - 1 : base;
+      if (!f) first = base;
+      const uint32_t end = f + c;
+      const uint32_t full = end >> kBucketShift;  // buckets this run completes
+      for (uint32_t k = 0; k < full; ++k) {
+        const uint32_t id = bucket_of(k, first, delta);
+        if (id < S.cap) S.bcnt[id] = kBucketRows;
+      }
+      S.fill[p] = end & (kBucketRows - 1);
+      S.cur[p] = bucket_of((end - 1) >> kBucketShift, first, delta);
+    }
+    if (on) {
+      S.wdesc[p] = make_uint4(f - off, first, delta, 0u);
+      S.hnext[p] = 0;
+    }
+  }
+}
+
+// Counting sort of tile t's rows into the LDS stage.
+template <typename K, int ITEMS, int NC, typename CellT>
+__device__ __forceinline__ void scatter_stage(const ScatterState<K, NC> &S, uint32_t t, const K (&kk)[ITEMS],
+                                              const CellT (&cell)[ITEMS][NC], const uint32_t (&pp)[ITEMS],
+                                              const uint32_t (&rk)[ITEMS]) {
+  const uint64_t b0 = (uint64_t)t * S.TR;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;
+    if (row < S.n) {
+      const uint32_t lp = S.hist[pp[i]] + rk[i];
+      S.skey[lp] = kk[i];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (c >= S.ncol) break;
+        if (S.colb[c] == 4) ((uint32_t *)(S.scol + S.coff[c]))[lp] = (uint32_t)cell[i][c];
+        else ((uint64_t *)(S.scol + S.coff[c]))[lp] = (uint64_t)cell[i][c];
+      }
+      S.spid[lp] = (uint16_t)pp[i];
+    }
+  }
+}
+
+// The staged rows to their destinations: consecutive rows of one run land in
+// consecutive rows of the partition (of a bucket, with BK).
+template <bool BK, typename K, int NC>
+__device__ __forceinline__ void scatter_write(const ScatterState<K, NC> &S, uint32_t rows) {
+  for (uint32_t jr = threadIdx.x; jr < rows; jr += blockDim.x) {
+    const uint4 d = S.wdesc[S.spid[jr]];
+    const uint32_t q = jr + d.x;
+    uint64_t dst = q;
+    if constexpr (BK) {
+      const uint32_t id = bucket_of(q >> kBucketShift, d.y, d.z);
+      if (id >= S.cap) continue;
+      dst = ((uint64_t)id << kBucketShift) | (q & (kBucketRows - 1));
+    }
+    S.okeys[dst] = S.skey[jr];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= S.ncol) break;
+      if (S.colb[c] == 4) ((uint32_t *)S.o.cols[c])[dst] = ((const uint32_t *)(S.scol + S.coff[c]))[jr];
+      else ((uint64_t *)S.o.cols[c])[dst] = ((const uint64_t *)(S.scol + S.coff[c]))[jr];
+    }
+  }
+}
+
+template <typename K, int ITEMS, int NC, int VW, bool BK>
+__global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
+                                                       uint32_t ntiles, K *okeys, PartOut o, uint32_t *bpart,
+                                                       uint32_t *bcnt, uint32_t *pool, uint32_t cap,
+                                                       const uint32_t *offs, const uint32_t *pbase) {
   using CellT = typename std::conditional<VW == 4, uint32_t, uint64_t>::type;
   constexpr bool kEarly = NC == 1;
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t scratch[17];
-  const uint32_t P = 1u << pbits;
-  const uint32_t TR = blockDim.x * ITEMS;
-  const int lane = (int)__lane_id();
-  // wdesc[p] = {q0, first, delta, -}: row j of the tile's run of p goes to
-  // position q = j + q0 of the bucket chain bucket_of(q >> shift, first, delta)
-  uint4 *wdesc = (uint4 *)smem;
-  uint32_t *hist = (uint32_t *)(wdesc + P);  // tile counts, then exclusive offsets
-  uint32_t *fill = hist + P;                 // rows in the open bucket (0: none open)
-  uint32_t *cur = fill + P;                  // the open bucket
-  char *stage = (char *)(cur + P);
+  ScatterState<K, NC> S;
+  S.keys = keys;
+  S.s = &s;
+  S.n = n;
+  S.P = 1u << pbits;
+  S.TR = blockDim.x * ITEMS;
+  S.cap = cap;
+  S.pbits = pbits;
+  S.ncol = NC == 1 ? 1 : s.ncols;
+  S.wdesc = (uint4 *)smem;
+  S.hist = (uint32_t *)(S.wdesc + S.P);
+  S.hnext = S.hist + S.P;
+  S.fill = S.hnext + S.P;
+  S.cur = S.fill + S.P;
+  char *stage = (char *)(S.cur + S.P);
   // stage layout: K keys[TR] | per column [TR] cells | u16 pid[TR]
-  K *skey = (K *)stage;
-  char *scol = stage + (size_t)TR * sizeof(K);
-  const int ncol = NC == 1 ? 1 : s.ncols;
-  uint32_t colb[NC];
-  size_t coff[NC];
+  S.skey = (K *)stage;
+  S.scol = stage + (size_t)S.TR * sizeof(K);
   size_t acc_off = 0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    colb[c] = VW ? (uint32_t)VW : c < ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
-    coff[c] = acc_off;
-    acc_off += (size_t)TR * colb[c];
+    S.colb[c] = VW ? (uint32_t)VW : c < S.ncol ? (uint32_t)dtype_bytes(s.col_dtype[c]) : 0u;
+    S.coff[c] = acc_off;
+    acc_off += (size_t)S.TR * S.colb[c];
   }
-  uint16_t *spid = (uint16_t *)(scol + acc_off);
+  S.spid = (uint16_t *)(S.scol + acc_off);
+  S.scratch = scratch;
+  S.bpart = bpart;
+  S.bcnt = bcnt;
+  S.pool = pool;
+  S.offs = offs;
+  S.pbase = pbase;
+  S.okeys = okeys;
+  S.o = o;
+  S.rlo = S.rhi = 0;
 
-  K kk[ITEMS], nkk[ITEMS];
-  CellT cell[ITEMS][NC], ncell[ITEMS][NC];
-  uint32_t rlo = 0, rhi = 0;  // this wave's reserve of bucket ids (wave-uniform)
   // chunk of consecutive tiles; chunks renumbered so the chunks of one XCD
   // are neighbours too (the grid is a multiple of the XCD count)
   const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
   uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
   const uint32_t tend = min(ntiles, tile + per);
   if (tile >= tend) return;  // uniform per workgroup
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-    fill[p] = 0;
-    cur[p] = 0;
+  for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {
+    S.fill[p] = 0;
+    S.cur[p] = 0;
+    S.hist[p] = 0;
   }
-  // rows past the end re-read the last row (in bounds); they are not staged
-#define TGA_SCATTER_LOAD(T, KK, CELL)                                                           \
-  {                                                                                             \
-    const uint64_t b0 = (uint64_t)(T) * TR;                                                     \
-    _Pragma("unroll") for (int i = 0; i < ITEMS; ++i) {                                         \
-      const uint64_t row = b0 + (uint64_t)i * blockDim.x + threadIdx.x;                         \
-      const uint64_t r = row < n ? row : n - 1;                                                 \
-      KK[i] = keys[r];                                                                          \
-      _Pragma("unroll") for (int c = 0; c < NC; ++c) {                                          \
-        if (c >= ncol) { CELL[i][c] = 0; continue; }                                            \
-        CELL[i][c] = colb[c] == 4 ? (CellT)((const uint32_t *)s.cols[c])[r]                     \
-                                  : (CellT)((const uint64_t *)s.cols[c])[r];                    \
-      }                                                                                         \
-    }                                                                                           \
+  K ka[ITEMS], kb[ITEMS];
+  CellT ca[ITEMS][NC], cb[ITEMS][NC];
+  uint32_t pp[ITEMS], rk[ITEMS];
+  scatter_load<K, ITEMS, NC, CellT>(S, tile, ka, ca);
+  lds_barrier();
+  scatter_rank<K, ITEMS, NC>(S, tile, ka, S.hist, pp, rk);
+  // all of the first tile's loads done before the next tile's go out, so no
+  // load is outstanding on set A where the loop starts (as after each tile)
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+  scatter_pin<K, ITEMS, NC, CellT>(ka, ca);
+  if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, tile + 1, kb, cb);  // clamped past the end
+
+  // One tile: X holds its ranked rows; with kEarly, Y is receiving the next
+  // tile's rows, and X is refilled with the tile after that.
+#define TGA_SCATTER_TILE(X, XC, Y, YC)                                                           \
+  {                                                                                              \
+    lds_barrier(); /* the tile's counts are complete, the previous writes done */               \
+    const uint32_t rows = block_exclusive_scan_lds<LdsBarrier>(S.hist, S.P, scratch);           \
+    scatter_extend<BK, K, NC>(S, rows, tile);                                                              \
+    scatter_stage<K, ITEMS, NC, CellT>(S, tile, X, XC, pp, rk);                                  \
+    /* vmcnt counts loads and stores in order: Y's loads are waited for     */                   \
+    /* here, before this tile's stores, not behind them                      */                  \
+    if constexpr (kEarly) {                                                                      \
+      __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0) */                              \
+      scatter_pin<K, ITEMS, NC, CellT>(Y, YC);                                                   \
+    }                                                                                            \
+    lds_barrier(); /* staged; the next histogram is zeroed */                                    \
+    const uint32_t next = tile + 1;                                                              \
+    if constexpr (!kEarly) {                                                                     \
+      if (next < tend) scatter_load<K, ITEMS, NC, CellT>(S, next, Y, YC);                        \
+    }                                                                                            \
+    scatter_write<BK, K, NC>(S, rows);                                                               \
+    if (next >= tend) break;                                                                     \
+    /* the next tile, ranked into the other histogram while the stores drain */                  \
+    scatter_rank<K, ITEMS, NC>(S, next, Y, S.hnext, pp, rk);                                     \
+    if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, next + 1, X, XC);                 \
+    uint32_t *t_ = S.hist;                                                                       \
+    S.hist = S.hnext;                                                                            \
+    S.hnext = t_;                                                                                \
+    tile = next;                                                                                 \
   }
-  TGA_SCATTER_LOAD(tile, kk, cell)
-  for (;;) {
-    const uint64_t b = (uint64_t)tile * TR;
-    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-    __syncthreads();
-    uint32_t pp[ITEMS], rk[ITEMS];
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-      pp[i] = key_partition((uint64_t)kk[i], pbits);
-      rk[i] = row < n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
+  if constexpr (kEarly) {
+    for (;;) {
+      TGA_SCATTER_TILE(ka, ca, kb, cb)
+      TGA_SCATTER_TILE(kb, cb, ka, ca)
     }
-    const uint32_t next = tile + 1;
-    if constexpr (kEarly) {
-      if (next < tend) TGA_SCATTER_LOAD(next, nkk, ncell)
-    }
-    __syncthreads();
-    const uint32_t rows = block_exclusive_scan_lds(hist, P, scratch);
-    // extend every partition's bucket chain by its run of this tile
-    for (uint32_t p0 = 0; p0 < P; p0 += blockDim.x) {
-      const uint32_t p = p0 + threadIdx.x;
-      const bool on = p < P;
-      uint32_t off = 0, c = 0, f = 0, first = 0, fresh = 0;
-      if (on) {
-        off = hist[p];
-        c = (p + 1 < P ? hist[p + 1] : rows) - off;
-        f = fill[p];
-        first = cur[p];
-        if (c) fresh = ((f + c + kBucketRows - 1) >> kBucketShift) - (f ? 1u : 0u);
-      }
-      // single fresh buckets: from the wave's reserve
-      const bool one = fresh == 1;
-      const uint64_t m1 = __ballot(one);
-      const uint32_t n1 = (uint32_t)__builtin_popcountll(m1);
-      const uint32_t j = (uint32_t)__builtin_popcountll(m1 & lanemask_lt());
-      uint32_t base = 0;
-      if (n1) {
-        const uint32_t have = rhi - rlo;
-        uint32_t got = 0;
-        if (have < n1) {  // refill (wave-uniform branch)
-          if (lane == 0) got = atomicAdd(pool, kReserve);
-          got = wave_uniform((uint32_t)__shfl((int)got, 0, kWave));
-        }
-        if (one) base = j < have ? rlo + j : got + (j - have);
-        if (have < n1) {
-          rlo = got + (n1 - have);
-          rhi = got + kReserve;
-        } else {
-          rlo += n1;
-        }
-      }
-      if (fresh > 1) base = atomicAdd(pool, fresh);  // several: contiguous, own atomic
-      uint32_t delta = 0;
-      if (on && c) {
-        for (uint32_t k = 0; k < fresh; ++k)
-          if (base + k < cap) bpart[base + k] = p;
-        delta = f ? base - 1 : base;
-        if (!f) first = base;
-        const uint32_t end = f + c;
-        const uint32_t full = end >> kBucketShift;  // buckets this run completes
-        for (uint32_t k = 0; k < full; ++k) {
-          const uint32_t id = bucket_of(k, first, delta);
-          if (id < cap) bcnt[id] = kBucketRows;
-        }
-        fill[p] = end & (kBucketRows - 1);
-        cur[p] = bucket_of((end - 1) >> kBucketShift, first, delta);
-      }
-      if (on) wdesc[p] = make_uint4(f - off, first, delta, 0u);
-    }
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-      if (row < n) {
-        const uint32_t lp = hist[pp[i]] + rk[i];
-        skey[lp] = kk[i];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          if (c >= ncol) break;
-          if (colb[c] == 4) ((uint32_t *)(scol + coff[c]))[lp] = (uint32_t)cell[i][c];
-          else ((uint64_t *)(scol + coff[c]))[lp] = (uint64_t)cell[i][c];
-        }
-        spid[lp] = (uint16_t)pp[i];
-      }
-    }
-    __syncthreads();
-    if constexpr (!kEarly) {
-      if (next < tend) TGA_SCATTER_LOAD(next, kk, cell)
-    }
-    for (uint32_t jr = threadIdx.x; jr < rows; jr += blockDim.x) {
-      const uint4 d = wdesc[spid[jr]];
-      const uint32_t q = jr + d.x;
-      const uint32_t id = bucket_of(q >> kBucketShift, d.y, d.z);
-      if (id >= cap) continue;
-      const uint64_t dst = ((uint64_t)id << kBucketShift) | (q & (kBucketRows - 1));
-      okeys[dst] = skey[jr];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        if (c >= ncol) break;
-        if (colb[c] == 4) ((uint32_t *)o.cols[c])[dst] = ((const uint32_t *)(scol + coff[c]))[jr];
-        else ((uint64_t *)o.cols[c])[dst] = ((const uint64_t *)(scol + coff[c]))[jr];
-      }
-    }
-    __syncthreads();
-    if (next >= tend) break;
-    tile = next;
-    if constexpr (kEarly) {
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        kk[i] = nkk[i];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) cell[i][c] = ncell[i][c];
-      }
-    }
+  } else {
+    for (;;) TGA_SCATTER_TILE(ka, ca, ka, ca)
   }
-#undef TGA_SCATTER_LOAD
-  // the open buckets are the last of their chains
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x)
-    if (fill[p] && cur[p] < cap) bcnt[cur[p]] = fill[p];
-  // unused reserve ids
-  for (uint32_t id = rlo + lane; id < rhi; id += kWave)
-    if (id < cap) bpart[id] = ~0u;
+#undef TGA_SCATTER_TILE
+  if constexpr (BK) {
+    // the open buckets are the last of their chains
+    for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x)
+      if (S.fill[p] && S.cur[p] < cap) bcnt[S.cur[p]] = S.fill[p];
+    // unused reserve ids
+    for (uint32_t id = S.rlo + (uint32_t)__lane_id(); id < S.rhi; id += kWave)
+      if (id < cap) bpart[id] = ~0u;
+  }
 }
 
 // Per-partition bucket and row totals from the bucket headers (pbk, prow
@@ -749,17 +974,29 @@ static hipError_t set_lds_limit(const void *fn, size_t bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename K, bool FX, int NC, int VW, bool FOLD>
+template <typename K, bool FX, int NC, int VW, bool FOLD, uint32_t SIG = 0>
 static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan &plan, const Layout &L, const GTable &g,
                                int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st, int emit,
                                const OutDesc &o, uint64_t *emit_count, uint32_t *ovf) {
   const size_t lds = L.bytes;
-  hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC, VW, FOLD>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_agg<K, FX, NC, VW, FOLD, SIG>, lds);
   if (e != hipSuccess) return e;
   const AccProg ap = make_acc_prog(s, L);
-  hipLaunchKernelGGL((k_agg<K, FX, NC, VW, FOLD>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g,
+  hipLaunchKernelGGL((k_agg<K, FX, NC, VW, FOLD, SIG>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, plan, L, g,
                      ap, nbits, max_probe, emit, o, emit_count, ovf);
   return hipGetLastError();
+}
+
+// The compile-time accumulator signature a schema matches (see prog_kind),
+// or 0: one 4-byte value column, the accumulators in one of the fixed orders.
+static uint32_t fixed_signature(const Schema &s) {
+  if (s.nfx > 0 || s.ncols != 1 || dtype_bytes(s.col_dtype[0]) != 4 || s.nacc > 4) return 0;
+  uint32_t sig = 0;
+  for (int a = 0; a < s.nacc; ++a) {
+    if (s.acc[a].kind != ACC_COUNT && s.acc[a].col != 0) return 0;
+    sig |= (uint32_t)(s.acc[a].kind + 1) << (4 * a);
+  }
+  return sig == kSigCountSum || sig == kSigCountSumMinMax ? sig : 0;
 }
 
 // One kernel per (key width, fp64 SUM present, value columns, wave folding):
@@ -774,9 +1011,13 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
   const bool fx = s.nfx > 0;
   const bool one = s.ncols <= 1;
   const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
+  const uint32_t sig = fixed_signature(s);
 #define TGA_AGG_ARGS keys, s, plan, L, g, nbits, max_probe, grid, block, st, emit, o, emit_count, ovf
 #define TGA_AGG_CASE(K)                                                                         \
   if (fx) return launch_agg_t<K, true, kMaxCols, 0, true>(TGA_AGG_ARGS);                        \
+  if (!fold && sig == kSigCountSumMinMax)                                                       \
+    return launch_agg_t<K, false, 1, 4, false, kSigCountSumMinMax>(TGA_AGG_ARGS);               \
+  if (!fold && sig == kSigCountSum) return launch_agg_t<K, false, 1, 4, false, kSigCountSum>(TGA_AGG_ARGS); \
   if (one) {                                                                                    \
     if (w8) return fold ? launch_agg_t<K, false, 1, 8, true>(TGA_AGG_ARGS)                      \
                         : launch_agg_t<K, false, 1, 8, false>(TGA_AGG_ARGS);                    \
@@ -807,23 +1048,24 @@ hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
 }
 
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows) {
-  size_t b = (size_t)28 << pbits;  // wdesc + hist + fill + cur
+  size_t b = (size_t)32 << pbits;  // wdesc + 2 histograms + fill + cur
   size_t row = (size_t)key_bytes + 2;
   for (int c = 0; c < s.ncols; ++c) row += (size_t)dtype_bytes(s.col_dtype[c]);
   return b + row * tile_rows;
 }
 
-template <typename K, int NC, int VW>
-static hipError_t launch_bucket_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
-                                  uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
-                                  hipStream_t st) {
+template <typename K, int NC, int VW, bool BK>
+static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                   uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs *bb,
+                                   const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
   constexpr int ITEMS = 8;
   const unsigned block = tile_rows / ITEMS;
   const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
-  hipError_t e = set_lds_limit<K>((const void *)&k_part_bucket<K, ITEMS, NC, VW>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC, VW, BK>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_part_bucket<K, ITEMS, NC, VW>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n,
-                     pbits, ntiles, (K *)okeys, o, bb.bpart, bb.bcnt, bb.pool, bb.cap);
+  hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC, VW, BK>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n,
+                     pbits, ntiles, (K *)okeys, o, bb ? bb->bpart : nullptr, bb ? bb->bcnt : nullptr,
+                     bb ? bb->pool : nullptr, bb ? bb->cap : 0u, offs, pbase);
   return hipGetLastError();
 }
 
@@ -838,21 +1080,67 @@ uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows) {
   return (n + kBucketRows - 1) / kBucketRows + (uint64_t)grid * ((1ULL << pbits) + block) + 1;
 }
 
-hipError_t launch_part_bucket(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
-                              uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs &bb, unsigned grid,
-                              hipStream_t st) {
+hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
+                               uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs *bb,
+                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
   const uint32_t ntiles = (uint32_t)((n + tile_rows - 1) / tile_rows);
   if (ntiles == 0) return hipSuccess;
   const bool one = s.ncols <= 1;
   const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
-#define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, grid, st
-#define TGA_SC_CASE(K)                                                                              \
-  if (one) return w8 ? launch_bucket_t<K, 1, 8>(TGA_SC_ARGS) : launch_bucket_t<K, 1, 4>(TGA_SC_ARGS); \
-  return launch_bucket_t<K, kMaxCols, 0>(TGA_SC_ARGS);
-  if (key_bytes == 4) { TGA_SC_CASE(uint32_t) }
-  TGA_SC_CASE(uint64_t)
+#define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st
+#define TGA_SC_CASE(K, BK)                                                                          \
+  if (one) return w8 ? launch_scatter_t<K, 1, 8, BK>(TGA_SC_ARGS) : launch_scatter_t<K, 1, 4, BK>(TGA_SC_ARGS); \
+  return launch_scatter_t<K, kMaxCols, 0, BK>(TGA_SC_ARGS);
+  if (bb) {
+    if (key_bytes == 4) { TGA_SC_CASE(uint32_t, true) }
+    TGA_SC_CASE(uint64_t, true)
+  }
+  if (key_bytes == 4) { TGA_SC_CASE(uint32_t, false) }
+  TGA_SC_CASE(uint64_t, false)
 #undef TGA_SC_CASE
 #undef TGA_SC_ARGS
+}
+
+hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
+                              hipStream_t st) {
+  const size_t lds = (size_t)(P + 1) * 4;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)&k_plan_chunks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, pbase, P, n, chmax, plan);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_count(int key_bytes, const void *keys, uint64_t n, int pbits, uint32_t tile_rows,
+                             uint32_t ntiles, uint32_t *cnt, hipStream_t st) {
+  const size_t lds = (size_t)4 << pbits;
+  if (key_bytes == 4)
+    hipLaunchKernelGGL(k_part_count<uint32_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint32_t *)keys, n,
+                       pbits, tile_rows, ntiles, cnt);
+  else
+    hipLaunchKernelGGL(k_part_count<uint64_t>, dim3(xcd_grid(ntiles)), dim3(512), lds, st, (const uint64_t *)keys, n,
+                       pbits, tile_rows, ntiles, cnt);
+  return hipGetLastError();
+}
+
+uint64_t colscan_desc_words(uint32_t ntiles, uint32_t P) {
+  return (uint64_t)((ntiles + kColTiles - 1) / kColTiles) * P;
+}
+
+hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint64_t *desc, uint32_t *ticket,
+                          uint32_t *offs, uint32_t *total, uint32_t *pbase, hipStream_t st) {
+  const uint32_t nchunks = (ntiles + kColTiles - 1) / kColTiles;
+  hipLaunchKernelGGL(k_colscan, dim3(nchunks), dim3(256), 0, st, cnt, ntiles, P, desc, ticket, offs, total);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t lds = (size_t)P * 4;
+  if (lds > 65536) {
+    e = hipFuncSetAttribute((const void *)&k_colbase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_colbase, dim3(1), dim3(1024), lds, st, total, P, pbase);
+  return hipGetLastError();
 }
 
 static hipError_t lds_attr(const void *fn, size_t bytes) {

@@ -80,7 +80,9 @@ typedef struct tga_options {
   int64_t chunk_rows;   /* max rows per aggregation work item */
   int32_t wave_fold;    /* 0 = tuner decides, 1 = fold equal keys of a wave before the LDS atomics, 2 = never */
   int32_t attempts;     /* tga_last_plan only: executions the last call needed (finer fanout / larger tables) */
-  int32_t reserved[6];
+  int32_t scatter_layout; /* partitioned copy: 0 = tuner, 1 = partition-contiguous (a histogram pass first),
+                             2 = per-workgroup buckets (no histogram pass) */
+  int32_t reserved[5];
 } tga_options;
 
 typedef struct tga_context tga_context;
