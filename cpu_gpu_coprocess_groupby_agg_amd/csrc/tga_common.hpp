@@ -148,6 +148,12 @@ TGA_HD uint32_t hash_partition(uint64_t h, int pbits) { return pbits ? (uint32_t
 TGA_HD uint32_t key_partition(uint64_t k, int pbits) {
   return pbits ? (uint32_t)((k * 0x9e3779b97f4a7c15ULL) >> 32) >> (32 - pbits) : 0u;
 }
+// LDS-slot hash of the integer row loop: the high word of k * M mod 2^64,
+// M odd (three 32-bit multiplies where key_hash needs two 64-bit ones and
+// the shifts between them).  M is unrelated to key_partition's multiplier,
+// so the keys of one partition still spread over the whole table; every
+// insert into and lookup in one LDS table uses the same hash.
+TGA_HD uint32_t key_slot_hash(uint64_t k) { return (uint32_t)((k * 0xd6e8feb86659fd93ULL) >> 32); }
 TGA_HD uint32_t hash_lds_slot(uint64_t h, uint32_t slots) {
   return (uint32_t)(((uint64_t)(uint32_t)h * slots) >> 32);
 }

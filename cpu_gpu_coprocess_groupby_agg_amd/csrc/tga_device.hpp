@@ -611,9 +611,9 @@ __device__ __forceinline__ uint64_t prog_operand(uint32_t kind, uint32_t map, ui
 // on purpose: the row loop then carries none of the global-table or schema
 // descriptors (they are read here, from `g` and `s`).
 template <typename K>
-__device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, uint64_t h, uint64_t raw0,
+__device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, uint64_t raw0,
                                        uint64_t raw1, uint64_t raw2, uint64_t raw3) {
-  const int64_t gs = g_find_or_insert<K>(*g, key, h);
+  const int64_t gs = g_find_or_insert<K>(*g, key, key_hash((uint64_t)key));
   if (gs < 0) return;
   const uint64_t cell[kMaxCols] = {raw0, raw1, raw2, raw3};
   for (int a = 0; a < s->nacc; ++a) g_apply(*g, a, s->acc[a].kind, gs, acc_operand(*s, a, cell));
@@ -728,14 +728,14 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     uint64_t nraw[U][NC];
     if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ne, lane, nkk, nraw);
     ent_next = list_entry(bl, b0, e0, it + 1, wave, nw);
-    uint64_t hh[U];
+    uint32_t hh[U];
     int slot[U];
     // first buckets of all U rows, read before any is resolved
     uint32_t s0[U];
     K k4[U][4];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      hh[u] = key_hash((uint64_t)kk[u]);
+      hh[u] = key_slot_hash((uint64_t)kk[u]);
       s0[u] = hash_lds_slot(hh[u], nb) << 2;
       lds_bucket<K>(tk, s0[u], k4[u]);
     }
@@ -769,7 +769,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)
-            spill_row<K>(gsp, ssp, kk[u], hh[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
+            spill_row<K>(gsp, ssp, kk[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
                          NC > 2 ? raw[u][NC > 2 ? 2 : 0] : 0, NC > 3 ? raw[u][NC > 3 ? 3 : 0] : 0);
         }
       }

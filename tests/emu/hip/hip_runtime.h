@@ -58,6 +58,7 @@ uint32_t first_lane(int line, uint32_t v);
 #define __syncthreads() ::emu::syncthreads(__LINE__)
 #define __builtin_amdgcn_s_barrier() ::emu::syncthreads(__LINE__)
 #define __builtin_amdgcn_s_waitcnt(imm) ((void)(imm))
+#define __builtin_amdgcn_sched_barrier(m) ((void)(m))
 #define __ballot(p) ::emu::ballot(__LINE__, (p))
 #define __shfl(...) ::emu::shfl(__LINE__, __VA_ARGS__)
 #define __shfl_up(...) ::emu::shfl_up(__LINE__, __VA_ARGS__)
