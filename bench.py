@@ -56,7 +56,7 @@ def parse():
 
 
 # profiler name of a timed step -> kernel symbol it launches
-KERNEL_SYMBOL = {"part_scatter": "k_part_bucket", "agg_partitions": "k_agg", "agg_direct": "k_agg",
+KERNEL_SYMBOL = {"part_scatter": "k_part_scatter", "agg_partitions": "k_agg", "agg_direct": "k_agg",
                  "compact": "k_compact", "copy16": "k_copy16"}
 
 

@@ -280,17 +280,18 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     p.chunk_rows = o.chunk_rows > 0 ? (uint64_t)o.chunk_rows : std::max<uint64_t>(1, (n + W - 1) / W);
   } else {
     // the largest table one workgroup's LDS holds (any slot count: the slot
-    // is a multiply-shift of the hash), filled to about 2/5 on average: the
-    // first 4-slot bucket then holds nearly every key, so almost no row
+    // is a multiply-shift of the hash), filled to at most 1/2 on average:
+    // the first 4-slot bucket then holds nearly every key, so almost no row
     // leaves the branch-free lookup (at 3/4 fill the aggregation runs ~4x
-    // slower on the 2^28-row, 2^20-group workload)
+    // slower on the 2^28-row, 2^20-group workload).  The fanout is a power
+    // of two, so the fill lands between 1/4 and 1/2.
     uint32_t tfull = (uint32_t)std::min<uint64_t>(65535, (lds_max - 512) / sb - 2);
     tfull &= ~63u;
     uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
     int pbits = o.partition_bits;
     if (pbits <= 0) {
-      const uint64_t P = next_pow2((est * 5 + 2 * (uint64_t)T - 1) / (2 * (uint64_t)T));
+      const uint64_t P = next_pow2((est * 2 + (uint64_t)T - 1) / (uint64_t)T);
       pbits = std::max(1, ilog2(P));
     }
     pbits = std::min(pbits, 12);
@@ -365,12 +366,20 @@ uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_s
   if (m >= n) return d;  // the sample is the whole input
   // GEE (Charikar et al. 2000): sqrt(n/m) f1 + (d - f1); good under skew
   const double gee = std::sqrt((double)n / (double)m) * (double)f1 + (double)(d - f1);
-  // equal-frequency model: f1/m = exp(-m/D) for D >> ... ; good for uniform keys,
-  // where GEE badly underestimates when most sampled keys are singletons
+  // equal-frequency model: a key of D equally frequent ones is a singleton
+  // of the sample with probability ~exp(-m/D), so f1/m = exp(-m/D)
   double uni = (double)n;
   if (f1 < m) uni = -(double)m / std::log((double)f1 / (double)m + 1e-300);
   if (f1 == 0) uni = (double)d;
-  const double e = std::max(gee, std::min(uni, (double)n));
+  uni = std::min(uni, (double)n);
+  // Without heavy hitters in the sample (its largest count is what sampling
+  // D equally frequent keys gives), the equal-frequency model is the right
+  // one: there GEE returns ~sqrt(n m) whatever D is (2x too high for 2^20
+  // keys in 2^28 rows).  With heavy hitters (skewed keys) take the larger of
+  // the two; an underestimate is caught by the overflow replan anyway.
+  const double mean = (double)m / std::max(uni, 1.0);
+  const bool flat = (double)top <= 8.0 + 4.0 * mean;
+  const double e = flat ? uni : std::max(gee, uni);
   return (uint64_t)std::min<double>((double)n, std::max<double>((double)d, e));
 }
 
