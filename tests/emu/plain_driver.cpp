@@ -240,6 +240,18 @@ int main(int argc, char **argv) {
   // that the waves refill their bucket reserves several times
   cases.push_back({"buckets_p1024", 64, keys_u64(3 * N, 200000, 33), {col_i32(3 * N, -99, 99, 34)}, {TGA_I32},
                    {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 10, 0, 200000)});
+  // the partition-contiguous layout: histogram, column scan, scatter to the
+  // scanned offsets, chunk plan over contiguous partitions
+  {
+    tga_options o = opts(TGA_PATH_PARTITIONED, 2, 6, 0, 30000);
+    o.scatter_layout = 1;
+    cases.push_back({"contiguous_u64", 64, keys_u64(2 * N, 30000, 51), {col_i32(2 * N, -99, 99, 52)}, {TGA_I32},
+                     {a4[0], a4[1], a4[2], a4[3]}, o});
+    tga_options o2 = opts(TGA_PATH_PARTITIONED, 2, 3, 0, 5000);
+    o2.scatter_layout = 1;
+    cases.push_back({"contiguous_multi_col", 32, keys_u32(N, 5000, 53), {col_i32(N, -99, 99, 54), col_i32(N, -7, 7, 55)},
+                     {TGA_I32, TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, o2});
+  }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
   int bad = 0, ran = 0;

@@ -54,6 +54,25 @@ def test_i64_keys_four_aggs(ctx, path):
     check_equal(got, ref)
 
 
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("ncols", [1, 2])
+def test_partition_layouts(ctx, layout, ncols):
+    # both partitioned-copy layouts: partition-contiguous (histogram + column
+    # scan, then the scatter to scanned offsets) and per-workgroup buckets
+    # (count-free scatter, bucket lists); one column (the single-column
+    # scatter with two register sets) and two (the multi-column one); enough
+    # tiles that the bucket reserves refill, and a last tile that is ragged
+    n, d = (1 << 22) + 4097, 150_000
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=41, salt=2)
+    cols = [ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=42 + c) for c in range(ncols)]
+    aggs = AGGS4 + ([("sum", 1), ("min", 1)] if ncols > 1 else [])
+    got = ctx.groupby(k, cols, aggs, path=tga.PATH_PARTITIONED, scatter_layout=layout)
+    assert ctx.last_plan()["scatter_layout"] == layout
+    ref = O.groupby(k.to_numpy(), [c.to_numpy() for c in cols], aggs)
+    check_equal(got, ref)
+    assert got["n_groups"] == d
+
+
 def test_partitioned_small_tables_many_partitions(ctx):
     n, d = 1 << 21, 200_000
     k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=15, salt=1)
