@@ -456,6 +456,7 @@ struct PartBufs {
   DBuf pcols[kMaxCols];
   BucketBufs bb{};
   bool buckets = false;
+  bool rec = false;  // pkeys holds (key, 4-byte value) records, no pcols
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
   void keep(std::vector<DBuf> &k) {
@@ -497,12 +498,15 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     bb.cap = (uint32_t)cap;
     TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, st));
   }
-  if (!pb.pkeys.alloc(ctx, std::max<uint64_t>(slots, 1) * q.key_bytes) ||
+  pb.rec = copy_as_records(s);
+  const uint64_t row_bytes = (uint64_t)q.key_bytes + (pb.rec ? 4 : 0);
+  if (!pb.pkeys.alloc(ctx, std::max<uint64_t>(slots, 1) * row_bytes) ||
       !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   pb.sp = s;
-  for (int c = 0; c < s.ncols; ++c) {
+  if (pb.rec) pb.sp.cols[0] = nullptr;  // the values travel in the records
+  for (int c = 0; c < s.ncols && !pb.rec; ++c) {
     if (!pb.pcols[c].alloc(ctx, std::max<uint64_t>(slots, 1) * (uint64_t)dtype_bytes(s.col_dtype[c])))
       return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
     po.cols[c] = pb.pcols[c].p;
@@ -577,7 +581,7 @@ tga_status aggregate_into(tga_context *ctx, const QuerySpec &q, const void *d_ke
       }
       TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
         return launch_agg(q.key_bytes, pb.pkeys.p, pb.sp, pb.ap, plan.layout, g, plan.nbits, plan.max_probe,
-                          plan.workgroups, plan.block, st, 0, nullptr, nullptr, plan.fold);
+                          plan.workgroups, plan.block, st, 0, nullptr, nullptr, plan.fold, nullptr, pb.rec);
       }));
       pb.keep(keep);
       if (r0 + kSuper < n) {  // reuse the buffers of this super-chunk
@@ -681,7 +685,8 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   }
   TGA_HIP(ctx, timed(ctx, "agg_partitions", [&] {
     return launch_agg(q.key_bytes, pb.pkeys.p, pb.sp, ap, plan.layout, g, plan.nbits, plan.max_probe, plan.workgroups,
-                      plan.block, st, 1, &od, emit_count, plan.fold, may_abandon ? ovf.as<uint32_t>() : nullptr);
+                      plan.block, st, 1, &od, emit_count, plan.fold, may_abandon ? ovf.as<uint32_t>() : nullptr,
+                      pb.rec);
   }));
   // the global table's groups follow the emitted ones
   const uint64_t ctiles = compact_tiles(g.S + 1);

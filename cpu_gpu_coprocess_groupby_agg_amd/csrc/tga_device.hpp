@@ -578,6 +578,40 @@ __device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schem
 // Raw 64-bit image of value column c at `row` (dtype packed in coldt, 4 bits
 // per column).  VW = 4 or 8: every value column has that width, so the load
 // is branch-free; VW = 0: the width is read from coldt.
+// Value width code of the row loaders for a partitioned copy stored as
+// records (key, 4-byte value): 8 bytes with 32-bit keys, 12 with 64-bit.
+constexpr int kVwRec4 = 100;
+
+// A (key, 4-byte value) record.  The 12-byte form is three 4-byte words:
+// the compiler merges their accesses into one dwordx3 load or store.  (A
+// 3-element vector type would not do: clang widens its loads and stores to
+// 16 bytes, which reaches into the next record.)
+struct Rec12 {
+  uint32_t lo, hi, val;
+};
+template <typename K>
+struct Rec4 {
+  static constexpr int kBytes = (int)sizeof(K) + 4;
+  __device__ __forceinline__ static void load(const void *base, uint64_t r, K &key, uint32_t &val) {
+    if constexpr (sizeof(K) == 8) {
+      const Rec12 v = ((const Rec12 *)base)[r];
+      key = (K)(((uint64_t)v.hi << 32) | v.lo);
+      val = v.val;
+    } else {
+      const uint2 v = ((const uint2 *)base)[r];
+      key = (K)v.x;
+      val = v.y;
+    }
+  }
+  __device__ __forceinline__ static void store(void *base, uint64_t r, K key, uint32_t val) {
+    if constexpr (sizeof(K) == 8) {
+      ((Rec12 *)base)[r] = Rec12{(uint32_t)(uint64_t)key, (uint32_t)((uint64_t)key >> 32), val};
+    } else {
+      ((uint2 *)base)[r] = make_uint2((uint32_t)key, val);
+    }
+  }
+};
+
 template <int VW>
 __device__ __forceinline__ uint64_t load_col(const void *const *cols, uint32_t coldt, int c, uint64_t row) {
   const uint32_t dt = (coldt >> (4 * c)) & 15u;
@@ -654,9 +688,16 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
   for (int u = 0; u < U; ++u) {
     const uint64_t row = base + (uint64_t)u * kWave + lane;
     const uint64_t r = row < e ? row : e - 1;
-    kk[u] = keys[r];
+    if constexpr (VW == kVwRec4) {  // records (key, 4-byte value) at `keys`
+      static_assert(NC == 1, "records carry one value column");
+      uint32_t v;
+      Rec4<K>::load(keys, r, kk[u], v);
+      raw[u][0] = (coldt & 15u) == DT_I32 ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
+    } else {
+      kk[u] = keys[r];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
+      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
+    }
   }
 }
 

@@ -37,7 +37,10 @@ hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, h
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
                       int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0,
-                      uint32_t *ovf = nullptr);
+                      uint32_t *ovf = nullptr, int rec = 0);
+// rec: `keys` holds (key, 4-byte value) records (a partitioned copy of a
+// one-column schema with a 4-byte value column), not keys and s.cols
+bool copy_as_records(const Schema &s);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
 uint64_t scan_tiles(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,

@@ -537,8 +537,10 @@ __device__ __forceinline__ void scatter_stage(const ScatterState<K, NC> &S, uint
 }
 
 // The staged rows to their destinations: consecutive rows of one run land in
-// consecutive rows of the partition (of a bucket, with BK).
-template <bool BK, typename K, int NC>
+// consecutive rows of the partition (of a bucket, with BK).  REC: the copy
+// is (key, 4-byte value) records, one store per row -- one write stream
+// instead of two, so half the partially written cache lines at run ends.
+template <bool BK, bool REC, typename K, int NC>
 __device__ __forceinline__ void scatter_write(const ScatterState<K, NC> &S, uint32_t rows) {
   for (uint32_t jr = threadIdx.x; jr < rows; jr += blockDim.x) {
     const uint4 d = S.wdesc[S.spid[jr]];
@@ -548,6 +550,10 @@ __device__ __forceinline__ void scatter_write(const ScatterState<K, NC> &S, uint
       const uint32_t id = bucket_of(q >> kBucketShift, d.y, d.z);
       if (id >= S.cap) continue;
       dst = ((uint64_t)id << kBucketShift) | (q & (kBucketRows - 1));
+    }
+    if constexpr (REC) {
+      Rec4<K>::store(S.okeys, dst, S.skey[jr], ((const uint32_t *)S.scol)[jr]);
+      continue;
     }
     S.okeys[dst] = S.skey[jr];
 #pragma unroll
@@ -559,13 +565,14 @@ __device__ __forceinline__ void scatter_write(const ScatterState<K, NC> &S, uint
   }
 }
 
-template <typename K, int ITEMS, int NC, int VW, bool BK>
+template <typename K, int ITEMS, int NC, int VW, bool BK, bool REC>
 __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ keys, Schema s, uint64_t n, int pbits,
                                                        uint32_t ntiles, K *okeys, PartOut o, uint32_t *bpart,
                                                        uint32_t *bcnt, uint32_t *pool, uint32_t cap,
                                                        const uint32_t *offs, const uint32_t *pbase) {
   using CellT = typename std::conditional<VW == 4, uint32_t, uint64_t>::type;
   constexpr bool kEarly = NC == 1;
+  static_assert(!REC || (NC == 1 && VW == 4), "records carry one 4-byte value");
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t scratch[17];
   ScatterState<K, NC> S;
@@ -646,7 +653,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
     if constexpr (!kEarly) {                                                                     \
       if (next < tend) scatter_load<K, ITEMS, NC, CellT>(S, next, Y, YC);                        \
     }                                                                                            \
-    scatter_write<BK, K, NC>(S, rows);                                                               \
+    scatter_write<BK, REC, K, NC>(S, rows);                                                               \
     if (next >= tend) break;                                                                     \
     /* the next tile, ranked into the other histogram while the stores drain */                  \
     scatter_rank<K, ITEMS, NC>(S, next, Y, S.hnext, pp, rk);                                     \
@@ -987,6 +994,10 @@ static hipError_t launch_agg_t(const void *keys, const Schema &s, const AggPlan 
   return hipGetLastError();
 }
 
+bool copy_as_records(const Schema &s) {
+  return s.nfx == 0 && s.ncols == 1 && dtype_bytes(s.col_dtype[0]) == 4;
+}
+
 // The compile-time accumulator signature a schema matches (see prog_kind),
 // or 0: one 4-byte value column, the accumulators in one of the fixed orders.
 static uint32_t fixed_signature(const Schema &s) {
@@ -1005,7 +1016,7 @@ static uint32_t fixed_signature(const Schema &s) {
 // fold (their limb vectors make per-row atomics costly).
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
-                      int emit, const OutDesc *od, uint64_t *emit_count, int fold, uint32_t *ovf) {
+                      int emit, const OutDesc *od, uint64_t *emit_count, int fold, uint32_t *ovf, int rec) {
   OutDesc o{};
   if (od) o = *od;
   const bool fx = s.nfx > 0;
@@ -1026,6 +1037,17 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
   }                                                                                             \
   return fold ? launch_agg_t<K, false, kMaxCols, 0, true>(TGA_AGG_ARGS)                         \
               : launch_agg_t<K, false, kMaxCols, 0, false>(TGA_AGG_ARGS);
+#define TGA_AGG_REC(K)                                                                          \
+  if (fold) return launch_agg_t<K, false, 1, kVwRec4, true>(TGA_AGG_ARGS);                      \
+  if (sig == kSigCountSumMinMax) return launch_agg_t<K, false, 1, kVwRec4, false, kSigCountSumMinMax>(TGA_AGG_ARGS); \
+  if (sig == kSigCountSum) return launch_agg_t<K, false, 1, kVwRec4, false, kSigCountSum>(TGA_AGG_ARGS); \
+  return launch_agg_t<K, false, 1, kVwRec4, false>(TGA_AGG_ARGS);
+  if (rec) {  // keys = (key, 4-byte value) records
+    if (fx || !one || w8) return hipErrorInvalidValue;
+    if (key_bytes == 4) { TGA_AGG_REC(uint32_t) }
+    TGA_AGG_REC(uint64_t)
+  }
+#undef TGA_AGG_REC
   if (key_bytes == 4) { TGA_AGG_CASE(uint32_t) }
   TGA_AGG_CASE(uint64_t)
 #undef TGA_AGG_CASE
@@ -1054,16 +1076,16 @@ size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t til
   return b + row * tile_rows;
 }
 
-template <typename K, int NC, int VW, bool BK>
+template <typename K, int NC, int VW, bool BK, bool REC = false>
 static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
                                    uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs *bb,
                                    const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
   constexpr int ITEMS = 8;
   const unsigned block = tile_rows / ITEMS;
   const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
-  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC, VW, BK>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC, VW, BK, REC>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC, VW, BK>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n,
+  hipLaunchKernelGGL((k_part_scatter<K, ITEMS, NC, VW, BK, REC>), dim3(grid), dim3(block), lds, st, (const K *)keys, s, n,
                      pbits, ntiles, (K *)okeys, o, bb ? bb->bpart : nullptr, bb ? bb->bcnt : nullptr,
                      bb ? bb->pool : nullptr, bb ? bb->cap : 0u, offs, pbase);
   return hipGetLastError();
@@ -1087,8 +1109,10 @@ hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s,
   if (ntiles == 0) return hipSuccess;
   const bool one = s.ncols <= 1;
   const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
+  const bool rec = copy_as_records(s);
 #define TGA_SC_ARGS keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st
 #define TGA_SC_CASE(K, BK)                                                                          \
+  if (rec) return launch_scatter_t<K, 1, 4, BK, true>(TGA_SC_ARGS);                                   \
   if (one) return w8 ? launch_scatter_t<K, 1, 8, BK>(TGA_SC_ARGS) : launch_scatter_t<K, 1, 4, BK>(TGA_SC_ARGS); \
   return launch_scatter_t<K, kMaxCols, 0, BK>(TGA_SC_ARGS);
   if (bb) {

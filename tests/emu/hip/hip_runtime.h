@@ -38,6 +38,7 @@ struct dim3 {
 struct __attribute__((aligned(16))) uint4 { uint32_t x, y, z, w; };
 inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return uint4{x, y, z, w}; }
 struct __attribute__((aligned(8))) uint2 { uint32_t x, y; };
+inline uint2 make_uint2(uint32_t x, uint32_t y) { return uint2{x, y}; }
 
 namespace emu {
 extern thread_local dim3 t_block_idx, t_block_dim, t_grid_dim;
