@@ -55,9 +55,10 @@ def parse():
     return ap.parse_args()
 
 
-# profiler name of a timed step -> kernel symbol it launches
-KERNEL_SYMBOL = {"part_scatter": "k_part_scatter", "agg_partitions": "k_agg", "agg_direct": "k_agg",
-                 "compact": "k_compact", "copy16": "k_copy16"}
+# profiler name of a timed step -> kernel symbols it may launch (the scatter
+# is the write-combining one or the plain one, as the plan says)
+KERNEL_SYMBOL = {"part_scatter": ("k_part_scatter_wc", "k_part_scatter"), "agg_partitions": ("k_agg",),
+                 "agg_direct": ("k_agg",), "compact": ("k_compact",), "copy16": ("k_copy16",)}
 
 
 def pmc_pass(counter, argv, workdir):
@@ -96,9 +97,10 @@ def measure_traffic(dom_name, args):
     WRITE_SIZE, each scaled by what the counter reads for the copy kernel
     moving a known 2 GiB in and 2 GiB out."""
     import tempfile
-    sym = KERNEL_SYMBOL.get(dom_name)
-    if sym is None:
+    syms = KERNEL_SYMBOL.get(dom_name)
+    if syms is None:
         return None, None
+    sym = None
     head = ["--only", "headline", "--steps", "1", "--warmup", "1", "--no-traffic", "--rows-log2", str(args.rows_log2)]
     copy = ["--only", "copy", "--no-traffic"]
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as wd:
@@ -106,6 +108,8 @@ def measure_traffic(dom_name, args):
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             h = pmc_pass(counter, head, os.path.join(wd, "h"))
             c = pmc_pass(counter, copy, os.path.join(wd, "c"))
+            if sym is None:
+                sym = next((x for x in syms if h and x in h), None)
             if not h or not c or sym not in h or "k_copy16" not in c:
                 return None, None
             copy_kib = float(np.median(c["k_copy16"]))

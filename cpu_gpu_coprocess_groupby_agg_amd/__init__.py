@@ -67,7 +67,7 @@ class Options(ctypes.Structure):
                 ("table_slots", ctypes.c_int32), ("tile_rows", ctypes.c_int32), ("workgroups", ctypes.c_int32),
                 ("block_threads", ctypes.c_int32), ("chunk_rows", ctypes.c_int64),
                 ("wave_fold", ctypes.c_int32), ("attempts", ctypes.c_int32), ("scatter_layout", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("write_combine", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class Query(ctypes.Structure):

@@ -322,6 +322,18 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   // aggregation ~12% faster (no partially filled buckets) but costs more in
   // the extra pass (2^28 rows, 2^20 groups on MI355X: 4.24 vs 4.68 ms).
   p.scatter_layout = o.scatter_layout == 1 ? 1 : 2;
+  // Write combining (bucket layout, one 4-byte value column): each
+  // workgroup keeps every partition's unfinished cache line in LDS, so the
+  // scatter writes whole lines only.  Its carry buffer (128 B per partition)
+  // takes LDS from the tile, which the tuner shrinks to 6144 or 4096 rows.
+  if (p.path == TGA_PATH_PARTITIONED && p.scatter_layout == 2 && o.write_combine != 2 &&
+      copy_as_records(q.schema)) {
+    const uint32_t wt = scatter_wc_tile(q.key_bytes, p.pbits, lds_max);
+    if (wt && (o.tile_rows <= 0 || (uint32_t)o.tile_rows == wt)) {
+      p.tile_rows = wt;
+      p.scatter_wc = 1;
+    }
+  }
   p.opt = o;
   p.top_share = top_share;
   p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < 12;
@@ -344,6 +356,7 @@ static void plan_to_options(const Plan &p, tga_options &o) {
   o.chunk_rows = (int64_t)p.chunk_rows;
   o.est_groups = (int64_t)(p.gslots / 2);
   o.scatter_layout = p.scatter_layout;
+  o.write_combine = p.scatter_wc ? 1 : 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -427,7 +440,8 @@ static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 // aggregation (2^28 rows, 2^20 groups on MI355X: scatter 2.46 ms with 192
 // or 256 workgroups, aggregation 1.52 vs 1.62 ms).
 static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan, uint64_t m) {
-  const size_t lds = scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
+  const size_t lds = plan.scatter_wc ? scatter_wc_lds_bytes(q.key_bytes, plan.pbits, plan.tile_rows)
+                                     : scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
   const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / std::max<size_t>(lds, 1)));
   const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
   size_t max_grid = ctx->cus * per_cu;
@@ -479,7 +493,7 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
   pb.buckets = plan.scatter_layout == 2;
   uint64_t slots = m;
   if (pb.buckets) {
-    const uint64_t cap = bucket_cap(m, grid, plan.pbits, plan.tile_rows);
+    const uint64_t cap = bucket_cap(m, grid, plan.pbits, scatter_block(plan.tile_rows, plan.scatter_wc));
     if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
     slots = cap << kBucketShift;
     // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
@@ -517,7 +531,7 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     BucketBufs &bb = pb.bb;
     TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
       return launch_part_scatter(q.key_bytes, keys, s, m, plan.pbits, plan.tile_rows, pb.pkeys.p, po, &bb, nullptr,
-                                 nullptr, grid, st);
+                                 nullptr, grid, st, plan.scatter_wc);
     }));
     pb.ap.blist = bb.blist;
     TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st); }));

@@ -385,18 +385,19 @@ struct LdsBarrier {
   __device__ __forceinline__ void operator()() const { lds_barrier(); }
 };
 
-// Exclusive scan of n u32 values held in LDS, in place; returns the total.
-// scratch needs blockDim/64 + 1 words.  All threads must call it.  Bar: the
-// barrier used between the phases (LdsBarrier when global accesses must stay
-// in flight).
-template <typename Bar = SyncBarrier>
-__device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
+// Exclusive scan of n u32 values; returns the total.  Element i is get(i);
+// its exclusive prefix goes to put(i, prefix).  get(i) is called twice, both
+// times before put(i).  scratch needs blockDim/64 + 1 words.  All threads
+// must call it.  Bar: the barrier used between the phases (LdsBarrier when
+// global accesses must stay in flight).
+template <typename Bar, typename Get, typename Put>
+__device__ __forceinline__ uint32_t block_exclusive_scan_fn(uint32_t n, uint32_t *scratch, Get get, Put put) {
   const Bar bar{};
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t per = (n + nt - 1) / nt;
   const uint32_t b = tid * per, e = min(n, b + per);
   uint32_t local = 0;
-  for (uint32_t i = b; i < e; ++i) local += arr[i];
+  for (uint32_t i = b; i < e; ++i) local += get(i);
   // inclusive wave scan of the per-thread sums
   const int lane = (int)__lane_id(), wave = (int)(tid >> 6);
   uint32_t x = local;
@@ -414,10 +415,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint
   }
   bar();
   uint32_t run = scratch[wave] + x - local;
-  for (uint32_t i = b; i < e; ++i) { const uint32_t v = arr[i]; arr[i] = run; run += v; }
+  for (uint32_t i = b; i < e; ++i) { const uint32_t v = get(i); put(i, run); run += v; }
   const uint32_t total = scratch[nt >> 6];
   bar();
   return total;
+}
+
+// The same over n values held in LDS, in place.
+template <typename Bar = SyncBarrier>
+__device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
+  return block_exclusive_scan_fn<Bar>(n, scratch, [&](uint32_t i) { return arr[i]; },
+                                      [&](uint32_t i, uint32_t v) { arr[i] = v; });
 }
 
 // ---------------------------------------------------------------------------

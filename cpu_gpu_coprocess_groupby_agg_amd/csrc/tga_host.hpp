@@ -147,6 +147,7 @@ struct Plan {
   int max_probe = 0;
   int fold = 0;  // fold equal keys of a wave before the LDS atomics
   int scatter_layout = 1;  // 1 = partition-contiguous, 2 = per-workgroup buckets
+  int scatter_wc = 0;      // bucket layout of records written as whole cache lines
   // re-planning: the options and skew the plan came from, and whether the
   // fanout was the tuner's (so a too-small estimate may raise it)
   tga_options opt{};

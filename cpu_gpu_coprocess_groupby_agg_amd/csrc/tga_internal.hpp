@@ -57,15 +57,24 @@ struct BucketBufs {
 };
 // persistent workgroups of the scatter for ntiles tiles (at most max_grid)
 unsigned bucket_grid(uint32_t ntiles, unsigned max_grid);
+// workgroup size of the scatter (wc: the write-combining one)
+unsigned scatter_block(uint32_t tile_rows, int wc);
 // buckets the scatter can take: ceil(n / kBucketRows) + grid * (2^pbits +
 // its workgroup size)
-uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows);
+uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block);
+// LDS of the write-combining scatter, and its largest tile (4096, 6144 or
+// 8192 rows) whose LDS fits lds_max, 0 if none does
+size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows);
+uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max);
 // rows [0, n) into the partitioned copy okeys / o.cols: into buckets with
 // bb, else partition-contiguous at pbase[p] + offs[tile][p] (from
-// launch_part_count + launch_colscan over the same tiles)
+// launch_part_count + launch_colscan over the same tiles).  wc: buckets of
+// (key, value) records written as whole cache lines (tile_rows from
+// scatter_wc_tile)
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
                                uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs *bb,
-                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st);
+                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st,
+                               int wc = 0);
 // Partition-contiguous layout: per-tile histogram cnt[ntiles][P]; its
 // column-wise scan offs (same shape, exclusive per partition), total[P] and
 // pbase[P+1] (partition starts); desc: colscan_desc_words() zeroed words

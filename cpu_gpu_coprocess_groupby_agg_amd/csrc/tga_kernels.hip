@@ -374,7 +374,7 @@ __host__ __device__ inline uint32_t bucket_of(uint32_t k, uint32_t first, uint32
   return k == 0 ? first : delta + k;
 }
 
-// LDS and register state of one k_part_bucket workgroup.
+// LDS and register state of one k_part_scatter workgroup.
 template <typename K, int NC>
 struct ScatterState {
   const K *keys;
@@ -382,7 +382,8 @@ struct ScatterState {
   uint64_t n;
   uint32_t P, TR, cap;
   int pbits, ncol;
-  uint4 *wdesc;     // {q0, first, delta, -}: row j of p's run -> position j + q0 of its chain
+  uint4 *wdesc;     // {q0, first, delta, f}: row j of p's run -> position j + q0 of its chain;
+                    // f = rows the open bucket held before the run (bucket layout)
   uint32_t *hist;   // the tile's counts, then exclusive offsets
   uint32_t *hnext;  // the next tile's counts
   uint32_t *fill;   // rows in the open bucket (0: none open)
@@ -507,7 +508,7 @@ __device__ __forceinline__ void scatter_extend(ScatterState<K, NC> &S, uint32_t 
       S.cur[p] = bucket_of((end - 1) >> kBucketShift, first, delta);
     }
     if (on) {
-      S.wdesc[p] = make_uint4(f - off, first, delta, 0u);
+      S.wdesc[p] = make_uint4(f - off, first, delta, f);
       S.hnext[p] = 0;
     }
   }
@@ -611,11 +612,28 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
   S.o = o;
   S.rlo = S.rhi = 0;
 
-  // chunk of consecutive tiles; chunks renumbered so the chunks of one XCD
-  // are neighbours too (the grid is a multiple of the XCD count)
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-  uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
-  const uint32_t tend = min(ntiles, tile + per);
+  // Tiles of this workgroup: tile, tile + step, ... below tend.  Buckets: a
+  // chunk of consecutive tiles (its runs of one partition append to one
+  // bucket chain), chunks renumbered so the chunks of one XCD are neighbours.
+  // Partition-contiguous: XCD x (workgroups are dispatched round-robin over
+  // the XCDs; the grid is a multiple of their count) owns the tiles
+  // [x * ntiles / 8, (x + 1) * ntiles / 8), and its m workgroups take them in
+  // turn, so the tiles in flight on one XCD are adjacent and so are their
+  // runs of each partition: the partial lines where one run ends and the
+  // next begins are completed in that XCD's L2, and HBM sees each
+  // partition's rows of m tiles as one region.
+  uint32_t tile, tend, step;
+  if constexpr (BK) {
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
+    tend = min(ntiles, tile + per);
+    step = 1;
+  } else {
+    const uint32_t x = blockIdx.x % kXcds, m = gridDim.x / kXcds;
+    tile = (uint32_t)((uint64_t)ntiles * x / kXcds) + blockIdx.x / kXcds;
+    tend = (uint32_t)((uint64_t)ntiles * (x + 1) / kXcds);
+    step = m;
+  }
   if (tile >= tend) return;  // uniform per workgroup
   for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {
     S.fill[p] = 0;
@@ -632,7 +650,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
   // load is outstanding on set A where the loop starts (as after each tile)
   __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
   scatter_pin<K, ITEMS, NC, CellT>(ka, ca);
-  if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, tile + 1, kb, cb);  // clamped past the end
+  if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, tile + step, kb, cb);  // clamped past the end
 
   // One tile: X holds its ranked rows; with kEarly, Y is receiving the next
   // tile's rows, and X is refilled with the tile after that.
@@ -649,7 +667,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
       scatter_pin<K, ITEMS, NC, CellT>(Y, YC);                                                   \
     }                                                                                            \
     lds_barrier(); /* staged; the next histogram is zeroed */                                    \
-    const uint32_t next = tile + 1;                                                              \
+    const uint32_t next = tile + step;                                                           \
     if constexpr (!kEarly) {                                                                     \
       if (next < tend) scatter_load<K, ITEMS, NC, CellT>(S, next, Y, YC);                        \
     }                                                                                            \
@@ -657,7 +675,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
     if (next >= tend) break;                                                                     \
     /* the next tile, ranked into the other histogram while the stores drain */                  \
     scatter_rank<K, ITEMS, NC>(S, next, Y, S.hnext, pp, rk);                                     \
-    if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, next + 1, X, XC);                 \
+    if constexpr (kEarly) scatter_load<K, ITEMS, NC, CellT>(S, next + step, X, XC);              \
     uint32_t *t_ = S.hist;                                                                       \
     S.hist = S.hnext;                                                                            \
     S.hnext = t_;                                                                                \
@@ -680,6 +698,206 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
     for (uint32_t id = S.rlo + (uint32_t)__lane_id(); id < S.rhi; id += kWave)
       if (id < cap) bpart[id] = ~0u;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Write-combining scatter: bucket layout, (key, 4-byte value) records.
+//
+// A partition's run within one tile is a few hundred bytes, so most of its
+// cache lines are only partly written by the tile, and the L2 evicts them
+// (as masked partial writes) before the next tile completes them.  Here no
+// partial line leaves the workgroup until the end: each partition's
+// unfinished line, < 32 dwords, is kept in an LDS carry buffer, and a tile
+// writes only complete, aligned 128-byte lines.  A bucket (kBucketRows
+// records of RD dwords) is a whole number of lines, so a line never straddles
+// two buckets.
+//
+// The partition's record stream is viewed as dwords: the carry (the first cd
+// dwords of the line that holds the open bucket's fill point, cd =
+// fill * RD mod 32) followed by the tile's run of c records (c * RD dwords);
+// the tile writes its first (cd + c * RD) / 32 lines and carries the rest.
+// Per tile the workgroup goes through two barrier intervals:
+//   [scan counts and task counts, extend the chains, stage the records]
+//   [tasks: write the complete lines and move the new partial lines to the
+//    carry buffer; rank the next tile]
+// then the next tile; the carries left at the end are written as partial
+// lines.  Loads, stage and ranking are those of k_part_scatter.
+//
+// LDS: wdesc[P] uint4 | hist, hnext, fill, cur, loff [P] u32 |
+//      carry [P][32] u32 | stage [TR * RD] u32 | lpid [tasks] u16
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLineDw = 32;  // dwords per 128-byte cache line
+
+// tasks of one tile: complete lines plus at most one carry-only task per
+// partition
+__host__ __device__ inline uint32_t wc_tasks_max(uint32_t P, uint32_t TR, uint32_t RD) {
+  return (P * (kLineDw - 1) + TR * RD) / kLineDw + 1 + P;
+}
+
+size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows) {
+  const uint32_t P = 1u << pbits, RD = (uint32_t)(key_bytes + 4) / 4;
+  return (size_t)P * (16 + 5 * 4 + kLineDw * 4) + (size_t)tile_rows * RD * 4 +
+         (((size_t)wc_tasks_max(P, tile_rows, RD) * 2 + 15) & ~(size_t)15);
+}
+
+template <typename K, int ITEMS>
+__global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ keys, Schema s, uint64_t n,
+                                                          int pbits, uint32_t ntiles, void *orec, uint32_t *bpart,
+                                                          uint32_t *bcnt, uint32_t *pool, uint32_t cap) {
+  constexpr uint32_t RD = (uint32_t)Rec4<K>::kBytes / 4;     // dwords per record
+  constexpr uint32_t LB = kBucketRows * RD / kLineDw;       // lines per bucket
+  static_assert(kBucketRows * RD % kLineDw == 0, "buckets are whole lines");
+  extern __shared__ __align__(16) char smem[];
+  __shared__ uint32_t scratch[17];
+  ScatterState<K, 1> S;
+  S.keys = keys;
+  S.s = &s;
+  S.n = n;
+  S.P = 1u << pbits;
+  S.TR = blockDim.x * ITEMS;
+  S.cap = cap;
+  S.pbits = pbits;
+  S.ncol = 1;
+  S.colb[0] = 4;
+  S.coff[0] = 0;
+  S.wdesc = (uint4 *)smem;
+  S.hist = (uint32_t *)(S.wdesc + S.P);
+  S.hnext = S.hist + S.P;
+  S.fill = S.hnext + S.P;
+  S.cur = S.fill + S.P;
+  uint32_t *loff = S.cur + S.P;       // first task of each partition in the tile's task list
+  uint32_t *carry = loff + S.P;        // [P][kLineDw]
+  uint32_t *stage = carry + S.P * kLineDw;  // the tile's records, grouped by partition
+  uint16_t *lpid = (uint16_t *)(stage + S.TR * RD);  // partition of each task of the tile
+  S.skey = nullptr;
+  S.scol = nullptr;
+  S.spid = nullptr;
+  S.scratch = scratch;
+  S.bpart = bpart;
+  S.bcnt = bcnt;
+  S.pool = pool;
+  S.offs = nullptr;
+  S.pbase = nullptr;
+  S.okeys = nullptr;
+  S.rlo = S.rhi = 0;
+  uint32_t *out = (uint32_t *)orec;
+
+  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+  uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
+  const uint32_t tend = min(ntiles, tile + per);
+  if (tile >= tend) return;  // uniform per workgroup
+  for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {
+    S.fill[p] = 0;
+    S.cur[p] = 0;
+    S.hist[p] = 0;
+  }
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  K ka[ITEMS], kb[ITEMS];
+  uint32_t ca[ITEMS][1], cb[ITEMS][1];
+  uint32_t pp[ITEMS], rk[ITEMS];
+  scatter_load<K, ITEMS, 1, uint32_t>(S, tile, ka, ca);
+  lds_barrier();
+  scatter_rank<K, ITEMS, 1>(S, tile, ka, S.hist, pp, rk);
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+  scatter_pin<K, ITEMS, 1, uint32_t>(ka, ca);
+  scatter_load<K, ITEMS, 1, uint32_t>(S, tile + 1, kb, cb);  // clamped past the end
+
+#define TGA_WC_TILE(X, XC, Y, YC)                                                                \
+  {                                                                                              \
+    lds_barrier(); /* counts complete; the previous tile's tasks done */                        \
+    /* rows | tasks << 16 per partition, scanned together: a partition with */                  \
+    /* rows has max(1, complete lines) tasks; its task 0 also moves the new */                  \
+    /* partial line to the carry buffer */                                                      \
+    const uint32_t tot = block_exclusive_scan_fn<LdsBarrier>(                                    \
+        S.P, scratch,                                                                            \
+        [&](uint32_t p) {                                                                        \
+          const uint32_t c = S.hist[p];                                                          \
+          const uint32_t nl = (((S.fill[p] * RD) & (kLineDw - 1)) + c * RD) / kLineDw;           \
+          return c | (c ? max(nl, 1u) : 0u) << 16;                                               \
+        },                                                                                       \
+        [&](uint32_t p, uint32_t v) {                                                            \
+          S.hist[p] = v & 0xffffu;                                                               \
+          loff[p] = v >> 16;                                                                     \
+        });                                                                                      \
+    const uint32_t rows = tot & 0xffffu, tasks = tot >> 16;                                      \
+    scatter_extend<true, K, 1>(S, rows, tile);                                                   \
+    for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {                                   \
+      const uint32_t l0 = loff[p], l1 = p + 1 < S.P ? loff[p + 1] : tasks;                       \
+      for (uint32_t l = l0; l < l1; ++l) lpid[l] = (uint16_t)p;                                  \
+    }                                                                                            \
+    {                                                                                            \
+      const uint64_t b0 = (uint64_t)tile * S.TR;                                                 \
+      _Pragma("unroll") for (int i = 0; i < ITEMS; ++i) {                                        \
+        if (b0 + (uint64_t)i * blockDim.x + threadIdx.x < S.n) {                                 \
+          uint32_t *r = stage + (S.hist[pp[i]] + rk[i]) * RD;                                    \
+          r[0] = (uint32_t)(uint64_t)X[i];                                                       \
+          if constexpr (RD == 3) r[1] = (uint32_t)((uint64_t)X[i] >> 32);                        \
+          r[RD - 1] = XC[i][0];                                                                  \
+        }                                                                                        \
+      }                                                                                          \
+    }                                                                                            \
+    __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0): Y's loads */                     \
+    scatter_pin<K, ITEMS, 1, uint32_t>(Y, YC);                                                   \
+    lds_barrier(); /* staged; task list built */                                                 \
+    /* tasks: 8 lanes x 16 bytes per line, 8 tasks per wave step.  Each */                     \
+    /* lane of task 0 reads the old carry and writes the new one at the */                      \
+    /* same 4 positions, reads first; no other task reads the carry */                          \
+    for (uint32_t g0 = wave * 8; g0 < tasks; g0 += nw * 8) {                                     \
+      const uint32_t g = g0 + (lane >> 3), part = lane & 7;                                      \
+      if (g < tasks) {                                                                           \
+        const uint32_t p = lpid[g];                                                              \
+        const uint32_t l = g - loff[p];                                                          \
+        const uint4 d = S.wdesc[p];                                                              \
+        const uint32_t f = d.w, off = d.w - d.x;                                                 \
+        const uint32_t c = (p + 1 < S.P ? S.hist[p + 1] : rows) - S.hist[p];                     \
+        const uint32_t cd = (f * RD) & (kLineDw - 1), all = cd + c * RD;                         \
+        const uint32_t nl = all / kLineDw;                                                       \
+        if (l < nl) {                                                                            \
+          uint32_t v[4];                                                                         \
+          _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                        \
+            const uint32_t q = l * kLineDw + part * 4 + i;                                       \
+            v[i] = q < cd ? carry[p * kLineDw + q] : stage[off * RD + q - cd];                   \
+          }                                                                                      \
+          const uint32_t lg = (f * RD) / kLineDw + l;                                            \
+          const uint32_t id = bucket_of(lg / LB, d.y, d.z);                                      \
+          if (id < S.cap)                                                                        \
+            *(uint4 *)(out + (uint64_t)id * (kBucketRows * RD) + (lg % LB) * kLineDw + part * 4) = \
+                make_uint4(v[0], v[1], v[2], v[3]);                                              \
+        }                                                                                        \
+        if (l == 0) {                                                                            \
+          _Pragma("unroll") for (int i = 0; i < 4; ++i) {                                        \
+            const uint32_t j = part * 4 + i, q = nl * kLineDw + j;                               \
+            if (q >= cd && q < all) carry[p * kLineDw + j] = stage[off * RD + q - cd];           \
+          }                                                                                      \
+        }                                                                                        \
+      }                                                                                          \
+    }                                                                                            \
+    const uint32_t next = tile + 1;                                                              \
+    if (next >= tend) break;                                                                     \
+    scatter_rank<K, ITEMS, 1>(S, next, Y, S.hnext, pp, rk);                                      \
+    scatter_load<K, ITEMS, 1, uint32_t>(S, next + 1, X, XC);                                     \
+    uint32_t *t_ = S.hist;                                                                       \
+    S.hist = S.hnext;                                                                            \
+    S.hnext = t_;                                                                                \
+    tile = next;                                                                                 \
+  }
+  for (;;) {
+    TGA_WC_TILE(ka, ca, kb, cb)
+    TGA_WC_TILE(kb, cb, ka, ca)
+  }
+#undef TGA_WC_TILE
+  lds_barrier();  // the last carries are in place
+  // carries left: partial lines at the fill point of the open buckets
+  for (uint32_t e = threadIdx.x; e < S.P * kLineDw; e += blockDim.x) {
+    const uint32_t p = e / kLineDw, i = e % kLineDw;
+    const uint32_t f = S.fill[p], id = S.cur[p];
+    if (i < ((f * RD) & (kLineDw - 1)) && id < cap)
+      out[(uint64_t)id * (kBucketRows * RD) + ((f * RD) & ~(kLineDw - 1)) + i] = carry[e];
+  }
+  for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x)
+    if (S.fill[p] && S.cur[p] < cap) bcnt[S.cur[p]] = S.fill[p];
+  for (uint32_t id = S.rlo + (uint32_t)__lane_id(); id < S.rhi; id += kWave)
+    if (id < cap) bpart[id] = ~0u;
 }
 
 // Per-partition bucket and row totals from the bucket headers (pbk, prow
@@ -1097,16 +1315,50 @@ unsigned bucket_grid(uint32_t ntiles, unsigned max_grid) {
   return xcd_grid(g);
 }
 
-uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, uint32_t tile_rows) {
-  const uint64_t block = tile_rows / 8;  // the scatter's ITEMS
+unsigned scatter_block(uint32_t tile_rows, int wc) { return wc ? 1024u : tile_rows / 8; }
+
+uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block) {
   return (n + kBucketRows - 1) / kBucketRows + (uint64_t)grid * ((1ULL << pbits) + block) + 1;
+}
+
+uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max) {
+  for (uint32_t items : {8u, 6u, 4u}) {
+    const uint32_t tr = items * 1024;
+    if (scatter_wc_lds_bytes(key_bytes, pbits, tr) + 256 <= lds_max) return tr;
+  }
+  return 0;
+}
+
+template <typename K, int ITEMS>
+static hipError_t launch_scatter_wc_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                      uint32_t ntiles, void *orec, const BucketBufs &bb, unsigned grid,
+                                      hipStream_t st) {
+  const size_t lds = scatter_wc_lds_bytes(sizeof(K), pbits, tile_rows);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter_wc<K, ITEMS>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_part_scatter_wc<K, ITEMS>), dim3(grid), dim3(1024), lds, st, (const K *)keys, s, n, pbits,
+                     ntiles, orec, bb.bpart, bb.bcnt, bb.pool, bb.cap);
+  return hipGetLastError();
 }
 
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
                                uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs *bb,
-                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
+                               const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st, int wc) {
   const uint32_t ntiles = (uint32_t)((n + tile_rows - 1) / tile_rows);
   if (ntiles == 0) return hipSuccess;
+  if (wc) {  // records into buckets, complete lines only
+    if (!bb || !copy_as_records(s) || tile_rows % 1024 || pbits > 16) return hipErrorInvalidValue;
+#define TGA_WC_CASE(K)                                                                                  \
+  switch (tile_rows / 1024) {                                                                           \
+    case 4: return launch_scatter_wc_t<K, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    case 6: return launch_scatter_wc_t<K, 6>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    case 8: return launch_scatter_wc_t<K, 8>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    default: return hipErrorInvalidValue;                                                               \
+  }
+    if (key_bytes == 4) { TGA_WC_CASE(uint32_t) }
+    TGA_WC_CASE(uint64_t)
+#undef TGA_WC_CASE
+  }
   const bool one = s.ncols <= 1;
   const bool w8 = s.ncols >= 1 && dtype_bytes(s.col_dtype[0]) == 8;
   const bool rec = copy_as_records(s);

@@ -82,7 +82,10 @@ typedef struct tga_options {
   int32_t attempts;     /* tga_last_plan only: executions the last call needed (finer fanout / larger tables) */
   int32_t scatter_layout; /* partitioned copy: 0 = tuner, 1 = partition-contiguous (a histogram pass first),
                              2 = per-workgroup buckets (no histogram pass) */
-  int32_t reserved[5];
+  int32_t write_combine;  /* bucket layout with one 4-byte value column: 0 or 1 = keep each partition's
+                             unfinished cache line in LDS and write whole lines (when the LDS holds it),
+                             2 = never */
+  int32_t reserved[4];
 } tga_options;
 
 typedef struct tga_context tga_context;

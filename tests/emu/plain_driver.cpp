@@ -169,8 +169,9 @@ int run_case(tga_context *ctx, const Case &c) {
   }
   tga_options plan;
   tga_last_plan(ctx, &plan);
-  std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots) %s\n", c.name.c_str(), (long long)n,
-              (long long)g, plan.path, plan.partition_bits, plan.table_slots, bad ? "FAIL" : "ok");
+  std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots, tile %d, layout %d, wc %d) %s\n",
+              c.name.c_str(), (long long)n, (long long)g, plan.path, plan.partition_bits, plan.table_slots,
+              plan.tile_rows, plan.scatter_layout, plan.write_combine, bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }
 
@@ -251,6 +252,18 @@ int main(int argc, char **argv) {
     o2.scatter_layout = 1;
     cases.push_back({"contiguous_multi_col", 32, keys_u32(N, 5000, 53), {col_i32(N, -99, 99, 54), col_i32(N, -7, 7, 55)},
                      {TGA_I32, TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, o2});
+  }
+  // write-combining scatter at 2^9 partitions (the most its carry buffer
+  // allows): 32-bit keys (8-byte records, 8192-row tiles) and 64-bit keys
+  // (12-byte records, 6144-row tiles); the same without write combining
+  for (int wc : {1, 2}) {
+    tga_options o = opts(TGA_PATH_PARTITIONED, 2, 9, 0, 100000);
+    o.write_combine = wc;
+    const std::string w = wc == 1 ? "wc_" : "nowc_";
+    cases.push_back({w + "u32_p512", 32, keys_u32(3 * N, 100000, 61), {col_i32(3 * N, -(1 << 30), 1 << 30, 62)},
+                     {TGA_I32}, {a4[0], a4[1]}, o});
+    cases.push_back({w + "u64_p512", 64, keys_u64(3 * N + 77, 100000, 63), {col_i32(3 * N + 77, -99, 99, 64)},
+                     {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, o});
   }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});

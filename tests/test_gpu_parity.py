@@ -73,6 +73,27 @@ def test_partition_layouts(ctx, layout, ncols):
     assert got["n_groups"] == d
 
 
+@pytest.mark.parametrize("wc", [1, 2])
+@pytest.mark.parametrize("key_bits", [32, 64])
+def test_write_combining(ctx, wc, key_bits):
+    # bucket layout of (key, value) records with and without the LDS carry
+    # of unfinished cache lines, at 2^9 partitions (the most the carry
+    # buffer fits): 8-byte records in 8192-row tiles, 12-byte records in
+    # 6144-row tiles; a ragged last tile
+    n, d = (1 << 22) + 4097, 150_000
+    gen = tga.GEN_KEYS_U64 if key_bits == 64 else tga.GEN_KEYS_U32
+    k = ctx.generate(gen, n, a=d, seed=43, salt=3)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=44)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, partition_bits=9, write_combine=wc)
+    plan = ctx.last_plan()
+    assert plan["write_combine"] == wc and plan["scatter_layout"] == 2
+    if wc == 1:
+        assert plan["tile_rows"] == (8192 if key_bits == 32 else 6144)
+    ref = O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4)
+    check_equal(got, ref)
+    assert got["n_groups"] == d
+
+
 def test_partitioned_small_tables_many_partitions(ctx):
     n, d = 1 << 21, 200_000
     k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=15, salt=1)
