@@ -434,18 +434,20 @@ hipError_t gtable_alloc(tga_context *ctx, const QuerySpec &q, uint64_t S, GTable
 static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 
 // Persistent scatter grid: as many workgroups as the LDS lets run at once,
-// with at least 8 tiles each.  With buckets, three quarters of that: the
-// scattered writes saturate each XCD's write path with fewer workgroups,
-// and every workgroup leaves one partly filled bucket per partition for the
-// aggregation (2^28 rows, 2^20 groups on MI355X: scatter 2.46 ms with 192
-// or 256 workgroups, aggregation 1.52 vs 1.62 ms).
+// with at least 8 tiles each.  The plain bucket scatter uses three quarters
+// of that: its scattered writes saturate each XCD's write path with fewer
+// workgroups, and every workgroup leaves one partly filled bucket per
+// partition for the aggregation (2^28 rows, 2^20 groups on MI355X: scatter
+// 2.46 ms with 192 or 256 workgroups, aggregation 1.52 vs 1.62 ms).  The
+// write-combining scatter is bound by its LDS work, not by its writes, and
+// uses every CU (1.76 ms with 192 workgroups, 1.43 ms with 256).
 static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan, uint64_t m) {
   const size_t lds = plan.scatter_wc ? scatter_wc_lds_bytes(q.key_bytes, plan.pbits, plan.tile_rows)
                                      : scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
   const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / std::max<size_t>(lds, 1)));
   const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
   size_t max_grid = ctx->cus * per_cu;
-  if (plan.scatter_layout == 2) max_grid = std::max<size_t>(8, max_grid * 3 / 4);
+  if (plan.scatter_layout == 2 && !plan.scatter_wc) max_grid = std::max<size_t>(8, max_grid * 3 / 4);
   return bucket_grid(ntiles, (unsigned)max_grid);
 }
 

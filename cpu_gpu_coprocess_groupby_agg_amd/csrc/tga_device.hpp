@@ -421,6 +421,34 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_fn(uint32_t n, uint32_t
   return total;
 }
 
+// Exclusive scan of n <= 64 * PER values by one wave, with no barrier;
+// returns the total to every lane.  Lane l takes elements [l*per, l*per+per).
+template <int PER, typename Get, typename Put>
+__device__ __forceinline__ uint32_t wave_exclusive_scan_fn(uint32_t n, Get get, Put put) {
+  const uint32_t lane = __lane_id();
+  const uint32_t per = (n + kWave - 1) / kWave;
+  const uint32_t b = lane * per;
+  uint32_t v[PER], local = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    v[k] = (uint32_t)k < per && b + k < n ? get(b + k) : 0u;
+    local += v[k];
+  }
+  uint32_t x = local;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o, kWave);
+    if ((int)lane >= o) x += y;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)x, kWave - 1, kWave);
+  uint32_t run = x - local;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    if ((uint32_t)k < per && b + k < n) put(b + k, run);
+    run += v[k];
+  }
+  return total;
+}
+
 // The same over n values held in LDS, in place.
 template <typename Bar = SyncBarrier>
 __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint32_t n, uint32_t *scratch) {
@@ -442,12 +470,15 @@ struct BucketList {
   const uint64_t *ents;  // null: contiguous mode
 };
 
-// List entry of block i of wave `wave` (0 past the end; 0 in contiguous mode).
+// List entry of block i of wave `wave` (0 in contiguous mode).  Past the end
+// of the list it is the last entry: the load is unconditional, so the
+// register it fills is never also set by another path while it is in flight
+// (which would make the compiler wait for every load in flight first).
 __device__ __forceinline__ uint64_t list_entry(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
                                                uint32_t nw) {
-  if (!bl.ents) return 0;
+  if (!bl.ents || e <= b) return 0;
   const uint64_t k = b + wave + i * nw;
-  return k < e ? bl.ents[k] : 0;
+  return bl.ents[k < e ? k : e - 1];
 }
 
 // Block i of wave `wave` (wave-uniform), given its list entry: rows
@@ -455,7 +486,7 @@ __device__ __forceinline__ uint64_t list_entry(const BucketList &bl, uint64_t b,
 __device__ __forceinline__ bool wave_block(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
                                            uint32_t nw, uint64_t ent, uint64_t &bb, uint64_t &be) {
   if (bl.ents) {
-    if ((ent >> 32) == 0) return false;
+    if (b + wave + i * nw >= e || (ent >> 32) == 0) return false;
     bb = (ent & 0xffffffffULL) << kBucketShift;
     be = bb + (ent >> 32);
     return true;
@@ -686,10 +717,15 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 // The loop is software-pipelined: the next block's key and value loads are
 // issued before the current block's LDS work, so every wave keeps its HBM
 // reads in flight while it probes and updates the table.
+// Register type of a prefetched value cell: 4-byte cells stay 32-bit until
+// widen_block, so no 64-bit register pair is tied to a load in flight.
+template <int VW>
+using CellOf = typename std::conditional<(VW == 4 || VW == kVwRec4), uint32_t, uint64_t>::type;
+
 template <typename K, int NC, int VW, int U>
 __device__ __forceinline__ void load_block(const K *__restrict__ keys, const void *const *cols, uint32_t ncols,
                                            uint32_t coldt, uint64_t base, uint64_t e, int lane, K *kk,
-                                           uint64_t (*raw)[NC]) {
+                                           CellOf<VW> (*raw)[NC]) {
   // rows past the end re-read the block's last row (in bounds, no branch),
   // so all U key and value loads are in flight together
 #pragma unroll
@@ -700,13 +736,44 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
       static_assert(NC == 1, "records carry one value column");
       uint32_t v;
       Rec4<K>::load(keys, r, kk[u], v);
-      raw[u][0] = (coldt & 15u) == DT_I32 ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
+      raw[u][0] = v;
+    } else if constexpr (VW == 4) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? ((const uint32_t *)cols[c])[r] : 0u;
+      kk[u] = keys[r];
     } else {
       kk[u] = keys[r];
 #pragma unroll
       for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
     }
   }
+}
+
+// The cells as 64-bit operands: 4-byte cells are widened (sign-extended for
+// int32 columns) only here, where the row loop first uses them -- widening
+// at the load would make the wave wait for the prefetched block as soon as
+// it is issued.
+template <int NC, int VW, int U>
+__device__ __forceinline__ void widen_block(uint32_t coldt, const CellOf<VW> (*cell)[NC], uint64_t (*raw)[NC]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if constexpr (VW == 4 || VW == kVwRec4)
+        raw[u][c] = ((coldt >> (4 * c)) & 15u) == DT_I32 ? (uint64_t)(int64_t)(int32_t)cell[u][c] : (uint64_t)cell[u][c];
+      else
+        raw[u][c] = cell[u][c];
+    }
+}
+
+// Volatile 4-byte read of an LDS word through a generic pointer to it, as an
+// LDS access (a generic one would also wait for every global load in flight).
+__device__ __forceinline__ uint32_t lds_read_volatile(const uint32_t *p) {
+#if defined(__AMDGCN__)
+  return *(const volatile __attribute__((address_space(3))) uint32_t *)p;
+#else
+  return *(const volatile uint32_t *)p;
+#endif
 }
 
 // Accumulator program of the row loop.  SIG == 0: decoded from the AccProg
@@ -766,16 +833,25 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   bool have = wave_block(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, e);
   uint64_t ent_next = list_entry(bl, b0, e0, 1, wave, nw);
   K kk[U];
-  uint64_t raw[U][NC];
-  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, e, lane, kk, raw);
+  CellOf<VW> cel[U][NC];
+  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, e, lane, kk, cel);
+  // The first block is waited for here, so the loop is entered with no load
+  // in flight, as it is re-entered: otherwise the compiler, merging the two
+  // states, waits inside every iteration for the loads it has just issued
+  // (the counter retires loads in order).
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+  // the give-up word is read in one iteration and tested in the next, so the
+  // wave never waits for the read
+  uint32_t stop = 0;
   for (uint64_t it = 1; have; ++it) {
-    if (giveup && wave_uniform(*(volatile uint32_t *)giveup)) break;
+    if (giveup && wave_uniform(stop)) break;
+    if (giveup) stop = lds_read_volatile(giveup);
     consumed += kWave * U;
     uint64_t nbase = 0, ne = 0;
     const bool next = wave_block(bl, b0, e0, it, wave, nw, ent_next, nbase, ne);
     K nkk[U];
-    uint64_t nraw[U][NC];
-    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ne, lane, nkk, nraw);
+    CellOf<VW> ncel[U][NC];
+    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ne, lane, nkk, ncel);
     ent_next = list_entry(bl, b0, e0, it + 1, wave, nw);
     uint32_t hh[U];
     int slot[U];
@@ -800,6 +876,8 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
       slot[u] = !act ? -1 : empty_key ? (int)S : m;
     }
     if (sent) *t.sentinel = 1;
+    uint64_t raw[U][NC];
+    widen_block<NC, VW, U>(coldt, cel, raw);
     // keys past their first bucket, and first sightings: probe on / insert
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -921,7 +999,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
       for (int u = 0; u < U; ++u) {
         kk[u] = nkk[u];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) raw[u][c] = nraw[u][c];
+        for (int c = 0; c < NC; ++c) cel[u][c] = ncel[u][c];
       }
     }
   }

@@ -382,8 +382,9 @@ struct ScatterState {
   uint64_t n;
   uint32_t P, TR, cap;
   int pbits, ncol;
-  uint4 *wdesc;     // {q0, first, delta, f}: row j of p's run -> position j + q0 of its chain;
-                    // f = rows the open bucket held before the run (bucket layout)
+  uint4 *wdesc;     // {q0, first, delta, f | c << 16}: row j of p's run -> position j + q0 of
+                    // its chain; f = rows the open bucket held before the run, c = the
+                    // run's rows (bucket layout)
   uint32_t *hist;   // the tile's counts, then exclusive offsets
   uint32_t *hnext;  // the next tile's counts
   uint32_t *fill;   // rows in the open bucket (0: none open)
@@ -508,7 +509,7 @@ __device__ __forceinline__ void scatter_extend(ScatterState<K, NC> &S, uint32_t 
       S.cur[p] = bucket_of((end - 1) >> kBucketShift, first, delta);
     }
     if (on) {
-      S.wdesc[p] = make_uint4(f - off, first, delta, f);
+      S.wdesc[p] = make_uint4(f - off, first, delta, f | c << 16);
       S.hnext[p] = 0;
     }
   }
@@ -727,6 +728,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
 //      carry [P][32] u32 | stage [TR * RD] u32 | lpid [tasks] u16
 // ---------------------------------------------------------------------------
 constexpr uint32_t kLineDw = 32;  // dwords per 128-byte cache line
+constexpr uint32_t kWcMaxP = 512;  // the largest fanout whose carry buffer fits
 
 // tasks of one tile: complete lines plus at most one carry-only task per
 // partition
@@ -808,17 +810,23 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
     /* rows | tasks << 16 per partition, scanned together: a partition with */                  \
     /* rows has max(1, complete lines) tasks; its task 0 also moves the new */                  \
     /* partial line to the carry buffer */                                                      \
-    const uint32_t tot = block_exclusive_scan_fn<LdsBarrier>(                                    \
-        S.P, scratch,                                                                            \
-        [&](uint32_t p) {                                                                        \
-          const uint32_t c = S.hist[p];                                                          \
-          const uint32_t nl = (((S.fill[p] * RD) & (kLineDw - 1)) + c * RD) / kLineDw;           \
-          return c | (c ? max(nl, 1u) : 0u) << 16;                                               \
-        },                                                                                       \
-        [&](uint32_t p, uint32_t v) {                                                            \
-          S.hist[p] = v & 0xffffu;                                                               \
-          loff[p] = v >> 16;                                                                     \
-        });                                                                                      \
+    /* (by wave 0 alone: no barrier inside) */                                                  \
+    if (wave == 0) {                                                                             \
+      const uint32_t t_ = wave_exclusive_scan_fn<kWcMaxP / kWave>(                                \
+          S.P,                                                                                   \
+          [&](uint32_t p) {                                                                      \
+            const uint32_t c = S.hist[p];                                                        \
+            const uint32_t nl = (((S.fill[p] * RD) & (kLineDw - 1)) + c * RD) / kLineDw;         \
+            return c | (c ? max(nl, 1u) : 0u) << 16;                                             \
+          },                                                                                     \
+          [&](uint32_t p, uint32_t v) {                                                          \
+            S.hist[p] = v & 0xffffu;                                                             \
+            loff[p] = v >> 16;                                                                   \
+          });                                                                                    \
+      if (lane == 0) scratch[0] = t_;                                                            \
+    }                                                                                            \
+    lds_barrier();                                                                               \
+    const uint32_t tot = scratch[0];                                                             \
     const uint32_t rows = tot & 0xffffu, tasks = tot >> 16;                                      \
     scatter_extend<true, K, 1>(S, rows, tile);                                                   \
     for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {                                   \
@@ -848,8 +856,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
         const uint32_t p = lpid[g];                                                              \
         const uint32_t l = g - loff[p];                                                          \
         const uint4 d = S.wdesc[p];                                                              \
-        const uint32_t f = d.w, off = d.w - d.x;                                                 \
-        const uint32_t c = (p + 1 < S.P ? S.hist[p + 1] : rows) - S.hist[p];                     \
+        const uint32_t f = d.w & 0xffffu, c = d.w >> 16, off = f - d.x;                          \
         const uint32_t cd = (f * RD) & (kLineDw - 1), all = cd + c * RD;                         \
         const uint32_t nl = all / kLineDw;                                                       \
         if (l < nl) {                                                                            \
@@ -1322,6 +1329,7 @@ uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block) {
 }
 
 uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max) {
+  if ((1u << pbits) > kWcMaxP) return 0;
   for (uint32_t items : {8u, 6u, 4u}) {
     const uint32_t tr = items * 1024;
     if (scatter_wc_lds_bytes(key_bytes, pbits, tr) + 256 <= lds_max) return tr;
@@ -1347,7 +1355,7 @@ hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s,
   const uint32_t ntiles = (uint32_t)((n + tile_rows - 1) / tile_rows);
   if (ntiles == 0) return hipSuccess;
   if (wc) {  // records into buckets, complete lines only
-    if (!bb || !copy_as_records(s) || tile_rows % 1024 || pbits > 16) return hipErrorInvalidValue;
+    if (!bb || !copy_as_records(s) || tile_rows % 1024 || (1u << pbits) > kWcMaxP) return hipErrorInvalidValue;
 #define TGA_WC_CASE(K)                                                                                  \
   switch (tile_rows / 1024) {                                                                           \
     case 4: return launch_scatter_wc_t<K, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
