@@ -878,7 +878,32 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     if (sent) *t.sentinel = 1;
     uint64_t raw[U][NC];
     widen_block<NC, VW, U>(coldt, cel, raw);
-    // keys past their first bucket, and first sightings: probe on / insert
+    // Rows whose key is not in its first bucket read the next one here.  A
+    // key placed past its first bucket (about 1% of the keys at the fill the
+    // tuner plans for) is nearly always in the next, and the probe loop below
+    // -- which some lane would enter in almost every block otherwise -- is
+    // left to first sightings and keys further on.  A key found is final; a
+    // miss goes on to the probe loop, which starts over from the first
+    // bucket.
+    {
+      bool far = false;
+#pragma unroll
+      for (int u = 0; u < U; ++u) far |= base + (uint64_t)u * kWave + lane < e && slot[u] < 0;
+      if (__ballot(far)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (base + (uint64_t)u * kWave + lane < e && slot[u] < 0) {
+            const uint32_t s1 = s0[u] + 4 == S ? 0u : s0[u] + 4;
+            lds_bucket<K>(tk, s1, k4[u]);
+            int m = -1;
+#pragma unroll
+            for (int j = 3; j >= 0; --j) m = k4[u][j] == kk[u] ? (int)(s1 + j) : m;
+            slot[u] = m;
+          }
+        }
+      }
+    }
+    // first sightings, and keys past their second bucket: probe on / insert
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint64_t row = base + (uint64_t)u * kWave + lane;
