@@ -635,12 +635,14 @@ tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, c
 // item holds whole write final rows straight to the output; only split
 // partitions and LDS spills go through the global table, sized for them.
 // One partitioned execution.  *overflow: the global table or the output was
-// too small (retry with larger ones).  *replan_est > 0: partitions processed
+// too small (retry with larger ones; *reduced: the table had fewer than
+// plan.gslots slots, so a full-size one may do).  *replan_est > 0: partitions processed
 // whole overflowed their LDS table and were abandoned; the value estimates
 // the group count for a finer partitioning (retry with it).
 static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, Plan &plan,
                                   bool full_table, PlainOut *out, void *ext_keys, void *const *ext_cols,
-                                  uint64_t ext_cap, uint64_t *groups, bool *overflow, uint64_t *replan_est) {
+                                  uint64_t ext_cap, uint64_t *groups, bool *overflow, uint64_t *replan_est,
+                                  bool *reduced) {
   *replan_est = 0;
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
@@ -660,21 +662,24 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
     if (e) return e;
   }
   const AggPlan &ap = pb.ap;
+  // Global table: split partitions and spills only.  A partition is split
+  // past chunk_rows, twice its expected rows, which takes a key holding more
+  // than half a partition's share of the rows.  Unless the key sample shows
+  // one, the table is sized without waiting for the partitioning (the GPU
+  // would idle meanwhile); one that turns out too small raises the overflow
+  // flag, and the call is retried with a full-size table.
   uint64_t split_rows = 0;
-  uint32_t used = 0;
-  TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
-  if (pb.buckets) TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
-  TGA_HIP(ctx, hipStreamSynchronize(st));
-  // the scatter never takes more buckets than bucket_cap(); a larger count
-  // would mean dropped rows
-  if (pb.buckets && used > pb.bb.cap)
-    return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
-  // global table: split partitions and spills only
+  if (!full_table && plan.top_share * (double)P >= 0.5) {
+    TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipStreamSynchronize(st));
+  }
   uint64_t S = plan.gslots;
+  *reduced = false;
   if (!full_table) {
     uint64_t want = 4096;
     while (want < 2 * split_rows && want < S) want <<= 1;
     S = std::min(S, want);
+    *reduced = S < plan.gslots;
   }
   GTable g{};
   std::vector<DBuf> gbufs;
@@ -716,12 +721,17 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   }));
   uint32_t hflags[kFlagWords];
   uint64_t hcount = 0;
-  uint32_t novf = 0;
+  uint32_t novf = 0, used = 0;
   TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof hflags, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipMemcpyAsync(&hcount, count, 8, hipMemcpyDeviceToHost, st));
   if (may_abandon) TGA_HIP(ctx, hipMemcpyAsync(&novf, ovf.p, 4, hipMemcpyDeviceToHost, st));
+  if (pb.buckets) TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
   if (ctx->prof.on) ctx->prof.resolve();
+  // the scatter never takes more buckets than bucket_cap(); a larger count
+  // would mean dropped rows
+  if (pb.buckets && used > pb.bb.cap)
+    return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
   if (novf) {
     // Abandoned work items: a table of T slots was full after r of the
     // item's R rows, so the item holds about T * R / r groups; the finished
@@ -768,10 +778,10 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
   if (plan.path == TGA_PATH_PARTITIONED && n > 0 && n < (1ULL << 31)) {
     bool grown = false;
     for (int attempt = 0; attempt < 8; ++attempt) {
-      bool overflow = false;
+      bool overflow = false, reduced = false;
       uint64_t replan = 0;
       tga_status st2 = run_partitioned(ctx, q, d_keys, n, plan, grown, out, ext_keys, ext_cols, ext_cap, groups,
-                                       &overflow, &replan);
+                                       &overflow, &replan, &reduced);
       if (st2) return st2;
       if (replan) {
         // the estimate was too low for the fanout: plan again from the
@@ -796,7 +806,9 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
         ctx->last_plan.attempts = attempt + 1;
         return TGA_OK;
       }
-      plan.gslots *= 4;  // the estimate was too low: larger tables, run again
+      // a reduced split/spill table that overflowed: the full-size one first;
+      // otherwise the estimate was too low: larger tables, run again
+      if (!(reduced && !grown)) plan.gslots *= 4;
       grown = true;
     }
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table kept overflowing");

@@ -169,9 +169,10 @@ int run_case(tga_context *ctx, const Case &c) {
   }
   tga_options plan;
   tga_last_plan(ctx, &plan);
-  std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots, tile %d, layout %d, wc %d) %s\n",
+  std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots, tile %d, layout %d, wc %d, "
+              "attempts %d) %s\n",
               c.name.c_str(), (long long)n, (long long)g, plan.path, plan.partition_bits, plan.table_slots,
-              plan.tile_rows, plan.scatter_layout, plan.write_combine, bad ? "FAIL" : "ok");
+              plan.tile_rows, plan.scatter_layout, plan.write_combine, plan.attempts, bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }
 
@@ -264,6 +265,17 @@ int main(int argc, char **argv) {
                      {TGA_I32}, {a4[0], a4[1]}, o});
     cases.push_back({w + "u64_p512", 64, keys_u64(3 * N + 77, 100000, 63), {col_i32(3 * N + 77, -99, 99, 64)},
                      {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, o});
+  }
+  {
+    // 40% of the rows on each of two keys (in different partitions) and a
+    // group-count hint (so no key sample): both partitions are split into
+    // chunks, the global table sized for no split overflows with their
+    // groups, and the call is run again with a full-size table
+    std::vector<uint64_t> k = keys_u64(N, 20000, 71);
+    for (int64_t i = 0; i < N; ++i)
+      if (i % 5 < 4) k[i] = i % 5 < 2 ? 5 : 9;
+    cases.push_back({"split_hint_retry", 64, k, {col_i32(N, -99, 99, 72)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
+                     opts(TGA_PATH_PARTITIONED, 2, 0, 0, 20001)});
   }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});

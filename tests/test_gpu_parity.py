@@ -94,6 +94,24 @@ def test_write_combining(ctx, wc, key_bits):
     assert got["n_groups"] == d
 
 
+def test_split_partitions_without_key_sample(ctx):
+    # 40% of the rows on each of two keys (partitions 0 and 4 of 8) and a
+    # group-count hint, so no key sample is taken: both partitions are split
+    # into chunks, the global table sized for no split overflows with their
+    # groups, and the call runs again with a full-size table
+    n = 1 << 20
+    rng = np.random.default_rng(81)
+    k = rng.integers(0, 1 << 62, size=20000, dtype=np.uint64)[rng.integers(0, 20000, size=n)]
+    i = np.arange(n)
+    k[i % 5 < 2] = 5
+    k[(i % 5 >= 2) & (i % 5 < 4)] = 9
+    v = rng.integers(-99, 100, size=n).astype(np.int32)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, est_groups=20002)
+    plan = ctx.last_plan()
+    assert plan["partition_bits"] == 3 and plan["attempts"] == 2
+    check_equal(got, O.groupby(k, [v], AGGS4))
+
+
 def test_partitioned_small_tables_many_partitions(ctx):
     n, d = 1 << 21, 200_000
     k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=15, salt=1)
