@@ -375,6 +375,12 @@ uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_s
     ++d;
     i = j;
   }
+  return estimate_from_stats(d, f1, top, m, n, top_share);
+}
+
+uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n, double *top_share) {
+  if (top_share) *top_share = 0.0;
+  if (m == 0 || n == 0) return 1;
   if (top_share) *top_share = (double)top / (double)m;
   if (m >= n) return d;  // the sample is the whole input
   // GEE (Charikar et al. 2000): sqrt(n/m) f1 + (d - f1); good under skew
@@ -400,14 +406,15 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
                            double *top_share) {
   if (top_share) *top_share = 0.0;
   if (n == 0) { *est = 1; return TGA_OK; }
+  // the sample is sorted and counted on the GPU; only three counts come back
   const uint64_t m = std::min<uint64_t>(n, 16384);
   DBuf buf;
-  if (!buf.alloc(ctx, m * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
-  TGA_HIP(ctx, launch_sample_keys(key_bytes, d_keys, n, m, buf.as<uint64_t>(), ctx->stream));
-  std::vector<uint64_t> h(m);
-  TGA_HIP(ctx, hipMemcpyAsync(h.data(), buf.p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (!buf.alloc(ctx, 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
+  TGA_HIP(ctx, launch_sample_stats(key_bytes, d_keys, n, (uint32_t)m, buf.as<uint64_t>(), ctx->stream));
+  uint64_t h[3];
+  TGA_HIP(ctx, hipMemcpyAsync(h, buf.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *est = estimate_from_sample(h.data(), m, n, top_share);
+  *est = estimate_from_stats(h[0], h[1], h[2], m, n, top_share);
   return TGA_OK;
 }
 

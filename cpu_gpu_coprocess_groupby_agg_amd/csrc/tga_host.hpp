@@ -167,6 +167,10 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
 // Distinct-count estimate of n rows from a sample of m keys (sorted in place).
 // *top_share (optional): the largest share of the sample one key takes.
 uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_share = nullptr);
+// The same from the sample's counts: d distinct keys, f1 of them seen once,
+// the most frequent seen top times.
+uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n,
+                             double *top_share = nullptr);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

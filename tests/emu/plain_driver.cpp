@@ -15,6 +15,11 @@
 #include <vector>
 
 #include "../../include/tga_groupby.h"
+#include "hip/hip_runtime.h"
+
+namespace tga {
+hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *out, hipStream_t st);
+}
 
 extern "C" int64_t tga_o_groupby(const void *keys, int key_bytes, int64_t n, const void *const *cols,
                                  const int *col_dtypes, int ncols, const int *agg_ops, const int *agg_cols, int naggs,
@@ -176,6 +181,55 @@ int run_case(tga_context *ctx, const Case &c) {
   return bad ? 1 : 0;
 }
 
+// The sample statistics kernel (bitonic sort in LDS, runs of equal keys)
+// against a sort and count on the host: uniform, skewed and single-key keys,
+// sample lengths that are and are not powers of two, keys equal to the
+// padding value.
+int check_sample_stats() {
+  int bad = 0;
+  struct S { const char *name; std::vector<uint64_t> k; uint32_t m; };
+  std::vector<S> sets;
+  sets.push_back({"uniform", keys_u64(100000, 30000, 81), 16384});
+  sets.push_back({"uniform_ragged", keys_u64(9001, 5000, 82), 9001});
+  {
+    std::vector<uint64_t> k = keys_u64(50000, 20000, 83);
+    for (size_t i = 0; i < k.size(); i += 3) k[i] = 7;
+    for (size_t i = 1; i < k.size(); i += 11) k[i] = ~0ULL;
+    sets.push_back({"skew_with_max_key", k, 12345});
+  }
+  sets.push_back({"single_key", std::vector<uint64_t>(20000, 42), 16384});
+  sets.push_back({"tiny", keys_u64(3, 3, 84), 3});
+  for (const S &c : sets) {
+    const uint64_t n = c.k.size();
+    std::vector<uint64_t> smp(c.m);
+    for (uint32_t i = 0; i < c.m; ++i) smp[i] = c.k[(uint64_t)(((unsigned __int128)i * n) / c.m)];
+    std::sort(smp.begin(), smp.end());
+    uint64_t d = 0, f1 = 0, top = 0;
+    for (uint64_t i = 0; i < c.m;) {
+      uint64_t j = i;
+      while (j < c.m && smp[j] == smp[i]) ++j;
+      ++d;
+      f1 += j - i == 1;
+      top = std::max<uint64_t>(top, j - i);
+      i = j;
+    }
+    uint64_t *dk = nullptr, *dout = nullptr, got[3] = {0, 0, 0};
+    hipMalloc(&dk, n * 8);
+    hipMalloc(&dout, 64);
+    hipMemcpy(dk, c.k.data(), n * 8, hipMemcpyHostToDevice);
+    const hipError_t e = tga::launch_sample_stats(8, dk, n, c.m, dout, nullptr);
+    hipMemcpy(got, dout, sizeof got, hipMemcpyDeviceToHost);
+    hipFree(dk);
+    hipFree(dout);
+    const bool ok = e == hipSuccess && got[0] == d && got[1] == f1 && got[2] == top;
+    std::printf("sample_stats_%s: m %u: distinct %llu/%llu, once %llu/%llu, top %llu/%llu %s\n", c.name, c.m,
+                (unsigned long long)got[0], (unsigned long long)d, (unsigned long long)got[1],
+                (unsigned long long)f1, (unsigned long long)got[2], (unsigned long long)top, ok ? "ok" : "FAIL");
+    bad += !ok;
+  }
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -287,6 +341,10 @@ int main(int argc, char **argv) {
     ++ran;
   }
   tga_context_destroy(ctx);
+  if (filter.empty() || std::string("sample_stats").find(filter) != std::string::npos) {
+    bad += check_sample_stats();
+    ++ran;
+  }
   std::printf(bad ? "FAIL %d of %d\n" : "OK %d\n", bad ? bad : ran, ran);
   return bad ? 1 : 0;
 }
