@@ -17,6 +17,9 @@ constexpr uint64_t kMaxGlobalProbe = 4096;
 // still parked at the exit (e.g. lane 0 before it fetches the next ticket).
 // A scalar exit condition keeps every lane of the wave on one path.
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t wave_uniform64(uint64_t v) {
+  return (uint64_t)wave_uniform((uint32_t)v) | (uint64_t)wave_uniform((uint32_t)(v >> 32)) << 32;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int l = (int)__lane_id();
@@ -496,6 +499,24 @@ __device__ __forceinline__ bool wave_block(const BucketList &bl, uint64_t b, uin
   return bb < e;
 }
 
+// The same as a wave-uniform first row and row count (1..kBucketRows), so
+// the row loop addresses a block from a scalar base with 32-bit lane
+// offsets; false past the end.
+__device__ __forceinline__ bool wave_block_n(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
+                                             uint32_t nw, uint64_t ent, uint64_t &bb, uint32_t &cnt) {
+  if (bl.ents) {
+    ent = wave_uniform64(ent);
+    if (b + wave + i * nw >= e || (ent >> 32) == 0) return false;
+    bb = (ent & 0xffffffffULL) << kBucketShift;
+    cnt = (uint32_t)(ent >> 32);
+    return true;
+  }
+  bb = b + (wave + i * nw) * (uint64_t)kBucketRows;
+  if (bb >= e) return false;
+  cnt = (uint32_t)min<uint64_t>(kBucketRows, e - bb);
+  return true;
+}
+
 // Aggregates the item's rows (see wave_block) into the LDS table.  Rows whose
 // key finds no free slot within max_probe go straight to the global table.
 // FX: the schema has exact fp64 SUM accumulators (their limb vectors cost
@@ -631,6 +652,19 @@ struct Rec12 {
 template <typename K>
 struct Rec4 {
   static constexpr int kBytes = (int)sizeof(K) + 4;
+  // record at byte offset `off` (32-bit) of `base`
+  __device__ __forceinline__ static void load_at(const void *base, uint32_t off, K &key, uint32_t &val) {
+    const char *p = (const char *)base + off;
+    if constexpr (sizeof(K) == 8) {
+      const Rec12 v = *(const Rec12 *)p;
+      key = (K)(((uint64_t)v.hi << 32) | v.lo);
+      val = v.val;
+    } else {
+      const uint2 v = *(const uint2 *)p;
+      key = (K)v.x;
+      val = v.y;
+    }
+  }
   __device__ __forceinline__ static void load(const void *base, uint64_t r, K &key, uint32_t &val) {
     if constexpr (sizeof(K) == 8) {
       const Rec12 v = ((const Rec12 *)base)[r];
@@ -722,20 +756,23 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 template <int VW>
 using CellOf = typename std::conditional<(VW == 4 || VW == kVwRec4), uint32_t, uint64_t>::type;
 
+// Rows [base, base + cnt) of a block (base and cnt wave-uniform, cnt >= 1):
+// lane l takes rows u * 64 + l.
 template <typename K, int NC, int VW, int U>
 __device__ __forceinline__ void load_block(const K *__restrict__ keys, const void *const *cols, uint32_t ncols,
-                                           uint32_t coldt, uint64_t base, uint64_t e, int lane, K *kk,
+                                           uint32_t coldt, uint64_t base, uint32_t cnt, int lane, K *kk,
                                            CellOf<VW> (*raw)[NC]) {
   // rows past the end re-read the block's last row (in bounds, no branch),
   // so all U key and value loads are in flight together
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const uint64_t row = base + (uint64_t)u * kWave + lane;
-    const uint64_t r = row < e ? row : e - 1;
+    const uint32_t j = min((uint32_t)(u * kWave + lane), cnt - 1);
+    const uint64_t r = base + j;
     if constexpr (VW == kVwRec4) {  // records (key, 4-byte value) at `keys`
       static_assert(NC == 1, "records carry one value column");
       uint32_t v;
-      Rec4<K>::load(keys, r, kk[u], v);
+      // a scalar block base plus a 32-bit lane offset
+      Rec4<K>::load_at((const char *)keys + base * (uint64_t)Rec4<K>::kBytes, j * (uint32_t)Rec4<K>::kBytes, kk[u], v);
       raw[u][0] = v;
     } else if constexpr (VW == 4) {
 #pragma unroll
@@ -829,12 +866,13 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   uint32_t consumed = 0;
   // bucket mode: the list entry of a block is read two blocks ahead, so no
   // wave waits on it before issuing that block's row loads
-  uint64_t base = 0, e = 0;
-  bool have = wave_block(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, e);
+  uint64_t base = 0;
+  uint32_t cnt = 0;
+  bool have = wave_block_n(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, cnt);
   uint64_t ent_next = list_entry(bl, b0, e0, 1, wave, nw);
   K kk[U];
   CellOf<VW> cel[U][NC];
-  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, e, lane, kk, cel);
+  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel);
   // The first block is waited for here, so the loop is entered with no load
   // in flight, as it is re-entered: otherwise the compiler, merging the two
   // states, waits inside every iteration for the loads it has just issued
@@ -847,11 +885,12 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     if (giveup && wave_uniform(stop)) break;
     if (giveup) stop = lds_read_volatile(giveup);
     consumed += kWave * U;
-    uint64_t nbase = 0, ne = 0;
-    const bool next = wave_block(bl, b0, e0, it, wave, nw, ent_next, nbase, ne);
+    uint64_t nbase = 0;
+    uint32_t ncnt = 0;
+    const bool next = wave_block_n(bl, b0, e0, it, wave, nw, ent_next, nbase, ncnt);
     K nkk[U];
     CellOf<VW> ncel[U][NC];
-    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ne, lane, nkk, ncel);
+    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel);
     ent_next = list_entry(bl, b0, e0, it + 1, wave, nw);
     uint32_t hh[U];
     int slot[U];
@@ -867,7 +906,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     bool sent = false;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool act = base + (uint64_t)u * kWave + lane < e;
+      const bool act = (uint32_t)(u * kWave + lane) < cnt;
       int m = -1;
 #pragma unroll
       for (int j = 3; j >= 0; --j) m = k4[u][j] == kk[u] ? (int)(s0[u] + j) : m;
@@ -888,11 +927,11 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     {
       bool far = false;
 #pragma unroll
-      for (int u = 0; u < U; ++u) far |= base + (uint64_t)u * kWave + lane < e && slot[u] < 0;
+      for (int u = 0; u < U; ++u) far |= (uint32_t)(u * kWave + lane) < cnt && slot[u] < 0;
       if (__ballot(far)) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if (base + (uint64_t)u * kWave + lane < e && slot[u] < 0) {
+          if ((uint32_t)(u * kWave + lane) < cnt && slot[u] < 0) {
             const uint32_t s1 = s0[u] + 4 == S ? 0u : s0[u] + 4;
             lds_bucket<K>(tk, s1, k4[u]);
             int m = -1;
@@ -905,14 +944,12 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     }
     // first sightings, and keys past their second bucket: probe on / insert
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t row = base + (uint64_t)u * kWave + lane;
-      if (row < e && slot[u] < 0) slot[u] = lds_find_or_insert(t, kk[u], hh[u], max_probe);
-    }
+    for (int u = 0; u < U; ++u)
+      if ((uint32_t)(u * kWave + lane) < cnt && slot[u] < 0) slot[u] = lds_find_or_insert(t, kk[u], hh[u], max_probe);
     // rows without an LDS slot (table full): the global table
     bool spill = false;
 #pragma unroll
-    for (int u = 0; u < U; ++u) spill |= (base + (uint64_t)u * kWave + lane < e) && slot[u] < 0;
+    for (int u = 0; u < U; ++u) spill |= (uint32_t)(u * kWave + lane) < cnt && slot[u] < 0;
     if (__ballot(spill)) {
       if (giveup) {
         if (spill) *giveup = 1;
@@ -920,7 +957,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
         if (spilled && spill) *spilled = 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          if ((base + (uint64_t)u * kWave + lane < e) && slot[u] < 0)
+          if ((uint32_t)(u * kWave + lane) < cnt && slot[u] < 0)
             spill_row<K>(gsp, ssp, kk[u], raw[u][0], NC > 1 ? raw[u][NC > 1 ? 1 : 0] : 0,
                          NC > 2 ? raw[u][NC > 2 ? 2 : 0] : 0, NC > 3 ? raw[u][NC > 3 ? 3 : 0] : 0);
         }
@@ -1019,7 +1056,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     have = next;
     if (next) {
       base = nbase;
-      e = ne;
+      cnt = ncnt;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         kk[u] = nkk[u];
