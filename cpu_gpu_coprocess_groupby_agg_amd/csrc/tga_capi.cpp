@@ -406,15 +406,25 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
                            double *top_share) {
   if (top_share) *top_share = 0.0;
   if (n == 0) { *est = 1; return TGA_OK; }
-  // the sample is sorted and counted on the GPU; only three counts come back
+  // the sample is counted on the GPU; only its counts come back
   const uint64_t m = std::min<uint64_t>(n, 16384);
   DBuf buf;
-  if (!buf.alloc(ctx, 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
-  TGA_HIP(ctx, launch_sample_stats(key_bytes, d_keys, n, (uint32_t)m, buf.as<uint64_t>(), ctx->stream));
-  uint64_t h[3];
+  if (!buf.alloc(ctx, m * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
+  uint64_t *smp = buf.as<uint64_t>() + 8;
+  TGA_HIP(ctx, launch_sample_stats(key_bytes, d_keys, n, (uint32_t)m, smp, buf.as<uint64_t>(), ctx->stream));
+  uint64_t h[4];
   TGA_HIP(ctx, hipMemcpyAsync(h, buf.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *est = estimate_from_stats(h[0], h[1], h[2], m, n, top_share);
+  if (!h[3]) {
+    *est = estimate_from_stats(h[0], h[1], h[2], m, n, top_share);
+    return TGA_OK;
+  }
+  // keys that crowd one quarter of the counting table: count the sample on
+  // the host instead
+  std::vector<uint64_t> v(m);
+  TGA_HIP(ctx, hipMemcpyAsync(v.data(), smp, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *est = estimate_from_sample(v.data(), m, n, top_share);
   return TGA_OK;
 }
 

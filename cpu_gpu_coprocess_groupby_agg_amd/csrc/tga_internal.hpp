@@ -94,9 +94,12 @@ uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
-// out[0..2] = {distinct keys, keys seen once, largest count of one key} of
-// the m <= 16384 keys sampled at rows i * n / m (one workgroup, sorted in LDS)
-hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *out, hipStream_t st);
+// out[0..3] = {distinct keys, keys seen once, largest count of one key,
+// overflow} of the m <= 16384 keys sampled at rows i * n / m (gathered into
+// smp[m], then counted in an LDS hash table by one workgroup; overflow 1:
+// the counts are incomplete, smp still holds the sample)
+hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
+                               hipStream_t st);
 hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
                               hipStream_t st);
 hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,

@@ -1164,78 +1164,83 @@ __global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *o
     out[i] = (uint64_t)keys[(uint64_t)(((unsigned __int128)i * n) / m)];
 }
 
-// Statistics of a strided key sample for the cardinality estimate, in one
-// workgroup: the m <= kSampleMax sampled keys (sample i is row i * n / m) are
-// sorted in LDS (bitonic, padded to a power of two with the largest key),
-// and out = {distinct keys, keys seen once, largest count of one key}.  The
-// padding sorts last, so the first m sorted values are the sample itself.
-constexpr uint32_t kSampleMax = 16384;
+// Statistics of a key sample for the cardinality estimate, in one
+// workgroup: of the m <= kSampleMax keys in `smp` (gathered by k_sample_keys
+// with the whole grid: one workgroup alone has too few loads in flight),
+// out = {distinct keys, keys seen once, largest count of one key, overflow}.
+// The keys are counted in an LDS hash table, one quarter of the key hash at
+// a time (so a table of kSampleSlots holds any quarter of a sample of
+// distinct keys with room to spare); the key equal to the empty marker is
+// counted on its own.  overflow = 1: some quarter did not fit (only keys
+// chosen against the hash can do that), the counts are incomplete.
+constexpr uint32_t kSampleMax = 16384, kSamplePer = kSampleMax / 1024, kSampleSlots = 8192;
 
-template <typename K>
-__global__ void __launch_bounds__(1024) k_sample_stats(const K *__restrict__ keys, uint64_t n, uint32_t m,
-                                                       uint32_t M, uint64_t *out) {
+__global__ void __launch_bounds__(1024) k_sample_stats(const uint64_t *__restrict__ smp, uint32_t m, uint64_t *out) {
   extern __shared__ __align__(16) char smem[];
-  uint64_t *v = (uint64_t *)smem;  // [M]
-  __shared__ uint32_t wmax[1024 / kWave];
-  __shared__ uint32_t s_d, s_f1, s_top;
-  const uint32_t tid = threadIdx.x, nt = blockDim.x;
-  for (uint32_t i = tid; i < M; i += nt)
-    v[i] = i < m ? (uint64_t)keys[(uint64_t)(((unsigned __int128)i * n) / m)] : ~0ULL;
+  uint64_t *tk = (uint64_t *)smem;               // [kSampleSlots]
+  uint32_t *tc = (uint32_t *)(tk + kSampleSlots);  // [kSampleSlots]
+  __shared__ uint32_t s_d, s_f1, s_top, s_empty, s_over;
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;  // nt = 1024
+  uint64_t v[kSamplePer];
+#pragma unroll
+  for (uint32_t r = 0; r < kSamplePer; ++r) {
+    const uint32_t i = tid + r * nt;
+    v[r] = i < m ? smp[i] : 0;
+  }
+  for (uint32_t sl = tid; sl < kSampleSlots; sl += nt) {
+    tk[sl] = ~0ULL;
+    tc[sl] = 0;
+  }
   if (tid == 0) {
-    s_d = 0;
-    s_f1 = 0;
-    s_top = 0;
+    s_d = s_f1 = s_top = s_empty = s_over = 0;
   }
   __syncthreads();
-  for (uint32_t k = 2; k <= M; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t t = tid; t < (M >> 1); t += nt) {
-        const uint32_t i = 2 * t - (t & (j - 1));  // the lower index of pair t: bit j clear
-        const uint32_t l = i + j;
-        const uint64_t a = v[i], b = v[l];
-        if (((i & k) == 0) == (a > b)) {
-          v[i] = b;
-          v[l] = a;
+  uint32_t empties = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kSamplePer; ++r) empties += tid + r * nt < m && v[r] == ~0ULL;
+  if (empties) atomicAdd(&s_empty, empties);
+  uint32_t d = 0, f1 = 0, top = 0;
+  for (uint32_t q = 0; q < 4; ++q) {
+#pragma unroll
+    for (uint32_t r = 0; r < kSamplePer; ++r) {
+      const uint64_t key = v[r];
+      const uint64_t h = key_hash(key);
+      if (tid + r * nt >= m || key == ~0ULL || (h >> 62) != q) continue;
+      uint32_t sl = (uint32_t)h & (kSampleSlots - 1);
+      bool done = false;
+      for (uint32_t p = 0; p < kSampleSlots && !done; ++p) {
+        const uint64_t prev = atomicCAS((unsigned long long *)&tk[sl], ~0ULL, (unsigned long long)key);
+        if (prev == ~0ULL || prev == key) {
+          atomicAdd(&tc[sl], 1u);
+          done = true;
+        } else {
+          sl = (sl + 1) & (kSampleSlots - 1);
         }
       }
-      __syncthreads();
+      if (!done) s_over = 1;
     }
-  }
-  // runs of equal keys over the first m: thread t takes [t*per, t*per+per)
-  const uint32_t per = (M + nt - 1) / nt;
-  const uint32_t b0 = min(m, tid * per), e0 = min(m, b0 + per);
-  auto starts = [&](uint32_t i) { return i == 0 || v[i] != v[i - 1]; };
-  uint32_t last = 0;  // the last run start at or before this thread's segment (element 0 starts one)
-  for (uint32_t i = b0; i < e0; ++i)
-    if (starts(i)) last = i;
-  // exclusive max-scan of `last` over the threads
-  const uint32_t lane = __lane_id(), wave = tid / kWave;
-  uint32_t x = last;
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t y = (uint32_t)__shfl_up((int)x, o, kWave);
-    if ((int)lane >= o) x = max(x, y);
-  }
-  if (lane == kWave - 1) wmax[wave] = x;
-  __syncthreads();
-  uint32_t before = (uint32_t)__shfl_up((int)x, 1, kWave);
-  if (lane == 0) before = 0;
-  for (uint32_t w = 0; w < wave; ++w) before = max(before, wmax[w]);
-  uint32_t cur = before, d = 0, f1 = 0, top = 0;
-  for (uint32_t i = b0; i < e0; ++i) {
-    const bool st = starts(i), en = i + 1 == m || v[i] != v[i + 1];
-    if (st) cur = i;
-    d += st;
-    f1 += st && en;
-    if (en) top = max(top, i - cur + 1);
+    __syncthreads();
+    // this quarter's counts; each thread clears the slots it read
+    for (uint32_t sl = tid; sl < kSampleSlots; sl += nt) {
+      const uint32_t c = tc[sl];
+      d += c != 0;
+      f1 += c == 1;
+      top = max(top, c);
+      tk[sl] = ~0ULL;
+      tc[sl] = 0;
+    }
+    __syncthreads();
   }
   atomicAdd(&s_d, d);
   atomicAdd(&s_f1, f1);
   atomicMax(&s_top, top);
   __syncthreads();
   if (tid == 0) {
-    out[0] = s_d;
-    out[1] = s_f1;
-    out[2] = s_top;
+    const uint32_t c = s_empty;
+    out[0] = s_d + (c != 0);
+    out[1] = s_f1 + (c == 1);
+    out[2] = max(s_top, c);
+    out[3] = s_over;
   }
 }
 
@@ -1552,21 +1557,15 @@ hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint6
   return hipGetLastError();
 }
 
-hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *out, hipStream_t st) {
+hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
+                               hipStream_t st) {
   if (m == 0 || m > kSampleMax) return hipErrorInvalidValue;
-  uint32_t M = 2;
-  while (M < m) M <<= 1;
-  const size_t lds = (size_t)M * 8;
-  hipError_t e;
-  if (key_bytes == 4) {
-    e = set_lds_limit<uint32_t>((const void *)&k_sample_stats<uint32_t>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sample_stats<uint32_t>, dim3(1), dim3(1024), lds, st, (const uint32_t *)keys, n, m, M, out);
-  } else {
-    e = set_lds_limit<uint64_t>((const void *)&k_sample_stats<uint64_t>, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sample_stats<uint64_t>, dim3(1), dim3(1024), lds, st, (const uint64_t *)keys, n, m, M, out);
-  }
+  hipError_t e = launch_sample_keys(key_bytes, keys, n, m, smp, st);
+  if (e != hipSuccess) return e;
+  const size_t lds = (size_t)kSampleSlots * 12;
+  e = set_lds_limit<uint64_t>((const void *)&k_sample_stats, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sample_stats, dim3(1), dim3(1024), lds, st, (const uint64_t *)smp, m, out);
   return hipGetLastError();
 }
 

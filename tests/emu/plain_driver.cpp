@@ -18,7 +18,8 @@
 #include "hip/hip_runtime.h"
 
 namespace tga {
-hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *out, hipStream_t st);
+hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
+                               hipStream_t st);
 }
 
 extern "C" int64_t tga_o_groupby(const void *keys, int key_bytes, int64_t n, const void *const *cols,
@@ -181,15 +182,17 @@ int run_case(tga_context *ctx, const Case &c) {
   return bad ? 1 : 0;
 }
 
-// The sample statistics kernel (bitonic sort in LDS, runs of equal keys)
-// against a sort and count on the host: uniform, skewed and single-key keys,
-// sample lengths that are and are not powers of two, keys equal to the
-// padding value.
+// The sample statistics kernel (keys counted in an LDS hash table) against a
+// sort and count on the host: uniform, skewed and single-key keys, sample
+// lengths that are and are not powers of two, keys equal to the table's
+// empty marker.
 int check_sample_stats() {
   int bad = 0;
-  struct S { const char *name; std::vector<uint64_t> k; uint32_t m; };
+  struct S { const char *name; std::vector<uint64_t> k; uint32_t m; int kb = 8; };
   std::vector<S> sets;
   sets.push_back({"uniform", keys_u64(100000, 30000, 81), 16384});
+  sets.push_back({"all_distinct", keys_u64(16384, 1ULL << 40, 85), 16384});
+  sets.push_back({"u32_keys", keys_u32(70000, 9000, 86), 16384, 4});
   sets.push_back({"uniform_ragged", keys_u64(9001, 5000, 82), 9001});
   {
     std::vector<uint64_t> k = keys_u64(50000, 20000, 83);
@@ -213,15 +216,22 @@ int check_sample_stats() {
       top = std::max<uint64_t>(top, j - i);
       i = j;
     }
-    uint64_t *dk = nullptr, *dout = nullptr, got[3] = {0, 0, 0};
+    uint64_t *dk = nullptr, *dout = nullptr, *dsmp = nullptr, got[4] = {0, 0, 0, 0};
     hipMalloc(&dk, n * 8);
     hipMalloc(&dout, 64);
-    hipMemcpy(dk, c.k.data(), n * 8, hipMemcpyHostToDevice);
-    const hipError_t e = tga::launch_sample_stats(8, dk, n, c.m, dout, nullptr);
+    hipMalloc(&dsmp, (size_t)c.m * 8);
+    if (c.kb == 4) {
+      std::vector<uint32_t> k32(c.k.begin(), c.k.end());
+      hipMemcpy(dk, k32.data(), n * 4, hipMemcpyHostToDevice);
+    } else {
+      hipMemcpy(dk, c.k.data(), n * 8, hipMemcpyHostToDevice);
+    }
+    const hipError_t e = tga::launch_sample_stats(c.kb, dk, n, c.m, dsmp, dout, nullptr);
     hipMemcpy(got, dout, sizeof got, hipMemcpyDeviceToHost);
     hipFree(dk);
     hipFree(dout);
-    const bool ok = e == hipSuccess && got[0] == d && got[1] == f1 && got[2] == top;
+    hipFree(dsmp);
+    const bool ok = e == hipSuccess && got[0] == d && got[1] == f1 && got[2] == top && got[3] == 0;
     std::printf("sample_stats_%s: m %u: distinct %llu/%llu, once %llu/%llu, top %llu/%llu %s\n", c.name, c.m,
                 (unsigned long long)got[0], (unsigned long long)d, (unsigned long long)got[1],
                 (unsigned long long)f1, (unsigned long long)got[2], (unsigned long long)top, ok ? "ok" : "FAIL");
