@@ -736,14 +736,33 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
     return launch_compact(q.key_bytes, g, s, od, cdesc.as<uint64_t>(), (uint32_t *)(cdesc.as<uint64_t>() + ctiles),
                           count, st, emit_count);
   }));
+  // flags | group count | abandoned items | buckets taken, in one copy
   uint32_t hflags[kFlagWords];
   uint64_t hcount = 0;
   uint32_t novf = 0, used = 0;
-  TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof hflags, hipMemcpyDeviceToHost, st));
-  TGA_HIP(ctx, hipMemcpyAsync(&hcount, count, 8, hipMemcpyDeviceToHost, st));
-  if (may_abandon) TGA_HIP(ctx, hipMemcpyAsync(&novf, ovf.p, 4, hipMemcpyDeviceToHost, st));
-  if (pb.buckets) TGA_HIP(ctx, hipMemcpyAsync(&used, pb.bb.pool, 4, hipMemcpyDeviceToHost, st));
-  TGA_HIP(ctx, hipStreamSynchronize(st));
+  {
+    WordGather wg{};
+    wg.src[0] = g.flags;
+    wg.n[0] = kFlagWords;
+    wg.src[1] = (const uint32_t *)count;
+    wg.n[1] = 2;
+    wg.src[2] = may_abandon ? ovf.as<uint32_t>() : (const uint32_t *)count;
+    wg.n[2] = 1;
+    wg.src[3] = pb.buckets ? pb.bb.pool : (const uint32_t *)count;
+    wg.n[3] = 1;
+    wg.count = 4;
+    DBuf sbuf;
+    if (!sbuf.alloc(ctx, kStatusWords * 4)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "status words");
+    uint32_t *dst = sbuf.as<uint32_t>();
+    TGA_HIP(ctx, launch_gather_words(wg, dst, st));
+    TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 4) * 4, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipStreamSynchronize(st));
+    const uint32_t *h = ctx->status_host;
+    std::memcpy(hflags, h, sizeof hflags);
+    std::memcpy(&hcount, h + kFlagWords, 8);
+    if (may_abandon) novf = h[kFlagWords + 2];
+    if (pb.buckets) used = h[kFlagWords + 3];
+  }
   if (ctx->prof.on) ctx->prof.resolve();
   // the scatter never takes more buckets than bucket_cap(); a larger count
   // would mean dropped rows
@@ -981,6 +1000,12 @@ TGA_API tga_status tga_context_create(int32_t device, uint64_t mem_budget_bytes,
     budget = (uint64_t)(fr * 0.9);
   }
   c->arena = new Arena(budget);
+  if (hipHostMalloc((void **)&c->status_host, kStatusWords * 4, hipHostMallocDefault) != hipSuccess) {
+    delete c->arena;
+    hipStreamDestroy(c->stream);
+    delete c;
+    return TGA_ERR_HIP;
+  }
   *out = c;
   return TGA_OK;
 }
@@ -994,6 +1019,8 @@ TGA_API tga_status tga_context_destroy(tga_context *ctx) {
     ctx->prof.destroy();
     delete ctx->arena;
     ctx->arena = nullptr;
+    hipHostFree(ctx->status_host);
+    ctx->status_host = nullptr;
     hipStreamDestroy(ctx->stream);
   }
   delete ctx;

@@ -73,6 +73,7 @@ struct tga_context {
   size_t lds_max = 65536;  // dynamic LDS bytes one workgroup may use
   std::recursive_mutex mu;
   tga::Arena *arena = nullptr;
+  uint32_t *status_host = nullptr;  // pinned, kStatusWords: a call's status words come back in one copy
   tga::Profiler prof;
   std::string err;
   tga_options last_plan{};

@@ -100,6 +100,15 @@ hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const
 // the counts are incomplete, smp still holds the sample)
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st);
+// Copies n[i] 4-byte words from src[i] (i < count <= 8) one after another to
+// dst (one workgroup), so a call's small results come back in one copy.
+constexpr int kStatusWords = 64;
+struct WordGather {
+  const uint32_t *src[8];
+  uint32_t n[8];
+  uint32_t count;
+};
+hipError_t launch_gather_words(const WordGather &wg, uint32_t *dst, hipStream_t st);
 hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
                               hipStream_t st);
 hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,

@@ -1244,6 +1244,14 @@ __global__ void __launch_bounds__(1024) k_sample_stats(const uint64_t *__restric
   }
 }
 
+__global__ void k_gather_words(WordGather wg, uint32_t *dst) {
+  uint32_t o = 0;
+  for (uint32_t i = 0; i < wg.count; ++i) {
+    for (uint32_t j = threadIdx.x; j < wg.n[i]; j += blockDim.x) dst[o + j] = wg.src[i][j];
+    o += wg.n[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -1554,6 +1562,14 @@ hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint6
     hipLaunchKernelGGL(k_sample_keys<uint32_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint32_t *)keys, n, m, out);
   else
     hipLaunchKernelGGL(k_sample_keys<uint64_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint64_t *)keys, n, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_words(const WordGather &wg, uint32_t *dst, hipStream_t st) {
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < wg.count; ++i) total += wg.n[i];
+  if (wg.count > 8 || total > (uint32_t)kStatusWords) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gather_words, dim3(1), dim3(64), 0, st, wg, dst);
   return hipGetLastError();
 }
 
