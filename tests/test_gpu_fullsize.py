@@ -1,0 +1,79 @@
+"""The plain operator at the full sizes of the 2^28-row and 2^29-row
+workloads, checked through properties that hold at any size (the oracle
+cannot aggregate 2^29 rows in the time a test has):
+
+- the groups: keys strictly increasing, the group count, every key of a
+  random row present, and COUNT summing to the row count;
+- the aggregates in total: SUM over the groups equals the column's exact
+  int64 total, the least MIN and the largest MAX equal the column's;
+- the aggregates per group, for sampled groups (the hottest ones among them
+  under Zipf keys): bit-exact against the oracle over that group's rows, fp64
+  SUM within 1e-12 relative of the oracle's compensated sum.
+"""
+import numpy as np
+import pytest
+
+import cpu_gpu_coprocess_groupby_agg_amd as tga
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+M64 = 2**64 - 1
+
+
+def check_groups(got, kh, n, rng):
+    keys = got["keys"]
+    assert np.all(keys[1:] > keys[:-1])  # distinct (the result is sorted by key)
+    assert int(got["aggs"][0].sum()) == n  # COUNT
+    rows = rng.integers(0, n, size=1000)
+    pos = np.searchsorted(keys, kh[rows])
+    assert np.all(pos < keys.size) and np.all(keys[np.minimum(pos, keys.size - 1)] == kh[rows])
+
+
+def check_sampled_groups(got, kh, cols, aggs, picks):
+    for j in picks:
+        key = got["keys"][j]
+        m = kh == key
+        ref = O.groupby(kh[m], [c[m] for c in cols], aggs)
+        assert ref["n_groups"] == 1 and ref["keys"][0] == key
+        for a, (g, r) in enumerate(zip(got["aggs"], ref["aggs"])):
+            if g.dtype == np.float64:
+                assert abs(g[j] - r[0]) <= 1e-12 * max(abs(r[0]), 1e-300), (j, a)
+            else:
+                assert g[j] == r[0], (j, a)
+
+
+def test_uniform_2p28_rows_2p20_keys(ctx):
+    n, d = 1 << 28, 1 << 20
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=2028, salt=0x5EED)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & M64, b=1 << 31, seed=2029)
+    aggs = ["count", ("sum", 0), ("min", 0), ("max", 0)]
+    got = ctx.groupby(k, [v], aggs)
+    kh, vh = k.to_numpy(), v.to_numpy()
+    rng = np.random.default_rng(28)
+    assert got["n_groups"] == d
+    check_groups(got, kh, n, rng)
+    cnt, sm, mn, mx = got["aggs"]
+    assert int(sm.sum()) == int(vh.astype(np.int64).sum())
+    assert mn.min() == vh.min() and mx.max() == vh.max()
+    check_sampled_groups(got, kh, [vh], aggs, rng.choice(d, 8, replace=False))
+
+
+def test_zipf_2p29_rows_2p24_keys_int_and_fp64(ctx):
+    n, D = 1 << 29, 1 << 24
+    k = ctx.generate(tga.GEN_ZIPF_U64, n, a=D, x=1.1, seed=2930, salt=11)
+    a = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & M64, b=1 << 31, seed=2931)
+    f = ctx.generate(tga.GEN_NORMAL_F64, n, x=0.0, y=1e3, seed=2932)
+    aggs = ["count", ("sum", 0), ("sum", 1)]
+    got = ctx.groupby(k, [a, f], aggs)
+    kh, ah, fh = k.to_numpy(), a.to_numpy(), f.to_numpy()
+    rng = np.random.default_rng(29)
+    assert 1 << 20 < got["n_groups"] <= D
+    check_groups(got, kh, n, rng)
+    cnt, si, sf = got["aggs"]
+    assert int(si.sum()) == int(ah.astype(np.int64).sum())
+    total = O.neumaier(fh)
+    assert abs(float(sf.sum()) - total) <= 1e-9 * float(np.abs(fh).sum())
+    hot = np.argsort(cnt)[-3:]  # the three hottest keys, about a fifth of the rows
+    assert int(cnt[hot].sum()) > n // 8
+    check_sampled_groups(got, kh, [ah, fh], aggs, list(hot) + list(rng.choice(got["n_groups"], 5, replace=False)))
