@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the reference-workload section (its driver's uniform / Zipf / self-similar keys)")
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--only", choices=["all", "headline", "copy"], default="all",
@@ -258,6 +260,38 @@ def make_batch(ctx, tga, nq, seed=7):
     return rows, dist, keys, vals
 
 
+def workloads(ctx, tga, args, steps=3):
+    """The reference driver's workload shapes (test/test.cu:96-129) at the
+    headline's row count: u32 keys of cardinality n * 10 / 100 with uniform,
+    Zipf(0.9) and self-similar(0.9) draws of the repeated keys (rows
+    shuffled), u32-range int32 values, COUNT + SUM; the group count must equal
+    the cardinality.  Per workload: ms per call and input rows/s, no hint."""
+    n = 1 << args.rows_log2
+    card = n * 10 // 100
+    out = {}
+    vals = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1 << 31, seed=97)
+    for dist, name in ((0, "uniform"), (1, "zipf_0.9"), (2, "selfsimilar_0.9")):
+        keys = ctx.generate(tga.GEN_REF_U32, n, a=card, b=dist, x=0.9, seed=96, salt=dist)
+        aggs = ["count", ("sum", 0)]
+        r, G = ctx.groupby_resident(keys, [vals], aggs)
+        ctx.free_result(r)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r, G = ctx.groupby_resident(keys, [vals], aggs)
+            ctx.free_result(r)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        plan = ctx.last_plan()
+        out[name] = {"ms": round(ms, 3), "rows_per_s": round(n / (ms * 1e-3), 1), "groups": int(G),
+                     "groups_equal_cardinality": int(G) == card, "partition_bits": plan["partition_bits"],
+                     "attempts": plan["attempts"]}
+        keys.free()
+        log(f"workload {name}: {out[name]}")
+    vals.free()
+    return {"rows": n, "cardinality": card, "key": "u32", "aggs": "COUNT, SUM(int64)", **out}
+
+
 def batched(ctx, tga, args):
     import ctypes
     from cpu_gpu_coprocess_groupby_agg_amd import lib
@@ -412,6 +446,8 @@ def main():
     vals.free()
     extra = {}
     pool = None
+    if args.only == "all" and not args.no_workloads:
+        extra["workloads"] = workloads(ctx, tga, args)
     if not args.no_batch:
         extra["batched"], pool = batched(ctx, tga, args)
     if not args.no_serve:

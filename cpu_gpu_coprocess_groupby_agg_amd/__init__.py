@@ -45,7 +45,7 @@ _OPS = {"count": COUNT, "sum": SUM, "min": MIN, "max": MAX}
 
 PATH_AUTO, PATH_DIRECT, PATH_PARTITIONED = 0, 1, 2
 
-GEN_KEYS_U32, GEN_KEYS_U64, GEN_VALS_I32, GEN_ZIPF_U64, GEN_NORMAL_F64 = 1, 2, 3, 4, 5
+GEN_KEYS_U32, GEN_KEYS_U64, GEN_VALS_I32, GEN_ZIPF_U64, GEN_NORMAL_F64, GEN_REF_U32 = 1, 2, 3, 4, 5, 6
 
 
 class TgaError(RuntimeError):
@@ -255,7 +255,8 @@ class Context:
         return d
 
     def generate(self, kind, n, a=0, b=0, x=0.0, y=0.0, seed=1, salt=0):
-        dt = {GEN_KEYS_U32: U32, GEN_KEYS_U64: U64, GEN_VALS_I32: I32, GEN_ZIPF_U64: U64, GEN_NORMAL_F64: F64}[kind]
+        dt = {GEN_KEYS_U32: U32, GEN_KEYS_U64: U64, GEN_VALS_I32: I32, GEN_ZIPF_U64: U64, GEN_NORMAL_F64: F64,
+              GEN_REF_U32: U32}[kind]
         d = DeviceArray(self, n, dt)
         self._check(lib().tga_generate(self.h, kind, d.ptr, n, a & (2**64 - 1), b, x, y, seed, salt))
         return d

@@ -26,7 +26,8 @@ struct PartOut {
   void *cols[kMaxCols];
 };
 
-enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_ZIPF_U64 = 4, GEN_NORMAL_F64 = 5 };
+enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_ZIPF_U64 = 4, GEN_NORMAL_F64 = 5,
+                     GEN_REF_U32 = 6 };
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);

@@ -1157,6 +1157,69 @@ __global__ void k_gen_normal_f64(double *out, uint64_t n, double mean, double si
   }
 }
 
+// The reference's workload (test/test.cu:96-129, data_generator.hpp:7-85),
+// restated as a counter-based generator: `card` distinct u32 keys (a 32-bit
+// bijection of their index, never the empty marker 0xffffffff), each written
+// once, the other n - card rows copying the key of an index drawn uniformly
+// (dist 0), from the Zipf distribution of Gray et al. 1994 with theta < 1
+// (dist 1: the zipfian_int_distribution the reference includes), or
+// self-similar with skew h (dist 2: index = card * u^(log h / log(1 - h))),
+// and the rows shuffled by a bijection of [0, n) (a 4-round Feistel network
+// on the next even bit width, cycle-walked into range).  So the group count
+// is exactly `card`.
+struct RefGenConst {
+  uint64_t n, card, seed;
+  uint32_t dist, half;  // Feistel half width in bits
+  double zetan, alpha, eta, zcut2, ss_exp;
+};
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {  // a bijection of the 32-bit integers
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t feistel(uint64_t x, uint32_t half, uint64_t seed) {
+  const uint64_t mask = (1ULL << half) - 1;
+  uint64_t l = x >> half, r = x & mask;
+#pragma unroll
+  for (uint32_t round = 0; round < 4; ++round) {
+    const uint64_t f = mix32((uint32_t)r ^ (uint32_t)(seed >> (16 * round)) ^ (round * 0x9e3779b9U)) & mask;
+    const uint64_t t = l ^ f;
+    l = r;
+    r = t;
+  }
+  return (l << half) | r;
+}
+
+__global__ void k_gen_ref_u32(uint32_t *out, RefGenConst c) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < c.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t s = feistel(i, c.half, c.seed);
+    while (s >= c.n) s = feistel(s, c.half, c.seed);  // cycle-walk: ends, the domain is < 4n
+    uint64_t idx = s;
+    if (s >= c.card) {
+      const double u = (double)(splitmix_at(c.seed ^ 0xa5a5a5a5ULL, s) >> 11) * 0x1.0p-53;  // [0, 1)
+      double x;
+      if (c.dist == 0) {
+        x = u * (double)c.card;
+      } else if (c.dist == 1) {
+        const double uz = u * c.zetan;
+        x = uz < 1.0 ? 0.0 : uz < c.zcut2 ? 1.0 : (double)c.card * pow(c.eta * u - c.eta + 1.0, c.alpha);
+      } else {
+        x = (double)c.card * pow(u, c.ss_exp);
+      }
+      idx = (uint64_t)x;
+      if (idx >= c.card) idx = c.card - 1;
+    }
+    uint32_t key = mix32((uint32_t)idx);
+    if (key == 0xffffffffU) key = mix32((uint32_t)c.card);  // the one index that maps to the empty marker
+    out[i] = key;
+  }
+}
+
 // strided key sample for the cardinality estimate
 template <typename K>
 __global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *out) {
@@ -1608,6 +1671,32 @@ hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, d
       break;
     }
     case GEN_NORMAL_F64: hipLaunchKernelGGL(k_gen_normal_f64, dim3(grid), dim3(256), 0, st, (double *)out, n, x, y, seed); break;
+    case GEN_REF_U32: {
+      // a = cardinality (1 <= a <= n, a < 2^32 - 1), b = dist (0 uniform, 1 Zipf, 2 self-similar), x = theta or h
+      if (a < 1 || a > n || a >= 0xffffffffULL || b > 2 || n >= (1ULL << 32)) return hipErrorInvalidValue;
+      if (b != 0 && !(x > 0.0 && x < 1.0)) return hipErrorInvalidValue;
+      RefGenConst c{};
+      c.n = n;
+      c.card = a;
+      c.seed = seed ^ (salt * 0x9e3779b97f4a7c15ULL);
+      c.dist = (uint32_t)b;
+      uint32_t w = 2;
+      while ((1ULL << w) < n) w += 2;
+      c.half = w / 2;
+      if (b == 1) {
+        double z = 0.0;  // zeta(card, theta), summed as the reference does
+        for (uint64_t i = 1; i <= a; ++i) z += std::pow(1.0 / (double)i, x);
+        const double z2 = 1.0 + std::pow(0.5, x);
+        c.zetan = z;
+        c.alpha = 1.0 / (1.0 - x);
+        c.eta = (1.0 - std::pow(2.0 / (double)a, 1.0 - x)) / (1.0 - z2 / z);
+        c.zcut2 = 1.0 + std::pow(0.5, x);
+      } else if (b == 2) {
+        c.ss_exp = std::log(x) / std::log(1.0 - x);
+      }
+      hipLaunchKernelGGL(k_gen_ref_u32, dim3(grid), dim3(256), 0, st, (uint32_t *)out, c);
+      break;
+    }
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -219,7 +219,10 @@ typedef enum tga_gen_kind {
   TGA_GEN_KEYS_U64 = 2,   /* a = distinct */
   TGA_GEN_VALS_I32 = 3,   /* a = lo (as int64 bits), b = hi - lo */
   TGA_GEN_ZIPF_U64 = 4,   /* a = distinct, x = exponent s */
-  TGA_GEN_NORMAL_F64 = 5  /* x = mean, y = sigma */
+  TGA_GEN_NORMAL_F64 = 5, /* x = mean, y = sigma */
+  TGA_GEN_REF_U32 = 6     /* u32 keys of the reference's workload (test/test.cu:96-129, data_generator.hpp:7-85):
+                             a = cardinality (exactly that many distinct keys), b = 0 uniform / 1 Zipf(theta < 1) /
+                             2 self-similar(h) draws of the repeated keys, x = theta or h; rows shuffled */
 } tga_gen_kind;
 /* Fills a device buffer with seeded counter-based synthetic data. */
 tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int64_t n, uint64_t a, uint64_t b, double x,

@@ -1,0 +1,46 @@
+"""The reference's workload shapes (test/test.cu:96-129: n rows, cardinality
+= n * c / 100, uniform / Zipf(0.9) / self-similar(0.9) draws of the repeated
+keys) from the GPU generator: the group count is exactly the cardinality,
+the aggregates are bit-exact against the oracle, and the skew profile
+matches the reference's own generator (built from its sources into
+oracle/_ref) at the same n, c and draw kind."""
+import os
+
+import numpy as np
+import pytest
+
+import cpu_gpu_coprocess_groupby_agg_amd as tga
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def top_shares(counts, n, tops):
+    c = np.sort(counts)[::-1]
+    return [float(c[:t].sum()) / n for t in tops]
+
+
+@pytest.mark.parametrize("dist", [0, 1, 2])
+def test_reference_workload(ctx, dist, tmp_path):
+    n, pct = 1 << 20, 10
+    card = n * pct // 100
+    k = ctx.generate(tga.GEN_REF_U32, n, a=card, b=dist, x=0.9, seed=7, salt=1)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1 << 31, seed=8)
+    kh, vh = k.to_numpy(), v.to_numpy()
+    assert not np.any(kh == 0xFFFFFFFF)  # the reference's empty key is never drawn
+    aggs = ["count", ("sum", 0)]
+    got = ctx.groupby(k, [v], aggs)
+    ref = O.groupby(kh, [vh], aggs)
+    assert got["n_groups"] == ref["n_groups"] == card
+    np.testing.assert_array_equal(got["keys"], ref["keys"])
+    for g, r in zip(got["aggs"], ref["aggs"]):
+        np.testing.assert_array_equal(g, r)
+    if not os.path.exists(O.REF_DATAGEN):
+        pytest.skip("the reference's generator is not built (oracle/Makefile target ref)")
+    rk, _ = O.ref_datagen(n, pct, dist, str(tmp_path / "kv.bin"))
+    _, rc = np.unique(rk, return_counts=True)
+    assert rc.size == card
+    tops = (10, 100, card // 10, card // 2)
+    ours, theirs = top_shares(got["aggs"][0], n, tops), top_shares(rc, n, tops)
+    for t, a, b in zip(tops, ours, theirs):
+        assert abs(a - b) <= 0.05 * b + 2e-3, (dist, t, a, b)
