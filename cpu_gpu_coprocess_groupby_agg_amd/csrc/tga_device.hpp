@@ -233,6 +233,10 @@ __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint
     if (atomicOr(&g.flags[FLAG_SENTINEL], 1u) == 0u) atomicAdd(&g.flags[FLAG_GROUPS], 1u);
     return (int64_t)g.S;
   }
+  // An overflowing table is run again larger by the host, so once the flag
+  // is up nothing more is inserted: a table found full would otherwise make
+  // every later insert probe it end to end.
+  if (*(volatile uint32_t *)&g.flags[FLAG_OVERFLOW]) return -1;
   K *keys = (K *)g.keys;
   const uint64_t S = g.S;
   uint64_t s = hash_global_slot(h, S);
@@ -243,6 +247,9 @@ __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint
     const K cur = keys[s];
     if (cur == key) return (int64_t)s;
     if (cur == KeyTraits<K>::kEmpty) {
+      // more groups than 7/8 of the slots: overloaded (probes grow without
+      // bound past that), let the host grow it
+      if (*(volatile uint32_t *)&g.flags[FLAG_GROUPS] >= S - S / 8) break;
       const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
       if (prev == KeyTraits<K>::kEmpty) {
         atomicAdd(&g.flags[FLAG_GROUPS], 1u);
