@@ -332,6 +332,20 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     if (wt && (o.tile_rows <= 0 || (uint32_t)o.tile_rows == wt)) {
       p.tile_rows = wt;
       p.scatter_wc = 1;
+    } else if (!wt && p.pbits - kTwoPassSubBits >= 1) {
+      // A fanout whose carry buffer does not fit: two passes, written
+      // whole-line, the first into 2^(pbits - 4) partitions, the second
+      // splitting each by the next 4 hash bits.  Past 2^9 partitions the
+      // one-pass scatter's tiles hold under 16 rows per partition and nearly
+      // every line it writes is partial (2^28 rows of 8-byte records at 2^12
+      // partitions: 7.7 ms in one pass, 1.0 + 1.2 ms in two).
+      const int b1 = p.pbits - kTwoPassSubBits;
+      const uint32_t t1 = scatter_wc_tile(q.key_bytes, b1, lds_max);
+      if (t1) {
+        p.tile_rows = t1;
+        p.scatter_wc = 1;
+        p.scatter_p1_bits = b1;
+      }
     }
   }
   p.opt = o;
@@ -490,10 +504,16 @@ struct PartBufs {
   BucketBufs bb{};
   bool buckets = false;
   bool rec = false;  // pkeys holds (key, 4-byte value) records, no pcols
+  // two-pass partitioning: the first pass's buffers, and its bucket pool
+  // (checked against its capacity once the call is done)
+  DBuf heads1, meta1, pkeys1, plan1, items1;
+  uint32_t *pool1 = nullptr;
+  uint32_t cap1 = 0;
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
   void keep(std::vector<DBuf> &k) {
-    for (DBuf *b : {&heads, &meta, &pkeys, &planbuf, &cnt, &offs, &total, &pbase, &desc})
+    for (DBuf *b : {&heads, &meta, &pkeys, &planbuf, &cnt, &offs, &total, &pbase, &desc, &heads1, &meta1, &pkeys1,
+                    &plan1, &items1})
       if (b->p) k.push_back(std::move(*b));
     for (auto &c : pcols)
       if (c.p) k.push_back(std::move(c));
@@ -504,34 +524,101 @@ struct PartBufs {
 // work-item plan of k_agg.  Partition-contiguous layout: per-tile histogram,
 // column scan, scatter to the scanned offsets.  Bucket layout: count-free
 // scatter into buckets, then the per-partition bucket lists.
+// Bucket headers, lists and counters of a bucket pool of `cap` buckets over
+// P partitions (counters zeroed on the stream).
+static tga_status bucket_bufs(tga_context *ctx, uint64_t cap, uint32_t P, DBuf &heads, DBuf &meta_buf,
+                              BucketBufs &bb) {
+  if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
+  // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
+  const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
+  if (!heads.alloc(ctx, cap * 16) || !meta_buf.alloc(ctx, meta_words * 8))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  bb.blist = heads.as<uint64_t>();
+  bb.bpart = (uint32_t *)(bb.blist + cap);
+  bb.bcnt = bb.bpart + cap;
+  uint64_t *meta = meta_buf.as<uint64_t>();
+  bb.pool = (uint32_t *)meta;
+  bb.pbk = (uint32_t *)(meta + 1);
+  bb.bcur = bb.pbk + P;
+  bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
+  bb.cap = (uint32_t)cap;
+  TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, ctx->stream));
+  return TGA_OK;
+}
+
+// Two-pass partitioning into 2^plan.pbits partitions (records, bucket
+// layout): the first pass into 2^plan.scatter_p1_bits partitions in the
+// buffers *1 of pb, then each of its partitions split again into the final
+// pool of pb.bb.  Its list entries are cut into work items of this many.
+constexpr uint32_t kPass2Entries = 1024;
+
+static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s,
+                                     uint64_t m, const Plan &plan, unsigned grid, PartBufs &pb, uint64_t &slots) {
+  hipStream_t st = ctx->stream;
+  const int b1 = plan.scatter_p1_bits, b2 = plan.pbits - b1;
+  const uint32_t P1 = 1u << b1, P = 1u << plan.pbits;
+  const uint64_t row_bytes = (uint64_t)q.key_bytes + 4;
+  // pass 1
+  const uint64_t cap1 = bucket_cap(m, grid, b1, scatter_block(plan.tile_rows, 1));
+  BucketBufs b1bufs{};
+  if (tga_status e = bucket_bufs(ctx, cap1, P1, pb.heads1, pb.meta1, b1bufs)) return e;
+  if (!pb.pkeys1.alloc(ctx, (cap1 << kBucketShift) * row_bytes) || !pb.plan1.alloc(ctx, (uint64_t)(P1 + 1) * 12 + 64) ||
+      !pb.items1.alloc(ctx, (uint64_t)(P1 + 2) * 4))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  pb.pool1 = b1bufs.pool;
+  pb.cap1 = (uint32_t)cap1;
+  PartOut po{};
+  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+    return launch_part_scatter(q.key_bytes, keys, s, m, b1, plan.tile_rows, pb.pkeys1.p, po, &b1bufs, nullptr, nullptr,
+                               grid, st, 1);
+  }));
+  AggPlan ap1 = part_plan(pb.plan1, P1);
+  ap1.blist = b1bufs.blist;
+  uint32_t *istart = pb.items1.as<uint32_t>(), *ticket = istart + P1 + 1;
+  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] {
+    hipError_t e = launch_bucket_lists(b1bufs, P1, ~0ULL >> 1, ap1, st);
+    if (e != hipSuccess) return e;
+    return launch_pass2_plan(ap1.pstart, P1, kPass2Entries, istart, ticket, st);
+  }));
+  // pass 2: at most P1 + cap1 / kPass2Entries work items, each leaving up to
+  // 2^b2 buckets open, and a reserve per wave
+  const unsigned grid2 = (unsigned)(ctx->cus + 7) / 8 * 8;  // every CU, a multiple of the XCD count
+  const uint64_t items_max = P1 + cap1 / kPass2Entries + 1;
+  const uint64_t cap = (m + kBucketRows - 1) / kBucketRows + items_max * (1ULL << b2) + (uint64_t)grid2 * 1024 + 1;
+  if (tga_status e = bucket_bufs(ctx, cap, P, pb.heads, pb.meta, pb.bb)) return e;
+  slots = cap << kBucketShift;
+  if (!pb.pkeys.alloc(ctx, slots * row_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  TGA_HIP(ctx, timed(ctx, "part_scatter2", [&] {
+    return launch_part_scatter2(q.key_bytes, pb.pkeys1.p, b1bufs.blist, ap1.pstart, istart, ticket, P1, plan.pbits,
+                                kPass2Entries, b2, pb.pkeys.p, pb.bb, grid2, st);
+  }));
+  return TGA_OK;
+}
+
 static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s, uint64_t m,
                                  const Plan &plan, PartBufs &pb) {
   hipStream_t st = ctx->stream;
   const uint32_t P = 1u << plan.pbits;
   const unsigned grid = scatter_grid(ctx, q, plan, m);
   pb.buckets = plan.scatter_layout == 2;
+  pb.rec = copy_as_records(s);
   uint64_t slots = m;
+  if (pb.buckets && plan.scatter_p1_bits > 0) {
+    if (!pb.rec) return fail(ctx, TGA_ERR_INVALID_ARG, "two-pass partitioning needs a record copy");
+    if (tga_status e = partition_two_pass(ctx, q, keys, s, m, plan, grid, pb, slots)) return e;
+    if (!pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+    pb.sp = s;
+    pb.sp.cols[0] = nullptr;
+    pb.ap = part_plan(pb.planbuf, P);
+    pb.ap.blist = pb.bb.blist;
+    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb, P, plan.chunk_rows, pb.ap, st); }));
+    return TGA_OK;
+  }
   if (pb.buckets) {
     const uint64_t cap = bucket_cap(m, grid, plan.pbits, scatter_block(plan.tile_rows, plan.scatter_wc));
-    if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
+    if (tga_status e = bucket_bufs(ctx, cap, P, pb.heads, pb.meta, pb.bb)) return e;
     slots = cap << kBucketShift;
-    // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
-    const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
-    if (!pb.heads.alloc(ctx, cap * 16) || !pb.meta.alloc(ctx, meta_words * 8))
-      return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
-    BucketBufs &bb = pb.bb;
-    bb.blist = pb.heads.as<uint64_t>();
-    bb.bpart = (uint32_t *)(bb.blist + cap);
-    bb.bcnt = bb.bpart + cap;
-    uint64_t *meta = pb.meta.as<uint64_t>();
-    bb.pool = (uint32_t *)meta;
-    bb.pbk = (uint32_t *)(meta + 1);
-    bb.bcur = bb.pbk + P;
-    bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
-    bb.cap = (uint32_t)cap;
-    TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, st));
   }
-  pb.rec = copy_as_records(s);
   const uint64_t row_bytes = (uint64_t)q.key_bytes + (pb.rec ? 4 : 0);
   if (!pb.pkeys.alloc(ctx, std::max<uint64_t>(slots, 1) * row_bytes) ||
       !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
@@ -739,7 +826,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   // flags | group count | abandoned items | buckets taken, in one copy
   uint32_t hflags[kFlagWords];
   uint64_t hcount = 0;
-  uint32_t novf = 0, used = 0;
+  uint32_t novf = 0, used = 0, used1 = 0;
   {
     WordGather wg{};
     wg.src[0] = g.flags;
@@ -750,24 +837,29 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
     wg.n[2] = 1;
     wg.src[3] = pb.buckets ? pb.bb.pool : (const uint32_t *)count;
     wg.n[3] = 1;
-    wg.count = 4;
+    wg.src[4] = pb.pool1 ? pb.pool1 : (const uint32_t *)count;
+    wg.n[4] = 1;
+    wg.count = 5;
     DBuf sbuf;
     if (!sbuf.alloc(ctx, kStatusWords * 4)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "status words");
     uint32_t *dst = sbuf.as<uint32_t>();
     TGA_HIP(ctx, launch_gather_words(wg, dst, st));
-    TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 4) * 4, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 5) * 4, hipMemcpyDeviceToHost, st));
     TGA_HIP(ctx, hipStreamSynchronize(st));
     const uint32_t *h = ctx->status_host;
     std::memcpy(hflags, h, sizeof hflags);
     std::memcpy(&hcount, h + kFlagWords, 8);
     if (may_abandon) novf = h[kFlagWords + 2];
     if (pb.buckets) used = h[kFlagWords + 3];
+    if (pb.pool1) used1 = h[kFlagWords + 4];
   }
   if (ctx->prof.on) ctx->prof.resolve();
   // the scatter never takes more buckets than bucket_cap(); a larger count
   // would mean dropped rows
   if (pb.buckets && used > pb.bb.cap)
     return fail(ctx, TGA_ERR_HIP, "bucket pool overflow (%u > %u)", used, pb.bb.cap);
+  if (pb.pool1 && used1 > pb.cap1)
+    return fail(ctx, TGA_ERR_HIP, "first-pass bucket pool overflow (%u > %u)", used1, pb.cap1);
   if (novf) {
     // Abandoned work items: a table of T slots was full after r of the
     // item's R rows, so the item holds about T * R / r groups; the finished

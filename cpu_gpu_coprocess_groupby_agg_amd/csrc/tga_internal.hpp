@@ -63,6 +63,18 @@ unsigned scatter_block(uint32_t tile_rows, int wc);
 // buckets the scatter can take: ceil(n / kBucketRows) + grid * (2^pbits +
 // its workgroup size)
 uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block);
+// Two-pass partitioning (fanouts above the write-combining scatter's 2^9):
+// the first pass's bucket lists (pstart: list offsets of its P1 partitions)
+// are cut into work items of items_per list entries (launch_pass2_plan), and
+// launch_part_scatter2 splits each into 2^pbits sub-partitions by hash bits
+// below the first pass's, writing records into the buckets of bb (8192-row
+// tiles; headers record the partition p1 * 2^pbits + sub-partition).
+hipError_t launch_pass2_plan(const uint64_t *pstart, uint32_t P1, uint32_t items_per, uint32_t *istart,
+                             uint32_t *ticket, hipStream_t st);
+hipError_t launch_part_scatter2(int key_bytes, const void *rec, const uint64_t *list, const uint64_t *pstart,
+                                const uint32_t *istart, uint32_t *ticket, uint32_t P1, int hash_bits,
+                                uint32_t items_per, int pbits, void *orec, const BucketBufs &bb, unsigned grid,
+                                hipStream_t st);
 // LDS of the write-combining scatter, and its largest tile (4096, 6144 or
 // 8192 rows) whose LDS fits lds_max, 0 if none does
 size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows);

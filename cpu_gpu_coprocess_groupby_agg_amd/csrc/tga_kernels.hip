@@ -400,6 +400,7 @@ struct ScatterState {
   K *okeys;
   PartOut o;
   uint32_t rlo, rhi;  // this wave's reserve of bucket ids (wave-uniform)
+  uint32_t pid_base;  // bucket headers record partition pid_base + p
 };
 
 // Rows of tile t into registers (rows past the end re-read the last row).
@@ -496,7 +497,7 @@ __device__ __forceinline__ void scatter_extend(ScatterState<K, NC> &S, uint32_t 
     uint32_t delta = 0;
     if (on && c) {
       for (uint32_t k = 0; k < fresh; ++k)
-        if (base + k < S.cap) S.bpart[base + k] = p;
+        if (base + k < S.cap) S.bpart[base + k] = S.pid_base + p;
       delta = f ? base - 1 : base;
       if (!f) first = base;
       const uint32_t end = f + c;
@@ -612,6 +613,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter(const K *__restrict__ key
   S.okeys = okeys;
   S.o = o;
   S.rlo = S.rhi = 0;
+  S.pid_base = 0;
 
   // Tiles of this workgroup: tile, tile + step, ... below tend.  Buckets: a
   // chunk of consecutive tiles (its runs of one partition append to one
@@ -742,10 +744,27 @@ size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows) {
          (((size_t)wc_tasks_max(P, tile_rows, RD) * 2 + 15) & ~(size_t)15);
 }
 
-template <typename K, int ITEMS>
+// TWO: the second pass of a two-pass partitioning.  The input is then the
+// first pass's records, read through its bucket lists in work items of at
+// most items_per list entries of one first-pass partition p1 (taken from a
+// ticket, so a hot partition is spread over many workgroups).  A row goes to
+// sub-partition key_partition(k, hash_bits) mod 2^pbits of p1, recorded in the
+// bucket headers as p1 * 2^pbits + that: the top hash_bits bits of the
+// partition hash, as one pass would give.  Each work item ends with its
+// carries written out and its buckets closed.
+struct Pass2In {
+  const void *rec;         // first-pass records
+  const uint64_t *list;    // first-pass bucket lists, rows << 32 | id, grouped by partition
+  const uint64_t *pstart;  // [P1 + 1] list offsets of the first-pass partitions
+  const uint32_t *istart;  // [P1 + 1] first work item of each first-pass partition
+  uint32_t *ticket;
+  uint32_t P1, hash_bits, items_per;
+};
+
+template <typename K, int ITEMS, bool TWO>
 __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ keys, Schema s, uint64_t n,
                                                           int pbits, uint32_t ntiles, void *orec, uint32_t *bpart,
-                                                          uint32_t *bcnt, uint32_t *pool, uint32_t cap) {
+                                                          uint32_t *bcnt, uint32_t *pool, uint32_t cap, Pass2In p2) {
   constexpr uint32_t RD = (uint32_t)Rec4<K>::kBytes / 4;     // dwords per record
   constexpr uint32_t LB = kBucketRows * RD / kLineDw;       // lines per bucket
   static_assert(kBucketRows * RD % kLineDw == 0, "buckets are whole lines");
@@ -782,31 +801,108 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
   S.pbase = nullptr;
   S.okeys = nullptr;
   S.rlo = S.rhi = 0;
+  S.pid_base = 0;
   uint32_t *out = (uint32_t *)orec;
+  const uint32_t hbits = TWO ? p2.hash_bits : (uint32_t)pbits;
+  constexpr uint32_t kBpt = ITEMS * 1024 / kBucketRows;  // first-pass buckets per tile (TWO)
+  __shared__ uint32_t s_item;
+  // TWO: the list entries of tiles t and t + 1, fetched two tiles ahead (at
+  // the top of tile t - 2, by the last wave) so no row load waits on one
+  __shared__ uint64_t s_ent[2][kBpt];
+  const uint32_t *col0 = (const uint32_t *)s.cols[0];
 
-  const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-  uint32_t tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
-  const uint32_t tend = min(ntiles, tile + per);
-  if (tile >= tend) return;  // uniform per workgroup
+  // Rows of tile t into registers (rows past the end re-read an existing
+  // one), with the mask of the valid ones.
+  uint64_t ie0 = 0, ie1 = 0;  // TWO: the work item's list entries
+  auto wc_load = [&](uint32_t t, K(&kk)[ITEMS], uint32_t(&cc)[ITEMS][1], uint32_t &vm) {
+    vm = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if constexpr (!TWO) {
+        const uint64_t row = (uint64_t)t * S.TR + (uint64_t)i * blockDim.x + threadIdx.x;
+        const uint64_t r = row < n ? row : n - 1;
+        kk[i] = keys[r];
+        cc[i][0] = col0[r];
+        vm |= (uint32_t)(row < n) << i;
+      } else {
+        const uint32_t j = (uint32_t)i * blockDim.x + threadIdx.x;
+        const uint64_t e = ie0 + (uint64_t)t * kBpt + (j >> kBucketShift);
+        const uint32_t rr = j & (kBucketRows - 1);
+        const uint64_t ent = s_ent[t & 1][j >> kBucketShift];
+        const bool ok = e < ie1 && rr < (uint32_t)(ent >> 32);
+        Rec4<K>::load(p2.rec, ((ent & 0xffffffffULL) << kBucketShift) + (ok ? rr : 0u), kk[i], cc[i][0]);
+        vm |= (uint32_t)ok << i;
+      }
+    }
+  };
+  auto ent_fetch = [&](uint32_t t) {
+    if constexpr (TWO) {
+      const uint32_t b = threadIdx.x - (blockDim.x - kBpt);
+      if (threadIdx.x >= blockDim.x - kBpt) {
+        const uint64_t e = ie0 + (uint64_t)t * kBpt + b;
+        s_ent[t & 1][b] = p2.list[e < ie1 ? e : ie1 - 1];
+      }
+    }
+  };
+  auto wc_rank = [&](const K(&kk)[ITEMS], uint32_t vm, uint32_t *h, uint32_t(&pp)[ITEMS], uint32_t(&rk)[ITEMS]) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      pp[i] = key_partition((uint64_t)kk[i], (int)hbits) & (S.P - 1);
+      rk[i] = (vm >> i) & 1u ? atomicAdd(&h[pp[i]], 1u) : 0u;
+    }
+  };
+
+  uint32_t tile = 0, tend = 0;
+  if constexpr (!TWO) {
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    tile = xcd_local_index(blockIdx.x, gridDim.x) * per;
+    tend = min(ntiles, tile + per);
+    if (tile >= tend) return;  // uniform per workgroup
+  }
+  const uint32_t items_total = TWO ? p2.istart[p2.P1] : 1u;
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (;;) {  // work items (one: this workgroup's tiles, without TWO)
+  if constexpr (TWO) {
+    if (threadIdx.x == 0) s_item = atomicAdd(p2.ticket, 1u);
+    __syncthreads();
+    const uint32_t item = wave_uniform(s_item);
+    if (item >= items_total) break;
+    uint32_t lo = 0, hi = p2.P1;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (p2.istart[mid] <= item) lo = mid; else hi = mid;
+    }
+    ie0 = p2.pstart[lo] + (uint64_t)(item - p2.istart[lo]) * p2.items_per;
+    ie1 = min(p2.pstart[lo + 1], ie0 + p2.items_per);
+    S.pid_base = lo * S.P;
+    tile = 0;
+    tend = (uint32_t)((ie1 - ie0 + kBpt - 1) / kBpt);
+  }
   for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x) {
     S.fill[p] = 0;
     S.cur[p] = 0;
     S.hist[p] = 0;
   }
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   K ka[ITEMS], kb[ITEMS];
   uint32_t ca[ITEMS][1], cb[ITEMS][1];
+  uint32_t va = 0, vb = 0;
   uint32_t pp[ITEMS], rk[ITEMS];
-  scatter_load<K, ITEMS, 1, uint32_t>(S, tile, ka, ca);
+  if constexpr (TWO) {
+    ent_fetch(tile);
+    ent_fetch(tile + 1);
+    __syncthreads();
+  }
+  wc_load(tile, ka, ca, va);
   lds_barrier();
-  scatter_rank<K, ITEMS, 1>(S, tile, ka, S.hist, pp, rk);
+  wc_rank(ka, va, S.hist, pp, rk);
   __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
   scatter_pin<K, ITEMS, 1, uint32_t>(ka, ca);
-  scatter_load<K, ITEMS, 1, uint32_t>(S, tile + 1, kb, cb);  // clamped past the end
+  wc_load(tile + 1, kb, cb, vb);  // clamped past the end
 
-#define TGA_WC_TILE(X, XC, Y, YC)                                                                \
+#define TGA_WC_TILE(X, XC, XV, Y, YC, YV)                                                        \
   {                                                                                              \
     lds_barrier(); /* counts complete; the previous tile's tasks done */                        \
+    ent_fetch(tile + 2); /* its slot's entries were read by the load of this tile */             \
     /* rows | tasks << 16 per partition, scanned together: a partition with */                  \
     /* rows has max(1, complete lines) tasks; its task 0 also moves the new */                  \
     /* partial line to the carry buffer */                                                      \
@@ -834,9 +930,8 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
       for (uint32_t l = l0; l < l1; ++l) lpid[l] = (uint16_t)p;                                  \
     }                                                                                            \
     {                                                                                            \
-      const uint64_t b0 = (uint64_t)tile * S.TR;                                                 \
       _Pragma("unroll") for (int i = 0; i < ITEMS; ++i) {                                        \
-        if (b0 + (uint64_t)i * blockDim.x + threadIdx.x < S.n) {                                 \
+        if ((XV >> i) & 1u) {                                                                    \
           uint32_t *r = stage + (S.hist[pp[i]] + rk[i]) * RD;                                    \
           r[0] = (uint32_t)(uint64_t)X[i];                                                       \
           if constexpr (RD == 3) r[1] = (uint32_t)((uint64_t)X[i] >> 32);                        \
@@ -881,16 +976,16 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
     }                                                                                            \
     const uint32_t next = tile + 1;                                                              \
     if (next >= tend) break;                                                                     \
-    scatter_rank<K, ITEMS, 1>(S, next, Y, S.hnext, pp, rk);                                      \
-    scatter_load<K, ITEMS, 1, uint32_t>(S, next + 1, X, XC);                                     \
+    wc_rank(Y, YV, S.hnext, pp, rk);                                                             \
+    wc_load(next + 1, X, XC, XV);                                                                \
     uint32_t *t_ = S.hist;                                                                       \
     S.hist = S.hnext;                                                                            \
     S.hnext = t_;                                                                                \
     tile = next;                                                                                 \
   }
   for (;;) {
-    TGA_WC_TILE(ka, ca, kb, cb)
-    TGA_WC_TILE(kb, cb, ka, ca)
+    TGA_WC_TILE(ka, ca, va, kb, cb, vb)
+    TGA_WC_TILE(kb, cb, vb, ka, ca, va)
   }
 #undef TGA_WC_TILE
   lds_barrier();  // the last carries are in place
@@ -903,6 +998,9 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
   }
   for (uint32_t p = threadIdx.x; p < S.P; p += blockDim.x)
     if (S.fill[p] && S.cur[p] < cap) bcnt[S.cur[p]] = S.fill[p];
+  if constexpr (!TWO) break;
+  __syncthreads();  // the item's state is read; the next item resets it
+  }  // work items
   for (uint32_t id = S.rlo + (uint32_t)__lane_id(); id < S.rhi; id += kWave)
     if (id < cap) bpart[id] = ~0u;
 }
@@ -1488,16 +1586,51 @@ uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max) {
   return 0;
 }
 
-template <typename K, int ITEMS>
+template <typename K, int ITEMS, bool TWO = false>
 static hipError_t launch_scatter_wc_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
                                       uint32_t ntiles, void *orec, const BucketBufs &bb, unsigned grid,
-                                      hipStream_t st) {
+                                      hipStream_t st, const Pass2In &p2 = Pass2In{}) {
   const size_t lds = scatter_wc_lds_bytes(sizeof(K), pbits, tile_rows);
-  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter_wc<K, ITEMS>, lds);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter_wc<K, ITEMS, TWO>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_part_scatter_wc<K, ITEMS>), dim3(grid), dim3(1024), lds, st, (const K *)keys, s, n, pbits,
-                     ntiles, orec, bb.bpart, bb.bcnt, bb.pool, bb.cap);
+  hipLaunchKernelGGL((k_part_scatter_wc<K, ITEMS, TWO>), dim3(grid), dim3(1024), lds, st, (const K *)keys, s, n,
+                     pbits, ntiles, orec, bb.bpart, bb.bcnt, bb.pool, bb.cap, p2);
   return hipGetLastError();
+}
+
+// Work items of the second pass: first-pass partition p owns items
+// [istart[p], istart[p+1]), one per items_per of its list entries; the
+// ticket is zeroed.  One workgroup.
+__global__ void __launch_bounds__(1024) k_pass2_plan(const uint64_t *pstart, uint32_t P1, uint32_t items_per,
+                                                     uint32_t *istart, uint32_t *ticket) {
+  __shared__ uint32_t scratch[1024 / kWave + 1];
+  const uint32_t tot = block_exclusive_scan_fn<SyncBarrier>(
+      P1, scratch,
+      [&](uint32_t p) { return (uint32_t)((pstart[p + 1] - pstart[p] + items_per - 1) / items_per); },
+      [&](uint32_t p, uint32_t v) { istart[p] = v; });
+  if (threadIdx.x == 0) {
+    istart[P1] = tot;
+    *ticket = 0;
+  }
+}
+
+hipError_t launch_pass2_plan(const uint64_t *pstart, uint32_t P1, uint32_t items_per, uint32_t *istart,
+                             uint32_t *ticket, hipStream_t st) {
+  if (P1 == 0 || P1 > 65536 || items_per == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_pass2_plan, dim3(1), dim3(1024), 0, st, pstart, P1, items_per, istart, ticket);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_scatter2(int key_bytes, const void *rec, const uint64_t *list, const uint64_t *pstart,
+                                const uint32_t *istart, uint32_t *ticket, uint32_t P1, int hash_bits,
+                                uint32_t items_per, int pbits, void *orec, const BucketBufs &bb, unsigned grid,
+                                hipStream_t st) {
+  if (pbits < 1 || hash_bits <= pbits || (1u << pbits) > kWcMaxP || items_per == 0) return hipErrorInvalidValue;
+  Pass2In p2{rec, list, pstart, istart, ticket, P1, (uint32_t)hash_bits, items_per};
+  const Schema s{};
+  if (key_bytes == 4)
+    return launch_scatter_wc_t<uint32_t, 8, true>(nullptr, s, 0, pbits, 8192, 0, orec, bb, grid, st, p2);
+  return launch_scatter_wc_t<uint64_t, 8, true>(nullptr, s, 0, pbits, 8192, 0, orec, bb, grid, st, p2);
 }
 
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,

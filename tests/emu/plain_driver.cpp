@@ -341,6 +341,17 @@ int main(int argc, char **argv) {
     cases.push_back({"split_hint_retry", 64, k, {col_i32(N, -99, 99, 72)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
                      opts(TGA_PATH_PARTITIONED, 2, 0, 0, 20001)});
   }
+  // two-pass partitioning (fanouts past the write-combining scatter's 2^9):
+  // 2^12 partitions of 32-bit keys; a hot key holding half the rows, so its
+  // first-pass partition is cut into several work items
+  cases.push_back({"two_pass_u32_p4096", 32, keys_u32(3 * N, 300000, 91), {col_i32(3 * N, -99, 99, 92)}, {TGA_I32},
+                   {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 12, 0, 300000)});
+  {
+    std::vector<uint64_t> k = keys_u64(7 * N / 2, 100000, 93);
+    for (size_t i = 0; i < k.size(); i += 2) k[i] = 1234567;
+    cases.push_back({"two_pass_hot_key", 64, k, {col_i32((int64_t)k.size(), -(1LL << 31), (1LL << 31) - 1, 94)},
+                     {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 11, 0, 100001)});
+  }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
   int bad = 0, ran = 0;
