@@ -249,8 +249,7 @@ static int ilog2(uint64_t x) {
 // Picks the path, the LDS table size, the partition fanout, tile and grid
 // from (rows, estimated groups) and the device; every explicit option wins.
 // ---------------------------------------------------------------------------
-Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est, const tga_options *opt,
-              double top_share) {
+Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est, const tga_options *opt) {
   tga_options o{};
   if (opt) o = *opt;
   Plan p;
@@ -349,7 +348,6 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     }
   }
   p.opt = o;
-  p.top_share = top_share;
   p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < 12;
   // global table at load <= 1/2 of the estimate, at most 2 n slots
   uint64_t gs = next_pow2(std::max<uint64_t>(1024, est * 2));
@@ -483,8 +481,9 @@ static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const P
 }
 
 // Plan buffer layout of the partitioned path: pstart[P+1] u64 | cstart[P+1]
-// u32 | ticket u32 | split_rows u64 (8-byte aligned).
-static AggPlan part_plan(const DBuf &buf, uint32_t P) {
+// u32 | ticket u32 | split_rows u64 (8-byte aligned) | global-table slots
+// u64.  gs_cap > 0: the plan kernel sizes the global table (AggPlan::gslots).
+static AggPlan part_plan(const DBuf &buf, uint32_t P, uint64_t gs_cap = 0, uint32_t gs_per_split = 0) {
   char *b = buf.as<char>();
   AggPlan ap{};
   ap.pstart = (uint64_t *)b;
@@ -492,6 +491,11 @@ static AggPlan part_plan(const DBuf &buf, uint32_t P) {
   ap.P = P;
   ap.ticket = ap.cstart + (P + 1);
   ap.split_rows = (uint64_t *)(b + (((uint64_t)(P + 1) * 12 + 4 + 7) & ~7ULL));
+  if (gs_cap) {
+    ap.gslots = ap.split_rows + 1;
+    ap.gs_cap = gs_cap;
+    ap.gs_per_split = gs_per_split;
+  }
   return ap;
 }
 
@@ -511,6 +515,9 @@ struct PartBufs {
   uint32_t cap1 = 0;
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
+  // > 0: the plan kernel sizes the global table, up to gs_cap slots
+  uint64_t gs_cap = 0;
+  uint32_t gs_per_split = 0;
   void keep(std::vector<DBuf> &k) {
     for (DBuf *b : {&heads, &meta, &pkeys, &planbuf, &cnt, &offs, &total, &pbase, &desc, &heads1, &meta1, &pkeys1,
                     &plan1, &items1})
@@ -609,7 +616,7 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     if (!pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
     pb.sp = s;
     pb.sp.cols[0] = nullptr;
-    pb.ap = part_plan(pb.planbuf, P);
+    pb.ap = part_plan(pb.planbuf, P, pb.gs_cap, pb.gs_per_split);
     pb.ap.blist = pb.bb.blist;
     TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb, P, plan.chunk_rows, pb.ap, st); }));
     return TGA_OK;
@@ -632,7 +639,7 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     po.cols[c] = pb.pcols[c].p;
     pb.sp.cols[c] = pb.pcols[c].p;
   }
-  pb.ap = part_plan(pb.planbuf, P);
+  pb.ap = part_plan(pb.planbuf, P, pb.gs_cap, pb.gs_per_split);
   if (pb.buckets) {
     BucketBufs &bb = pb.bb;
     TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
@@ -760,36 +767,35 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
   const bool may_abandon = plan.may_replan;
   if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, (uint64_t)ovf_cap << 32, st));
+  // Global table: split partitions and LDS spills only.  A partition is
+  // split past chunk_rows (twice its expected rows, so only a key holding
+  // more than half a partition's share of the rows splits one), and it holds
+  // about T/2 groups.  The plan kernel sizes the table from the split
+  // partitions it counts, 2T slots each (at most 2 slots a row), so the host
+  // never waits for the partitioning; the allocation is that size's bound:
+  // fewer than P/2 partitions can exceed 2 n/P rows.  A table that turns out
+  // too small (spills from partitions holding more groups than their LDS
+  // table) raises the overflow flag, and the call is retried with the
+  // full-size one.
   PartBufs pb;
+  const uint64_t T = plan.table_slots;
+  uint64_t S = plan.gslots;
+  if (!full_table) {
+    S = std::min(S, next_pow2(std::max<uint64_t>(4096, T * P)));
+    pb.gs_cap = S;
+    pb.gs_per_split = (uint32_t)(2 * T);
+  }
   {
     tga_status e = partition_rows(ctx, q, d_keys, s, n, plan, pb);
     if (e) return e;
   }
   const AggPlan &ap = pb.ap;
-  // Global table: split partitions and spills only.  A partition is split
-  // past chunk_rows, twice its expected rows, which takes a key holding more
-  // than half a partition's share of the rows.  Unless the key sample shows
-  // one, the table is sized without waiting for the partitioning (the GPU
-  // would idle meanwhile); one that turns out too small raises the overflow
-  // flag, and the call is retried with a full-size table.
-  uint64_t split_rows = 0;
-  if (!full_table && plan.top_share * (double)P >= 0.5) {
-    TGA_HIP(ctx, hipMemcpyAsync(&split_rows, ap.split_rows, 8, hipMemcpyDeviceToHost, st));
-    TGA_HIP(ctx, hipStreamSynchronize(st));
-  }
-  uint64_t S = plan.gslots;
-  *reduced = false;
-  if (!full_table) {
-    uint64_t want = 4096;
-    while (want < 2 * split_rows && want < S) want <<= 1;
-    S = std::min(S, want);
-    *reduced = S < plan.gslots;
-  }
   GTable g{};
   std::vector<DBuf> gbufs;
   DBuf flags;
   if (gtable_alloc(ctx, q, S, g, gbufs, flags) != hipSuccess)
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table of %llu slots", (unsigned long long)S);
+  g.S_dev = ap.gslots;
   TGA_HIP(ctx, timed(ctx, "gtable_init", [&] { return launch_gtable_init(q.key_bytes, g, s, st); }));
   // outputs
   const uint64_t cap = ext_keys ? ext_cap : plan.gslots + 1;
@@ -827,6 +833,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   uint32_t hflags[kFlagWords];
   uint64_t hcount = 0;
   uint32_t novf = 0, used = 0, used1 = 0;
+  uint64_t S_used = S;
   {
     WordGather wg{};
     wg.src[0] = g.flags;
@@ -839,12 +846,14 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
     wg.n[3] = 1;
     wg.src[4] = pb.pool1 ? pb.pool1 : (const uint32_t *)count;
     wg.n[4] = 1;
-    wg.count = 5;
+    wg.src[5] = ap.gslots ? (const uint32_t *)ap.gslots : (const uint32_t *)count;
+    wg.n[5] = 2;
+    wg.count = 6;
     DBuf sbuf;
     if (!sbuf.alloc(ctx, kStatusWords * 4)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "status words");
     uint32_t *dst = sbuf.as<uint32_t>();
     TGA_HIP(ctx, launch_gather_words(wg, dst, st));
-    TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 5) * 4, hipMemcpyDeviceToHost, st));
+    TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 7) * 4, hipMemcpyDeviceToHost, st));
     TGA_HIP(ctx, hipStreamSynchronize(st));
     const uint32_t *h = ctx->status_host;
     std::memcpy(hflags, h, sizeof hflags);
@@ -852,7 +861,9 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
     if (may_abandon) novf = h[kFlagWords + 2];
     if (pb.buckets) used = h[kFlagWords + 3];
     if (pb.pool1) used1 = h[kFlagWords + 4];
+    if (ap.gslots) std::memcpy(&S_used, h + kFlagWords + 5, 8);
   }
+  *reduced = S_used < plan.gslots;
   if (ctx->prof.on) ctx->prof.resolve();
   // the scatter never takes more buckets than bucket_cap(); a larger count
   // would mean dropped rows
@@ -915,13 +926,13 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
         // the estimate was too low for the fanout: plan again from the
         // group count the abandoned partitions imply, at least doubling P
         const int old_bits = plan.pbits;
-        Plan np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &plan.opt, plan.top_share);
+        Plan np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &plan.opt);
         if (np.path != TGA_PATH_PARTITIONED) np = plan;
         if (np.pbits <= old_bits) {
           tga_options o2 = plan.opt;
           o2.partition_bits = std::min(12, old_bits + 1);
           o2.est_groups = (int64_t)replan;
-          np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &o2, plan.top_share);
+          np = autotune(ctx, q, n, std::max<uint64_t>(replan, 1), &o2);
           np.may_replan = np.pbits < 12;
           np.opt = plan.opt;
         }
@@ -1052,12 +1063,11 @@ tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *va
   if (st) return st;
   for (int c = 0; c < n_values; ++c) q.schema.cols[c] = sg.vals[c].data;
   uint64_t est = opt && opt->est_groups > 0 ? (uint64_t)opt->est_groups : 0;
-  double top = 0.0;
   if (!est) {
-    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est, &top);
+    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est);
     if (st) return st;
   }
-  plan = autotune(ctx, q, (uint64_t)n_rows, est, opt, top);
+  plan = autotune(ctx, q, (uint64_t)n_rows, est, opt);
   return TGA_OK;
 }
 }  // namespace

@@ -44,7 +44,7 @@ enum AccKind : int32_t {
 };
 
 // global-table status words
-enum : int { FLAG_SENTINEL = 0, FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_GROUPS = 3, kFlagWords = 4 };
+enum : int { FLAG_SENTINEL = 0, FLAG_OVERFLOW = 1, FLAG_RANGE = 2, kFlagWords = 3 };
 
 struct Acc {
   int32_t kind;
@@ -64,11 +64,15 @@ struct Schema {
 };
 
 // Global table in HBM: the LDS table's SoA shape with S regular slots.
+// With S_dev set, the kernels use the slot count stored there (a power of
+// two <= S, written on the GPU by the work plan) in place of S: the arrays
+// are allocated for S and only the first *S_dev slots (+1) are used.
 struct GTable {
   void *keys;               // [S+1]
   void *acc[kMaxAccs];      // [S+1] each (ACC_SUM_F: [kLimbs][S+1])
   uint64_t S;
   uint32_t *flags;          // kFlagWords status words
+  const uint64_t *S_dev;    // null: S slots
 };
 
 // Dense output: keys plus one column per aggregate, `capacity` rows.

@@ -269,10 +269,11 @@ TGA_API tga_status tga_compat_intra_partition(tga_context *ctx, tga_partset *ps,
   while (S < 2 * (uint64_t)max_load) S <<= 1;
   GTable g{};
   std::vector<DBuf> gbufs;
-  DBuf flags, dk, dv, rk, rs;
+  DBuf flags, dk, dv, rk, rs, gcount;
   auto fail_free = [&](tga_status s) { hipHostFree(ok); hipHostFree(ov); return s; };
   if (gtable_alloc(ctx, q, S, g, gbufs, flags) != hipSuccess || !dk.alloc(ctx, (uint64_t)max_load * 4) ||
-      !dv.alloc(ctx, (uint64_t)max_load * 4) || !rk.alloc(ctx, (S + 1) * 4) || !rs.alloc(ctx, (S + 1) * 8))
+      !dv.alloc(ctx, (uint64_t)max_load * 4) || !rk.alloc(ctx, (S + 1) * 4) || !rs.alloc(ctx, (S + 1) * 8) ||
+      !gcount.alloc(ctx, 8))
     return fail_free(fail(ctx, TGA_ERR_OUT_OF_MEMORY, "compat intra buffers"));
   std::vector<uint64_t> hs;
   for (size_t i = 0; i < np; ++i) {
@@ -308,10 +309,12 @@ TGA_API tga_status tga_compat_intra_partition(tga_context *ctx, tga_partset *ps,
         tga_status s2 = aggregate_into(ctx, qc, dk.p, take, plan, g, keep);
         if (s2) return s2;
         uint32_t hf[kFlagWords];
+        TGA_HIP(ctx, launch_fill_u64(gcount.as<uint64_t>(), 1, 0, st));
+        TGA_HIP(ctx, launch_count_groups(4, g, gcount.as<uint64_t>(), st));
         TGA_HIP(ctx, hipMemcpyAsync(hf, g.flags, sizeof hf, hipMemcpyDeviceToHost, st));
+        TGA_HIP(ctx, hipMemcpyAsync(&already, gcount.p, 8, hipMemcpyDeviceToHost, st));
         TGA_HIP(ctx, hipStreamSynchronize(st));
         if (hf[FLAG_OVERFLOW]) return fail(ctx, TGA_ERR_TABLE_OVERFLOW, "hash table overflow");
-        already = hf[FLAG_GROUPS];
         tail -= take;
         rest -= take;
       }

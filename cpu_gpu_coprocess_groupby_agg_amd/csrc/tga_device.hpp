@@ -8,7 +8,7 @@
 namespace tga {
 
 constexpr int kWave = 64;
-constexpr uint64_t kMaxGlobalProbe = 4096;
+constexpr uint64_t kMaxGlobalProbe = 256;
 
 // A value every work-item read from the same LDS word after a barrier, as a
 // wave-uniform scalar.  Work-queue loops exit on such a value; read from LDS
@@ -226,36 +226,32 @@ __device__ __forceinline__ void lds_apply_limbs(const LdsTable<K> &t, int a, int
 // global table in HBM: the same SoA shape with `S` regular slots
 // ---------------------------------------------------------------------------
 
+// No shared counter is touched: every insert incrementing (or every probe
+// reading) one word serialises the whole GPU on that word's L2 channel
+// (26.8M inserts took ~140 ms).  The table's load shows in its probe
+// lengths instead: linear probing with this hash on 2^26 slots probes at
+// most 51 slots at load 1/2 (the tables' sizing), 231 at 3/4, and past
+// kMaxGlobalProbe = 256 dozens of times at 4/5, so a probe that long raises
+// the overflow flag and the host runs the call again with a larger table.
+// Once the flag is up nothing more is inserted; it is read every 32 probe
+// steps, so only long probes read it.
 template <typename K>
 __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint64_t h) {
-  // flags[FLAG_GROUPS] counts the groups inserted so far
   if (key == KeyTraits<K>::kEmpty) {
-    if (atomicOr(&g.flags[FLAG_SENTINEL], 1u) == 0u) atomicAdd(&g.flags[FLAG_GROUPS], 1u);
+    atomicOr(&g.flags[FLAG_SENTINEL], 1u);
     return (int64_t)g.S;
   }
-  // An overflowing table is run again larger by the host, so once the flag
-  // is up nothing more is inserted: a table found full would otherwise make
-  // every later insert probe it end to end.
-  if (*(volatile uint32_t *)&g.flags[FLAG_OVERFLOW]) return -1;
   K *keys = (K *)g.keys;
   const uint64_t S = g.S;
   uint64_t s = hash_global_slot(h, S);
-  // Tables are sized for load <= 1/2; a probe this long means the table is
-  // overloaded, so stop and let the host grow it (never scan the whole table).
   const uint64_t limit = S < kMaxGlobalProbe ? S : kMaxGlobalProbe;
   for (uint64_t i = 0; i < limit; ++i) {
+    if ((i & 31) == 31 && *(volatile uint32_t *)&g.flags[FLAG_OVERFLOW]) return -1;
     const K cur = keys[s];
     if (cur == key) return (int64_t)s;
     if (cur == KeyTraits<K>::kEmpty) {
-      // more groups than 7/8 of the slots: overloaded (probes grow without
-      // bound past that), let the host grow it
-      if (*(volatile uint32_t *)&g.flags[FLAG_GROUPS] >= S - S / 8) break;
       const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
-      if (prev == KeyTraits<K>::kEmpty) {
-        atomicAdd(&g.flags[FLAG_GROUPS], 1u);
-        return (int64_t)s;
-      }
-      if (prev == key) return (int64_t)s;
+      if (prev == KeyTraits<K>::kEmpty || prev == key) return (int64_t)s;
     }
     s = (s + 1 == S) ? 0 : s + 1;
   }

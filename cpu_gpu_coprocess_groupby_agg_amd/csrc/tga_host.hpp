@@ -155,17 +155,16 @@ struct Plan {
   int scatter_layout = 1;  // 1 = partition-contiguous, 2 = per-workgroup buckets
   int scatter_wc = 0;      // bucket layout of records written as whole cache lines
   int scatter_p1_bits = 0; // > 0: two passes, the first into 2^scatter_p1_bits partitions
-  // re-planning: the options and skew the plan came from, and whether the
-  // fanout was the tuner's (so a too-small estimate may raise it)
+  // re-planning: the options the plan came from, and whether the fanout was
+  // the tuner's (so a too-small estimate may raise it)
   tga_options opt{};
-  double top_share = 0.0;
   bool may_replan = false;
 };
 
 // Autotuner: plan from (rows, estimated groups, schema, device), honouring
 // any explicit field of `opt`.
 Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint64_t est_groups,
-              const tga_options *opt, double top_share = 0.0);
+              const tga_options *opt);
 
 // Cardinality estimate from a strided key sample (GEE estimator,
 // Charikar et al. 2000).  Needs the context lock.

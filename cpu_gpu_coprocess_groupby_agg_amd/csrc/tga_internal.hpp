@@ -20,6 +20,13 @@ struct AggPlan {
   uint32_t *ticket;
   uint64_t *split_rows;  // rows of partitions cut into more than one item
   const uint64_t *blist;
+  // Global-table sizing by the plan kernel (gslots null: none).  Only the
+  // partitions cut into several items aggregate through the global table,
+  // so it takes the power of two >= min(2 * their rows, gs_per_split * their
+  // count), at least 4096 slots and at most gs_cap.
+  uint64_t *gslots;
+  uint64_t gs_cap;
+  uint32_t gs_per_split;
 };
 
 struct PartOut {
@@ -105,6 +112,8 @@ hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax,
                                hipStream_t st);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
+// adds the group count of g (occupied slots + the empty-key slot) to *out
+hipError_t launch_count_groups(int key_bytes, const GTable &g, uint64_t *out, hipStream_t st);
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 // out[0..3] = {distinct keys, keys seen once, largest count of one key,

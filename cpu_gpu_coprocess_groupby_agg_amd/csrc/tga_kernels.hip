@@ -44,6 +44,7 @@ __global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, uin
 
 template <typename K>
 __global__ void k_gtable_init(GTable g, Schema s) {
+  if (g.S_dev) g.S = *g.S_dev;
   const uint64_t n = g.S + 1;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -88,6 +89,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
   __shared__ GTable s_g;
   __shared__ Schema s_s;
   __shared__ uint32_t s_abort[2];  // table full / rows consumed (abort mode)
+  if (g.S_dev) g.S = *g.S_dev;
   if (threadIdx.x == 0) {
     s_g = g;
     s_s = s;
@@ -160,12 +162,25 @@ __global__ void k_plan_direct(AggPlan plan, uint64_t n, uint32_t nchunks) {
   }
 }
 
+// Rows and count of the partitions cut into several work items, and the
+// global-table size they imply (AggPlan::gslots).  One thread.
+__device__ __forceinline__ void plan_split_totals(const AggPlan &plan, uint64_t srows, uint32_t nsplit) {
+  if (plan.split_rows) *plan.split_rows = srows;
+  if (!plan.gslots) return;
+  const uint64_t want = min(2 * srows, (uint64_t)plan.gs_per_split * nsplit);
+  uint64_t S = 4096;
+  while (S < want && S < plan.gs_cap) S <<= 1;
+  *plan.gslots = min(S, plan.gs_cap);
+}
+
 // (c) Partition-contiguous layout: partition p has rows [pbase[p], pbase[p+1]); it becomes
 // ceil(rows/chmax) work items.
 __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax,
                                                       AggPlan plan) {
   __shared__ uint32_t scratch[17];
-  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
+  __shared__ unsigned long long s_srows;
+  __shared__ uint32_t s_nsplit;
+  if (threadIdx.x == 0) { s_srows = 0; s_nsplit = 0; }
   __syncthreads();
   extern __shared__ uint32_t cnt[];  // P + 1
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
@@ -174,13 +189,19 @@ __global__ void __launch_bounds__(1024) k_plan_chunks(const uint32_t *pbase, uin
     plan.pstart[p] = b;
     const uint64_t len = e - b;
     cnt[p] = (uint32_t)((len + chmax - 1) / chmax);
-    if (cnt[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)len);
+    if (cnt[p] > 1) {
+      atomicAdd(&s_srows, (unsigned long long)len);
+      atomicAdd(&s_nsplit, 1u);
+    }
   }
   if (threadIdx.x == 0) { plan.pstart[P] = n; *plan.ticket = 0; }
   __syncthreads();
   const uint32_t total = block_exclusive_scan_lds(cnt, P, scratch);
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) plan.cstart[p] = cnt[p];
-  if (threadIdx.x == 0) plan.cstart[P] = total;
+  if (threadIdx.x == 0) {
+    plan.cstart[P] = total;
+    plan_split_totals(plan, s_srows, s_nsplit);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,14 +1063,19 @@ __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const
                                                       uint64_t chmax, uint32_t *bcur, AggPlan plan) {
   extern __shared__ uint32_t cnt[];  // [2P]
   __shared__ uint32_t scratch[17];
+  __shared__ unsigned long long s_srows;
+  __shared__ uint32_t s_nsplit;
   uint32_t *nbk = cnt, *nit = cnt + P;
-  if (threadIdx.x == 0 && plan.split_rows) *plan.split_rows = 0;
+  if (threadIdx.x == 0) { s_srows = 0; s_nsplit = 0; }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
     const uint64_t rows = prow[p];
     nbk[p] = pbk[p];
     nit[p] = (uint32_t)((rows + chmax - 1) / chmax);
-    if (nit[p] > 1 && plan.split_rows) atomicAdd((unsigned long long *)plan.split_rows, (unsigned long long)rows);
+    if (nit[p] > 1) {
+      atomicAdd(&s_srows, (unsigned long long)rows);
+      atomicAdd(&s_nsplit, 1u);
+    }
   }
   __syncthreads();
   const uint32_t buckets = block_exclusive_scan_lds(nbk, P, scratch);
@@ -1063,6 +1089,7 @@ __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const
     plan.pstart[P] = buckets;
     plan.cstart[P] = items;
     *plan.ticket = 0;
+    plan_split_totals(plan, s_srows, s_nsplit);
   }
 }
 
@@ -1112,85 +1139,109 @@ __global__ void __launch_bounds__(kCompactThreads) k_compact(GTable g, Schema s,
   __shared__ uint64_t s_excl;
   __shared__ uint32_t wcnt[kCompactItems][kCompactThreads / kWave];
   __shared__ uint32_t s_total;
-  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint64_t tile = s_tile;
+  if (g.S_dev) g.S = *g.S_dev;
   const uint64_t n = g.S + 1;
   const uint64_t TS = (uint64_t)kCompactThreads * kCompactItems;
-  const uint64_t tile_base = tile * TS;
+  // workgroups beyond the used tiles leave at once (the others take them all)
+  if ((uint64_t)blockIdx.x * TS >= n) return;
   const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
   const K *keys = (const K *)g.keys;
-  bool occ[kCompactItems];
-  uint32_t wpre[kCompactItems];
+  // Tiles in ticket order until the table's last one: the look-back waits
+  // only on lower tickets, all taken by workgroups already running.
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t tile_base = tile * TS;
+    if (tile_base >= n) break;
+    bool occ[kCompactItems];
+    uint32_t wpre[kCompactItems];
 #pragma unroll
-  for (int i = 0; i < kCompactItems; ++i) {
-    const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
-    bool oc = false;
-    if (slot < g.S) oc = keys[slot] != KeyTraits<K>::kEmpty;
-    else if (slot == g.S) oc = g.flags[FLAG_SENTINEL] != 0;
-    occ[i] = oc;
-    const uint64_t bal = __ballot(oc);
-    wpre[i] = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
-    if (lane == 0) wcnt[i][wave] = (uint32_t)__builtin_popcountll(bal);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int i = 0; i < kCompactItems; ++i)
-      for (int w = 0; w < kCompactThreads / kWave; ++w) { const uint32_t c = wcnt[i][w]; wcnt[i][w] = run; run += c; }
-    s_total = run;
-  }
-  __syncthreads();
-  if (wave == 0) {
-    const uint64_t ex = lookback_exclusive(desc, tile, s_total);
-    if (lane == 0) {
-      const uint64_t b0 = base ? *base : 0;
-      s_excl = b0 + ex;
-      if (tile_base + TS >= n) *count_out = b0 + ex + s_total;  // last tile
+    for (int i = 0; i < kCompactItems; ++i) {
+      const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
+      bool oc = false;
+      if (slot < g.S) oc = keys[slot] != KeyTraits<K>::kEmpty;
+      else if (slot == g.S) oc = g.flags[FLAG_SENTINEL] != 0;
+      occ[i] = oc;
+      const uint64_t bal = __ballot(oc);
+      wpre[i] = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
+      if (lane == 0) wcnt[i][wave] = (uint32_t)__builtin_popcountll(bal);
     }
-  }
-  __syncthreads();
-  const uint64_t excl = s_excl;
-#pragma unroll
-  for (int i = 0; i < kCompactItems; ++i) {
-    if (!occ[i]) continue;
-    const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
-    const uint64_t pos = excl + wcnt[i][wave] + wpre[i];
-    if (pos >= o.capacity) {  // output too small: the caller grows it or reports the count
-      atomicOr(&g.flags[FLAG_OVERFLOW], 1u);
-      continue;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int i = 0; i < kCompactItems; ++i)
+        for (int w = 0; w < kCompactThreads / kWave; ++w) { const uint32_t c = wcnt[i][w]; wcnt[i][w] = run; run += c; }
+      s_total = run;
     }
-    ((K *)o.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
-    #pragma unroll
-  for (int a = 0; a < kMaxAccs; ++a) {
-    if (a >= s.nacc) break;
-      const int kind = s.acc[a].kind;
-      switch (kind) {
-        case ACC_COUNT: case ACC_SUM_I:
-          ((uint64_t *)o.cols[a])[pos] = ((const uint64_t *)g.acc[a])[slot];
-          break;
-        case ACC_MIN_I32: case ACC_MAX_I32:
-          ((uint32_t *)o.cols[a])[pos] = ((const uint32_t *)g.acc[a])[slot];
-          break;
-        case ACC_SUM_F: {
-          int64_t L[kLimbs];
+    __syncthreads();
+    if (wave == 0) {
+      const uint64_t ex = lookback_exclusive(desc, tile, s_total);
+      if (lane == 0) {
+        const uint64_t b0 = base ? *base : 0;
+        s_excl = b0 + ex;
+        if (tile_base + TS >= n) *count_out = b0 + ex + s_total;  // last tile
+      }
+    }
+    __syncthreads();
+    const uint64_t excl = s_excl;
 #pragma unroll
-          for (int l = 0; l < kLimbs; ++l) L[l] = ((const int64_t *)g.acc[a])[(uint64_t)l * n + slot];
-          ((double *)o.cols[a])[pos] = fx_to_double(L);
-          break;
-        }
-        default: {  // ACC_MIN_I64 / ACC_MAX_I64 back to the column dtype
-          const int64_t v = ((const int64_t *)g.acc[a])[slot];
-          const int dt = s.acc[a].col_dtype;
-          if (dt == DT_U64) ((uint64_t *)o.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
-          else if (dt == DT_F64) ((uint64_t *)o.cols[a])[pos] = ordered_to_f64_bits(v);
-          else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)o.cols[a])[pos] = (uint32_t)v;
-          else ((int64_t *)o.cols[a])[pos] = v;
-          break;
+    for (int i = 0; i < kCompactItems; ++i) {
+      if (!occ[i]) continue;
+      const uint64_t slot = tile_base + (uint64_t)i * kCompactThreads + threadIdx.x;
+      const uint64_t pos = excl + wcnt[i][wave] + wpre[i];
+      if (pos >= o.capacity) {  // output too small: the caller grows it or reports the count
+        atomicOr(&g.flags[FLAG_OVERFLOW], 1u);
+        continue;
+      }
+      ((K *)o.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
+#pragma unroll
+      for (int a = 0; a < kMaxAccs; ++a) {
+        if (a >= s.nacc) break;
+        const int kind = s.acc[a].kind;
+        switch (kind) {
+          case ACC_COUNT: case ACC_SUM_I:
+            ((uint64_t *)o.cols[a])[pos] = ((const uint64_t *)g.acc[a])[slot];
+            break;
+          case ACC_MIN_I32: case ACC_MAX_I32:
+            ((uint32_t *)o.cols[a])[pos] = ((const uint32_t *)g.acc[a])[slot];
+            break;
+          case ACC_SUM_F: {
+            int64_t L[kLimbs];
+#pragma unroll
+            for (int l = 0; l < kLimbs; ++l) L[l] = ((const int64_t *)g.acc[a])[(uint64_t)l * n + slot];
+            ((double *)o.cols[a])[pos] = fx_to_double(L);
+            break;
+          }
+          default: {  // ACC_MIN_I64 / ACC_MAX_I64 back to the column dtype
+            const int64_t v = ((const int64_t *)g.acc[a])[slot];
+            const int dt = s.acc[a].col_dtype;
+            if (dt == DT_U64) ((uint64_t *)o.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
+            else if (dt == DT_F64) ((uint64_t *)o.cols[a])[pos] = ordered_to_f64_bits(v);
+            else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)o.cols[a])[pos] = (uint32_t)v;
+            else ((int64_t *)o.cols[a])[pos] = v;
+            break;
+          }
         }
       }
     }
+    __syncthreads();  // s_tile, wcnt and s_excl are rewritten by the next tile
   }
+}
+
+// Groups in a global table (occupied slots plus the empty-key slot), added
+// to *out: one atomic per wave.
+template <typename K>
+__global__ void k_count_groups(GTable g, uint64_t *out) {
+  if (g.S_dev) g.S = *g.S_dev;
+  const K *keys = (const K *)g.keys;
+  uint32_t c = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < g.S; i += (uint64_t)gridDim.x * blockDim.x)
+    c += keys[i] != KeyTraits<K>::kEmpty;
+  if (blockIdx.x == 0 && threadIdx.x == 0) c += g.flags[FLAG_SENTINEL] != 0;
+  const int lane = (int)__lane_id();
+  for (int o = kWave / 2; o > 0; o >>= 1) c += (uint32_t)__shfl((int)c, lane ^ o, kWave);
+  if (lane == 0 && c) atomicAdd((unsigned long long *)out, (unsigned long long)c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1442,9 +1493,10 @@ hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t s
 }
 
 hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st) {
-  const uint64_t n = (g.S + 1) * kLimbs;
-  if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, g, s);
-  else hipLaunchKernelGGL(k_gtable_init<uint64_t>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, g, s);
+  // sized for the allocation; with S_dev set most of it may go unused
+  const unsigned grid = grid_for((g.S + 1) * kLimbs, 256, g.S_dev ? 2048 : 8192);
+  if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid), dim3(256), 0, st, g, s);
+  else hipLaunchKernelGGL(k_gtable_init<uint64_t>, dim3(grid), dim3(256), 0, st, g, s);
   return hipGetLastError();
 }
 
@@ -1735,6 +1787,13 @@ hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax,
   return hipGetLastError();
 }
 
+hipError_t launch_count_groups(int key_bytes, const GTable &g, uint64_t *out, hipStream_t st) {
+  const unsigned grid = grid_for(g.S + 1, 256, 2048);
+  if (key_bytes == 4) hipLaunchKernelGGL(k_count_groups<uint32_t>, dim3(grid), dim3(256), 0, st, g, out);
+  else hipLaunchKernelGGL(k_count_groups<uint64_t>, dim3(grid), dim3(256), 0, st, g, out);
+  return hipGetLastError();
+}
+
 uint64_t compact_tiles(uint64_t slots_plus_one) {
   const uint64_t TS = (uint64_t)kCompactThreads * kCompactItems;
   return (slots_plus_one + TS - 1) / TS;
@@ -1742,13 +1801,14 @@ uint64_t compact_tiles(uint64_t slots_plus_one) {
 
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base) {
-  const uint64_t tiles = compact_tiles(g.S + 1);
+  // workgroups loop over the tiles (with S_dev set, over the used ones only)
+  const unsigned grid = (unsigned)std::min<uint64_t>(compact_tiles(g.S + 1), 2048);
   if (key_bytes == 4)
-    hipLaunchKernelGGL(k_compact<uint32_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
-                       count_out, base);
+    hipLaunchKernelGGL(k_compact<uint32_t>, dim3(grid), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket, count_out,
+                       base);
   else
-    hipLaunchKernelGGL(k_compact<uint64_t>, dim3((unsigned)tiles), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket,
-                       count_out, base);
+    hipLaunchKernelGGL(k_compact<uint64_t>, dim3(grid), dim3(kCompactThreads), 0, st, g, s, o, desc, ticket, count_out,
+                       base);
   return hipGetLastError();
 }
 

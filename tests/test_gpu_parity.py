@@ -114,8 +114,8 @@ def test_two_pass_partitioning(ctx, key_bits, hot):
 def test_split_partitions_without_key_sample(ctx):
     # 40% of the rows on each of two keys (partitions 0 and 4 of 8) and a
     # group-count hint, so no key sample is taken: both partitions are split
-    # into chunks, the global table sized for no split overflows with their
-    # groups, and the call runs again with a full-size table
+    # into chunks, and the work plan sizes the global table for their groups
+    # on the GPU, so one attempt does
     n = 1 << 20
     rng = np.random.default_rng(81)
     k = rng.integers(0, 1 << 62, size=20000, dtype=np.uint64)[rng.integers(0, 20000, size=n)]
@@ -125,8 +125,22 @@ def test_split_partitions_without_key_sample(ctx):
     v = rng.integers(-99, 100, size=n).astype(np.int32)
     got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, est_groups=20002)
     plan = ctx.last_plan()
-    assert plan["partition_bits"] == 3 and plan["attempts"] == 2
+    assert plan["partition_bits"] == 3 and plan["attempts"] == 1
     check_equal(got, O.groupby(k, [v], AGGS4))
+
+
+def test_spills_overflow_planned_global_table(ctx):
+    # 20000 groups over 4 partitions with 256-slot LDS tables: nearly every
+    # group spills to the global table, which the work plan sized for split
+    # partitions only (none here); it overflows and the call runs again with
+    # the full-size table
+    n = 1 << 20
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=20000, seed=83, salt=5)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=84)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, partition_bits=2, table_slots=256, est_groups=20000)
+    plan = ctx.last_plan()
+    assert plan["partition_bits"] == 2 and plan["attempts"] == 2
+    check_equal(got, O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4))
 
 
 def test_partitioned_small_tables_many_partitions(ctx):
