@@ -155,6 +155,7 @@ struct Plan {
   int scatter_layout = 1;  // 1 = partition-contiguous, 2 = per-workgroup buckets
   int scatter_wc = 0;      // bucket layout of records written as whole cache lines
   int scatter_p1_bits = 0; // > 0: two passes, the first into 2^scatter_p1_bits partitions
+  int max_pbits = 12;      // the largest fanout this query's layouts reach
   // re-planning: the options the plan came from, and whether the fanout was
   // the tuner's (so a too-small estimate may raise it)
   tga_options opt{};

@@ -346,6 +346,8 @@ int main(int argc, char **argv) {
   // first-pass partition is cut into several work items
   cases.push_back({"two_pass_u32_p4096", 32, keys_u32(3 * N, 300000, 91), {col_i32(3 * N, -99, 99, 92)}, {TGA_I32},
                    {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 12, 0, 300000)});
+  cases.push_back({"two_pass_u32_p8192", 32, keys_u32(3 * N, 300000, 95), {col_i32(3 * N, -99, 99, 96)}, {TGA_I32},
+                   {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 13, 0, 300000)});
   {
     std::vector<uint64_t> k = keys_u64(7 * N / 2, 100000, 93);
     for (size_t i = 0; i < k.size(); i += 2) k[i] = 1234567;

@@ -94,20 +94,21 @@ def test_write_combining(ctx, wc, key_bits):
     assert got["n_groups"] == d
 
 
+@pytest.mark.parametrize("bits", [12, 13])
 @pytest.mark.parametrize("key_bits,hot", [(32, False), (64, False), (64, True)])
-def test_two_pass_partitioning(ctx, key_bits, hot):
-    # 2^12 partitions: records written whole-line into 2^8 partitions, each
-    # split again into 16; with a hot key holding half the rows its
-    # first-pass partition spans several second-pass work items
+def test_two_pass_partitioning(ctx, key_bits, hot, bits):
+    # 2^12 (2^13) partitions: records written whole-line into 2^8 (2^9)
+    # partitions, each split again into 16; with a hot key holding half the
+    # rows its first-pass partition spans several second-pass work items
     n, d = (1 << 22) + 333, 1 << 19
     gen = tga.GEN_KEYS_U64 if key_bits == 64 else tga.GEN_KEYS_U32
     k = ctx.generate(gen, n, a=d, seed=51, salt=4).to_numpy()
     if hot:
         k[::2] = 987654321
     v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=52).to_numpy()
-    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, partition_bits=12)
+    got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, partition_bits=bits)
     plan = ctx.last_plan()
-    assert plan["partition_bits"] == 12 and plan["write_combine"] == 1
+    assert plan["partition_bits"] == bits and plan["write_combine"] == 1
     check_equal(got, O.groupby(k, [v], AGGS4))
 
 
