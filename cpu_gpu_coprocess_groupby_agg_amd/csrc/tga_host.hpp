@@ -168,16 +168,24 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint6
               const tga_options *opt);
 
 // Cardinality estimate from a strided key sample (GEE estimator,
-// Charikar et al. 2000).  Needs the context lock.
+// Charikar et al. 2000, or the equal-frequency model), replaced by an exact
+// count over a 2^-10 hash slice of all keys when the sample leaves the
+// count open.  Needs the context lock.
 tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est,
                            double *top_share = nullptr);
 // Distinct-count estimate of n rows from a sample of m keys (sorted in place).
 // *top_share (optional): the largest share of the sample one key takes.
-uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_share = nullptr);
+// *reliable (optional): false when the sample, mostly singletons, leaves
+// the group count open (see estimate_groups).
+uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_share = nullptr,
+                              bool *reliable = nullptr);
 // The same from the sample's counts: d distinct keys, f1 of them seen once,
 // the most frequent seen top times.
 uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n,
-                             double *top_share = nullptr);
+                             double *top_share = nullptr, bool *reliable = nullptr);
+// An unreliable sample estimate over at least this many rows is replaced by
+// an exact count over a hash slice of the keys.
+constexpr uint64_t kSliceMinRows = 1ULL << 22;
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

@@ -114,6 +114,13 @@ uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 // adds the group count of g (occupied slots + the empty-key slot) to *out
 hipError_t launch_count_groups(int key_bytes, const GTable &g, uint64_t *out, hipStream_t st);
+// Exact distinct count of the keys whose partition hash has its top
+// kSliceBits bits 0 (out[0]; out[1] != 0: the set overflowed, a lower bound).
+// set: slice_set_slots(n) keys of workspace (filled here).
+constexpr int kSliceBits = 10;
+uint32_t slice_set_slots(uint64_t n);
+hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
+                                 unsigned grid, hipStream_t st);
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 // out[0..3] = {distinct keys, keys seen once, largest count of one key,

@@ -1387,6 +1387,99 @@ __global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *o
 // chosen against the hash can do that), the counts are incomplete.
 constexpr uint32_t kSampleMax = 16384, kSamplePer = kSampleMax / 1024, kSampleSlots = 8192;
 
+// Cardinality from a hash slice: the distinct keys whose partition hash
+// (key_partition's top kSliceBits bits) is 0, counted exactly over all n
+// rows, so D ~ count * 2^kSliceBits with relative error ~ (D/1024)^-1/2
+// whatever the key frequencies.  Each workgroup takes a contiguous range of
+// rows and dedups its slice keys in an LDS set (a hot key only reads it);
+// then it inserts them into the global set, counting new keys with one
+// atomic per wave.  Past half full, a workgroup's keys go to the global set
+// directly.  out[0] += the count; out[1] != 0: the global set overflowed and
+// the count is a lower bound.  The empty-marker key is not counted.
+constexpr uint32_t kSliceLds = 4096;
+
+template <typename K>
+__device__ __forceinline__ uint32_t slice_set_insert(K *set, uint32_t slots, K k, unsigned long long *out) {
+  uint32_t s = (uint32_t)(((uint64_t)key_slot_hash((uint64_t)k) * slots) >> 32);
+  for (uint32_t i = 0; i < 256; ++i) {
+    const K cur = set[s];
+    if (cur == k) return 0;
+    if (cur == KeyTraits<K>::kEmpty) {
+      const K prev = atomicCAS(&set[s], KeyTraits<K>::kEmpty, k);
+      if (prev == KeyTraits<K>::kEmpty) return 1;
+      if (prev == k) return 0;
+    }
+    s = (s + 1 == slots) ? 0 : s + 1;
+  }
+  atomicOr(&out[1], 1ULL);
+  return 0;
+}
+
+// VEC: keys 16-byte aligned, read 16 bytes a lane, 4 loads in flight.
+template <typename K, bool VEC>
+__global__ void __launch_bounds__(1024) k_slice_distinct(const K *__restrict__ keys, uint64_t n, K *set,
+                                                         uint32_t slots, unsigned long long *out) {
+  __shared__ K lset[kSliceLds];
+  __shared__ uint32_t s_fill;
+  for (uint32_t i = threadIdx.x; i < kSliceLds; i += blockDim.x) lset[i] = KeyTraits<K>::kEmpty;
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  uint32_t added = 0;
+  auto visit = [&](K k) {
+    if (key_partition((uint64_t)k, kSliceBits) != 0 || k == KeyTraits<K>::kEmpty) return;
+    if (*(volatile uint32_t *)&s_fill < kSliceLds / 2) {
+      uint32_t s = key_slot_hash((uint64_t)k) & (kSliceLds - 1);
+      for (int i = 0; i < 64; ++i) {
+        const K cur = lset[s];
+        if (cur == k) return;
+        if (cur == KeyTraits<K>::kEmpty) {
+          const K prev = atomicCAS(&lset[s], KeyTraits<K>::kEmpty, k);
+          if (prev == KeyTraits<K>::kEmpty) atomicAdd(&s_fill, 1u);
+          if (prev == KeyTraits<K>::kEmpty || prev == k) return;
+        }
+        s = (s + 1) & (kSliceLds - 1);
+      }
+    }
+    added += slice_set_insert<K>(set, slots, k, out);
+  };
+  constexpr uint32_t E = VEC ? 16 / sizeof(K) : 1;  // keys per load
+  constexpr int U = 4;
+  const uint64_t nv = n / E;
+  const uint64_t per = (nv + gridDim.x - 1) / gridDim.x;
+  const uint64_t b = blockIdx.x * per, e = min(nv, b + per);
+  for (uint64_t r0 = b + threadIdx.x; r0 < e; r0 += (uint64_t)U * blockDim.x) {
+    K ks[U][E];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t r = r0 + (uint64_t)u * blockDim.x;
+      if (r >= e) continue;
+      if constexpr (VEC) {
+        const uint4 q = ((const uint4 *)keys)[r];
+        __builtin_memcpy(ks[u], &q, 16);
+      } else {
+        ks[u][0] = keys[r];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (r0 + (uint64_t)u * blockDim.x >= e) continue;
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) visit(ks[u][j]);
+    }
+  }
+  if (VEC && blockIdx.x == 0)  // rows past the last whole vector
+    for (uint64_t r = nv * E + threadIdx.x; r < n; r += blockDim.x) visit(keys[r]);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kSliceLds; i += blockDim.x) {
+    const K k = lset[i];
+    if (k != KeyTraits<K>::kEmpty) added += slice_set_insert<K>(set, slots, k, out);
+  }
+  const int lane = (int)__lane_id();
+  for (int o = kWave / 2; o > 0; o >>= 1) added += (uint32_t)__shfl((int)added, lane ^ o, kWave);
+  if (lane == 0 && added) atomicAdd(&out[0], (unsigned long long)added);
+}
+
+
 __global__ void __launch_bounds__(1024) k_sample_stats(const uint64_t *__restrict__ smp, uint32_t m, uint64_t *out) {
   extern __shared__ __align__(16) char smem[];
   uint64_t *tk = (uint64_t *)smem;               // [kSampleSlots]
@@ -1838,6 +1931,32 @@ hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint
   e = set_lds_limit<uint64_t>((const void *)&k_sample_stats, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_sample_stats, dim3(1), dim3(1024), lds, st, (const uint64_t *)smp, m, out);
+  return hipGetLastError();
+}
+
+uint32_t slice_set_slots(uint64_t n) {
+  uint64_t S = 4096;
+  while (S < 4 * (n >> kSliceBits) && S < (1ULL << 31)) S <<= 1;
+  return (uint32_t)S;
+}
+
+hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
+                                 unsigned grid, hipStream_t st) {
+  if (slots == 0 || (slots & (slots - 1)) || grid == 0) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u64(out, 2, 0, st);
+  if (e != hipSuccess) return e;
+  const bool vec = ((uintptr_t)keys & 15) == 0;
+  if (key_bytes == 4) {
+    if ((e = launch_fill_u32((uint32_t *)set, slots, 0xFFFFFFFFu, st)) != hipSuccess) return e;
+    auto *fn = vec ? &k_slice_distinct<uint32_t, true> : &k_slice_distinct<uint32_t, false>;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), 0, st, (const uint32_t *)keys, n, (uint32_t *)set, slots,
+                       (unsigned long long *)out);
+  } else {
+    if ((e = launch_fill_u64((uint64_t *)set, slots, ~0ULL, st)) != hipSuccess) return e;
+    auto *fn = vec ? &k_slice_distinct<uint64_t, true> : &k_slice_distinct<uint64_t, false>;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), 0, st, (const uint64_t *)keys, n, (uint64_t *)set, slots,
+                       (unsigned long long *)out);
+  }
   return hipGetLastError();
 }
 

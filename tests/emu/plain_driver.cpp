@@ -15,11 +15,15 @@
 #include <vector>
 
 #include "../../include/tga_groupby.h"
+#include "../../cpu_gpu_coprocess_groupby_agg_amd/csrc/tga_common.hpp"
 #include "hip/hip_runtime.h"
 
 namespace tga {
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st);
+uint32_t slice_set_slots(uint64_t n);
+hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
+                                 unsigned grid, hipStream_t st);
 }
 
 extern "C" int64_t tga_o_groupby(const void *keys, int key_bytes, int64_t n, const void *const *cols,
@@ -240,6 +244,73 @@ int check_sample_stats() {
   return bad;
 }
 
+// The hash-slice distinct count against the distinct keys of the slice
+// counted on the host: hot keys in and out of the slice, the empty-marker
+// key (not counted), one workgroup whose LDS set passes half full (its keys
+// then go to the global set directly), and a global set too small for the
+// slice (the overflow word is set); row counts that leave a partial last
+// 16-byte load, and keys not 16-byte aligned (one key a load).
+int check_slice_distinct() {
+  int bad = 0;
+  struct S { const char *name; std::vector<uint64_t> k; int kb; unsigned grid; uint32_t slots; int off = 0; };
+  std::vector<S> sets;
+  sets.push_back({"uniform_u64", keys_u64(400001, 300000, 101), 8, 6, 0});
+  sets.push_back({"u32", keys_u32(300003, 1u << 31, 102), 4, 5, 0});
+  sets.push_back({"u32_unaligned", keys_u32(300001, 1u << 31, 105), 4, 3, 0, 1});
+  sets.push_back({"u64_unaligned", keys_u64(200000, 1ULL << 40, 106), 8, 3, 0, 1});
+  {
+    std::vector<uint64_t> k = keys_u64(200000, 150000, 103);
+    uint64_t hot_in = 0, hot_out = 0;
+    for (uint64_t c = 1; !hot_in || !hot_out; ++c)
+      (tga::key_partition(c, 10) == 0 ? hot_in : hot_out) = c;
+    for (size_t i = 0; i < k.size(); i += 3) k[i] = hot_in;
+    for (size_t i = 1; i < k.size(); i += 5) k[i] = hot_out;
+    for (size_t i = 2; i < k.size(); i += 97) k[i] = ~0ULL;
+    sets.push_back({"hot_and_marker", k, 8, 4, 0});
+  }
+  sets.push_back({"lds_past_half", keys_u64(2600000, 1ULL << 40, 104), 8, 1, 0});
+  {
+    std::vector<uint64_t> k;
+    for (uint64_t c = 1; k.size() < 6000; ++c)
+      if (tga::key_partition(c * 0x9E3779B97F4A7C15ULL, 10) == 0) k.push_back(c * 0x9E3779B97F4A7C15ULL);
+    sets.push_back({"set_overflow", k, 8, 2, 4096});
+  }
+  for (const S &c : sets) {
+    const uint64_t n = c.k.size();
+    std::vector<uint64_t> in;
+    const uint64_t empty = c.kb == 4 ? 0xFFFFFFFFULL : ~0ULL;
+    for (uint64_t v : c.k)
+      if (tga::key_partition(v, 10) == 0 && v != empty) in.push_back(v);
+    std::sort(in.begin(), in.end());
+    const uint64_t want = (uint64_t)(std::unique(in.begin(), in.end()) - in.begin());
+    const uint32_t slots = c.slots ? c.slots : tga::slice_set_slots(n);
+    void *dk = nullptr, *dset = nullptr;
+    uint64_t *dout = nullptr, got[2] = {0, 0};
+    hipMalloc(&dk, (n + 1) * 8);
+    hipMalloc(&dset, (size_t)slots * 8);
+    hipMalloc(&dout, 16);
+    char *kp = (char *)dk + (size_t)c.off * c.kb;
+    if (c.kb == 4) {
+      std::vector<uint32_t> k32(c.k.begin(), c.k.end());
+      hipMemcpy(kp, k32.data(), n * 4, hipMemcpyHostToDevice);
+    } else {
+      hipMemcpy(kp, c.k.data(), n * 8, hipMemcpyHostToDevice);
+    }
+    const hipError_t e = tga::launch_slice_distinct(c.kb, kp, n, dset, slots, dout, c.grid, nullptr);
+    hipMemcpy(got, dout, sizeof got, hipMemcpyDeviceToHost);
+    hipFree(dk);
+    hipFree(dset);
+    hipFree(dout);
+    const bool over = c.slots != 0;
+    const bool ok = e == hipSuccess && (over ? got[1] != 0 && got[0] <= want : got[1] == 0 && got[0] == want);
+    std::printf("slice_distinct_%s: %llu rows, slice keys %llu/%llu, overflow %llu %s\n", c.name,
+                (unsigned long long)n, (unsigned long long)got[0], (unsigned long long)want,
+                (unsigned long long)got[1], ok ? "ok" : "FAIL");
+    bad += !ok;
+  }
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -366,6 +437,10 @@ int main(int argc, char **argv) {
   tga_context_destroy(ctx);
   if (filter.empty() || std::string("sample_stats").find(filter) != std::string::npos) {
     bad += check_sample_stats();
+    ++ran;
+  }
+  if (filter.empty() || std::string("slice_distinct").find(filter) != std::string::npos) {
+    bad += check_slice_distinct();
     ++ran;
   }
   std::printf(bad ? "FAIL %d of %d\n" : "OK %d\n", bad ? bad : ran, ran);
