@@ -50,8 +50,9 @@ def parse():
                     help="skip the reference-workload section (its driver's uniform / Zipf / self-similar keys)")
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
-    ap.add_argument("--only", choices=["all", "headline", "copy"], default="all",
-                    help="headline: only the 2^28-row group-by (profiling runs); copy: only the copy reference")
+    ap.add_argument("--only", choices=["all", "headline", "copy", "workloads"], default="all",
+                    help="headline: only the 2^28-row group-by (profiling runs); copy: only the copy reference; "
+                         "workloads: only the reference-workload section")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that fill roofline.traffic")
     return ap.parse_args()
@@ -427,6 +428,9 @@ def main():
     log(f"copy reference: {copy}")
     if args.only == "copy":
         print(json.dumps({"copy": copy}))
+        return
+    if args.only == "workloads":
+        print(json.dumps({"workloads": workloads(ctx, tga, args)}))
         return
     h, keys, vals = headline(ctx, tga, args)
     log(f"headline: {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s, kernels {h['kernels_ms_per_step']}")
