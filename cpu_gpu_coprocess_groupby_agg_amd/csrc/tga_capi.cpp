@@ -591,8 +591,12 @@ static tga_status bucket_bufs(tga_context *ctx, uint64_t cap, uint32_t P, DBuf &
 // Two-pass partitioning into 2^plan.pbits partitions (records, bucket
 // layout): the first pass into 2^plan.scatter_p1_bits partitions in the
 // buffers *1 of pb, then each of its partitions split again into the final
-// pool of pb.bb.  Its list entries are cut into work items of this many.
-constexpr uint32_t kPass2Entries = 1024;
+// pool of pb.bb.  Its list entries are cut into work items of 1024 entries
+// (2^18 rows) below 2^8 first-pass partitions, 512 from there (2^28 rows:
+// at 2^6 first-pass partitions 1.15 ms against 1.19 with 512; at 2^9, 1.23
+// against 1.27 ms, and 1.53 against 1.71 ms under Zipf(0.9) keys, whose hot
+// partitions spread over more items).
+static uint32_t pass2_entries(uint32_t P1) { return P1 >= 256 ? 512 : 1024; }
 
 static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s,
                                      uint64_t m, const Plan &plan, unsigned grid, PartBufs &pb, uint64_t &slots) {
@@ -616,23 +620,24 @@ static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const
   }));
   AggPlan ap1 = part_plan(pb.plan1, P1);
   ap1.blist = b1bufs.blist;
+  const uint32_t per_item = pass2_entries(P1);
   uint32_t *istart = pb.items1.as<uint32_t>(), *ticket = istart + P1 + 1;
   TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] {
     hipError_t e = launch_bucket_lists(b1bufs, P1, ~0ULL >> 1, ap1, st);
     if (e != hipSuccess) return e;
-    return launch_pass2_plan(ap1.pstart, P1, kPass2Entries, istart, ticket, st);
+    return launch_pass2_plan(ap1.pstart, P1, per_item, istart, ticket, st);
   }));
-  // pass 2: at most P1 + cap1 / kPass2Entries work items, each leaving up to
+  // pass 2: at most P1 + cap1 / per_item work items, each leaving up to
   // 2^b2 buckets open, and a reserve per wave
   const unsigned grid2 = (unsigned)(ctx->cus + 7) / 8 * 8;  // every CU, a multiple of the XCD count
-  const uint64_t items_max = P1 + cap1 / kPass2Entries + 1;
+  const uint64_t items_max = P1 + cap1 / per_item + 1;
   const uint64_t cap = (m + kBucketRows - 1) / kBucketRows + items_max * (1ULL << b2) + (uint64_t)grid2 * 1024 + 1;
   if (tga_status e = bucket_bufs(ctx, cap, P, pb.heads, pb.meta, pb.bb)) return e;
   slots = cap << kBucketShift;
   if (!pb.pkeys.alloc(ctx, slots * row_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   TGA_HIP(ctx, timed(ctx, "part_scatter2", [&] {
     return launch_part_scatter2(q.key_bytes, pb.pkeys1.p, b1bufs.blist, ap1.pstart, istart, ticket, P1, plan.pbits,
-                                kPass2Entries, b2, pb.pkeys.p, pb.bb, grid2, st);
+                                per_item, b2, pb.pkeys.p, pb.bb, grid2, st);
   }));
   return TGA_OK;
 }
