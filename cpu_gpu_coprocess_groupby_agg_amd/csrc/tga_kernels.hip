@@ -74,15 +74,30 @@ __global__ void k_gtable_init(GTable g, Schema s) {
 // (a)/(c) LDS aggregation
 // ---------------------------------------------------------------------------
 
+// Segment of work item `item`: the largest p with cstart[p] <= item (cstart
+// is the exclusive prefix of the items per segment).  While no segment
+// before it was split or empty, item i is segment i: two independent loads
+// settle that, where the search takes log2(P) dependent ones.
+__device__ __forceinline__ uint32_t plan_segment(const AggPlan &plan, uint32_t item) {
+  if (item < plan.P && plan.cstart[item] == item && plan.cstart[item + 1] == item + 1) return item;
+  uint32_t lo = 0, hi = plan.P;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (plan.cstart[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // Persistent: workgroups pull work items from plan.ticket until none remain.
 // Item -> (segment p, slice j of nch) through plan.cstart; rows of segment p
-// are [pstart[p], pstart[p+1]).
+// are [pstart[p], pstart[p+1]).  Thread 0 takes the next item's ticket and
+// finds its segment while the other waves finish the current item's rows.
 template <typename K, bool FX, int NC, int VW, bool FOLD, uint32_t SIG>
 __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema s, AggPlan plan, Layout L,
                                               GTable g, AccProg ap, int nbits, int max_probe, int emit, OutDesc od,
                                               uint64_t *emit_count, uint32_t *ovf) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ uint32_t s_item, s_sentinel;
+  __shared__ uint32_t s_item, s_seg, s_sentinel;
   __shared__ uint32_t scratch[1024 / kWave + 1];
   __shared__ uint64_t s_base;
   // descriptors the out-of-line global-table path reads
@@ -96,17 +111,17 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
   }
   LdsTable<K> t{smem, L, &s_sentinel};
   const uint32_t total = plan.cstart[plan.P];
+  auto take_item = [&] {  // thread 0
+    const uint32_t it = atomicAdd(plan.ticket, 1u);
+    s_item = it;
+    s_seg = it < total ? plan_segment(plan, it) : 0u;
+  };
+  if (threadIdx.x == 0) take_item();
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(plan.ticket, 1u);
     __syncthreads();
     const uint32_t item = wave_uniform(s_item);
     if (item >= total) break;
-    uint32_t lo = 0, hi = plan.P;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (plan.cstart[mid] <= item) lo = mid; else hi = mid;
-    }
-    const uint32_t p = lo;
+    const uint32_t p = wave_uniform(s_seg);
     const uint32_t j = item - plan.cstart[p], nch = plan.cstart[p + 1] - plan.cstart[p];
     const uint64_t sb = plan.pstart[p], se = plan.pstart[p + 1], len = se - sb;
     const uint64_t b = sb + len * j / nch, e = sb + len * (j + 1) / nch;
@@ -131,6 +146,8 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
       agg_rows_int<K, NC, VW, FOLD, SIG>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
                                          abandon ? s_abort : nullptr, nullptr, bl);
     }
+    // every wave has read s_item / s_seg (before the barrier after lds_init)
+    if (threadIdx.x == 0) take_item();
     __syncthreads();
     if (abandon && s_abort[0]) {
       if (threadIdx.x == 0) {
@@ -147,7 +164,8 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     } else {
       lds_flush<K>(t, s, g);
     }
-    __syncthreads();
+    // the barrier at the top of the loop keeps the next lds_init behind
+    // every read of this table
   }
 }
 
