@@ -288,15 +288,25 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     tfull &= ~63u;
     uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
+    const bool recs = o.scatter_layout != 1 && o.write_combine != 2 && copy_as_records(q.schema);
     int pbits = o.partition_bits;
     if (pbits <= 0) {
       const uint64_t P = next_pow2((est * 2 + (uint64_t)T - 1) / (uint64_t)T);
       pbits = std::max(1, ilog2(P));
+      // Tables a quarter full where the one-pass write-combining scatter
+      // still reaches that fanout (2^28 rows of u32 keys over 2^20 groups:
+      // aggregation 1.00 ms at 2^8 partitions, 40% full, 0.63 ms at 2^9).
+      // Past it, the second pass costs more than the fuller tables (2^21
+      // groups: 2.25 ms at 2^9, 2.79 ms at 2^10), and among two-pass fanouts
+      // a quarter is not reliably faster than a half.
+      if (recs) {
+        const int b4 = std::max(1, ilog2(next_pow2((est * 4 + (uint64_t)T - 1) / (uint64_t)T)));
+        if (b4 > pbits && scatter_wc_tile(q.key_bytes, b4, lds_max)) pbits = b4;
+      }
     }
     // 2^12 partitions at most, 2^13 where two passes of records reach them
     // (2^28 rows over 26.8M groups: 2^12 partitions fill their tables to
     // 65%, and the aggregation takes 2.7 ms against 1.2 at 2^13)
-    const bool recs = o.scatter_layout != 1 && o.write_combine != 2 && copy_as_records(q.schema);
     p.max_pbits = recs && scatter_wc_tile(q.key_bytes, 13 - kTwoPassSubBits, lds_max) ? 13 : 12;
     pbits = std::min(pbits, p.max_pbits);
     p.pbits = pbits;
