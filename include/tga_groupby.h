@@ -70,7 +70,10 @@ typedef enum tga_path { TGA_PATH_AUTO = 0, TGA_PATH_DIRECT = 1, TGA_PATH_PARTITI
 
 /* Every field 0 = chosen by the autotuner from (rows, estimated groups). */
 typedef struct tga_options {
-  int64_t est_groups;   /* cardinality hint; 0 = estimate from a sample */
+  int64_t est_groups;   /* cardinality hint; 0 = estimate from a 16384-key sample,
+                           or, when the sample bounds the count only loosely, from
+                           an exact count over a 2^-10 hash slice of all keys (one
+                           read of the key column) */
   int32_t path;         /* tga_path */
   int32_t partition_bits; /* log2 of the partition fanout */
   int32_t table_slots;  /* LDS hash-table slots per workgroup */
