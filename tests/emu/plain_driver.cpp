@@ -22,6 +22,8 @@ namespace tga {
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st);
 uint32_t slice_set_slots(uint64_t n);
+uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n, double *top_share,
+                             bool *reliable);
 hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
                                  unsigned grid, hipStream_t st);
 }
@@ -311,6 +313,38 @@ int check_slice_distinct() {
   return bad;
 }
 
+// When the 16384-key sample is trusted and when the hash-slice count is
+// asked for instead, on the sample counts of typical inputs (expected counts
+// of a 16384-row sample; rows n):
+// - 2^20 equally frequent keys in 2^28 rows: ~128 repeated pairs, so d 16256,
+//   f1 16128; the estimate is ~2^20, but every unseen row could be a key of
+//   its own, so the count is open (the slice count runs);
+// - 1000 keys in 2^28 rows: all seen, none once: exact, trusted;
+// - a sample of 16384 distinct keys from 2^20 rows: D ~ n, trusted;
+// - self-similar(0.9) draws over 838,860 keys in 2^23 rows: 90% of the
+//   sample on 10% of the keys (d 15100, f1 14000, top 4) reads ~100K, open;
+// - the whole input sampled (m = n): exact, trusted.
+int check_estimate_rule() {
+  struct E { const char *name; uint64_t d, f1, top, m, n; bool reliable; uint64_t lo, hi; };
+  const E cases[] = {
+      {"uniform_2p20_keys", 16256, 16128, 2, 16384, 1ULL << 28, false, 900000, 1200000},
+      {"covered_1000_keys", 1000, 0, 30, 16384, 1ULL << 28, true, 1000, 1000},
+      {"all_distinct", 16384, 16384, 1, 16384, 1ULL << 20, true, 1ULL << 20, 1ULL << 20},
+      {"self_similar_tail", 15100, 14000, 4, 16384, 1ULL << 23, false, 0, ~0ULL},
+      {"whole_input", 700, 300, 9, 5000, 5000, true, 700, 700},
+  };
+  int bad = 0;
+  for (const E &c : cases) {
+    bool rel = !c.reliable;
+    const uint64_t est = tga::estimate_from_stats(c.d, c.f1, c.top, c.m, c.n, nullptr, &rel);
+    const bool ok = rel == c.reliable && est >= c.lo && est <= c.hi;
+    std::printf("estimate_rule_%s: estimate %llu, reliable %d %s\n", c.name, (unsigned long long)est, (int)rel,
+                ok ? "ok" : "FAIL");
+    bad += !ok;
+  }
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -441,6 +475,10 @@ int main(int argc, char **argv) {
   }
   if (filter.empty() || std::string("slice_distinct").find(filter) != std::string::npos) {
     bad += check_slice_distinct();
+    ++ran;
+  }
+  if (filter.empty() || std::string("estimate_rule").find(filter) != std::string::npos) {
+    bad += check_estimate_rule();
     ++ran;
   }
   std::printf(bad ? "FAIL %d of %d\n" : "OK %d\n", bad ? bad : ran, ran);
