@@ -427,6 +427,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
   b.okeys.resize(n);
   b.ocols.clear();
   b.ocols.resize(n);
+  b.graph_out.assign(n, 1);
   std::vector<std::vector<BItem>> lists(kItemTypes);
   size_t lds_need = 0;
   const uint64_t total_done = (uint64_t)n * kItemTypes;
@@ -581,6 +582,14 @@ tga_status batch_finalize(BatchExec &b, std::vector<uint64_t> &groups) {
       uint64_t G = 0;
       tga_status st = run_plain(ctx, b.spec[i], b.keys[i], b.rows[i], p, &po, nullptr, nullptr, 0, &G);
       if (st) return st;
+      // The device query record (and a captured graph replaying it) still
+      // targets the batch's own output buffers: they stay owned by the batch
+      // until it is destroyed, and the rerun's result is kept beside them.
+      if (i < (int32_t)b.graph_out.size() && b.graph_out[i]) {
+        b.bufs.push_back(std::move(b.okeys[i]));
+        for (int a = 0; a < b.spec[i].schema.nacc; ++a) b.bufs.push_back(std::move(b.ocols[i][a]));
+        b.graph_out[i] = 0;
+      }
       b.okeys[i] = std::move(po.keys);
       for (int a = 0; a < b.spec[i].schema.nacc; ++a) b.ocols[i][a] = std::move(po.cols[a]);
       b.cap[i] = po.capacity;

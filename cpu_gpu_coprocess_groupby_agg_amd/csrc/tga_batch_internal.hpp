@@ -57,6 +57,10 @@ struct BatchExec {
   std::vector<DBuf> okeys;                         // per query output keys
   std::vector<std::array<DBuf, kMaxAccs>> ocols;   // per query output columns
   std::vector<uint64_t> cap;
+  // 1: okeys/ocols of the query are the buffers the batch's kernels write
+  // (referenced by the device query records and the captured graph); 0: they
+  // hold a rerun's result and nothing on the device points at them
+  std::vector<uint8_t> graph_out;
   uint32_t nitems = 0;
   size_t lds = 0;
   unsigned grid = 0;

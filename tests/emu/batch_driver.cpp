@@ -25,7 +25,9 @@ extern "C" int64_t tga_o_groupby(const void *keys, int key_bytes, int64_t n, con
     }                                                                                 \
   } while (0)
 
-struct Spec { int64_t n; uint64_t d; int bits; bool hint; };
+// est > 0: the hint passed instead of d (far below the truth: the query's
+// table overflows and it is rerun alone, at every replay)
+struct Spec { int64_t n; uint64_t d; int bits; bool hint; int64_t est = 0; };
 
 int main(int argc, char **argv) {
   const int mode = argc > 1 ? std::atoi(argv[1]) : 0;
@@ -35,6 +37,7 @@ int main(int argc, char **argv) {
       {1 << 18, 1 << 16, 32, true},  {1 << 20, 1 << 18, 32, true}, {300000, 100000, 64, true},
       {1 << 17, 30, 64, false},      {0, 1, 32, true},           {1, 1, 32, true},
       {200000, 50, 32, false},       {100000, 90000, 32, false}, {70000, 64, 32, true},
+      {1 << 18, 100000, 32, true, 10}, {20000, 6000, 64, true, 10},
   };
   tga_context *ctx = nullptr;
   if (tga_context_create(0, 0, &ctx)) { std::fprintf(stderr, "context\n"); return 1; }
@@ -73,16 +76,44 @@ int main(int argc, char **argv) {
     qs[i].aggs = aggs;
     qs[i].n_aggs = 4;
     qs[i].n_rows = s.n;
-    qs[i].est_groups = s.hint ? (int64_t)s.d : 0;
+    qs[i].est_groups = s.est > 0 ? s.est : s.hint ? (int64_t)s.d : 0;
   }
   tga_batch *b = nullptr;
   CHECK(tga_batch_create(ctx, qs.data(), (int32_t)nq, mode, &b));
   emu_watch_batch(b);
   int bad = 0;
+  // A second batch built between replays and held across the next one takes
+  // its small output buffers from the arena's free lists, where an overflowed
+  // query's released buffers would sit: the first batch's kernels must never
+  // write into memory it gave back (a rerun must not release the buffers its
+  // device record still targets).
+  const size_t hq = 11;
+  int64_t hsum = 0;
+  for (int32_t v : hv[hq]) hsum += v;
   for (int rep = 0; rep < replays; ++rep) {
     std::printf("replay %d\n", rep);
     std::fflush(stdout);
+    tga_batch *held = nullptr;
+    if (rep > 0) {
+      CHECK(tga_batch_create(ctx, &qs[hq], 1, 0, &held));
+      CHECK(tga_batch_run(held));
+    }
     CHECK(tga_batch_run(b));
+    if (held) {
+      int64_t g = 0;
+      CHECK(tga_batch_num_groups(held, 0, &g));
+      std::vector<int64_t> c(std::max<int64_t>(g, 1)), sm(std::max<int64_t>(g, 1));
+      CHECK(tga_batch_copy_to_host(held, 0, 0, c.data(), g));
+      CHECK(tga_batch_copy_to_host(held, 0, 1, sm.data(), g));
+      int64_t tc = 0, ts = 0;
+      for (int64_t r = 0; r < g; ++r) { tc += c[r]; ts += sm[r]; }
+      if (g != (int64_t)specs[hq].d || tc != specs[hq].n || ts != hsum) {
+        std::printf("held batch result changed by the replay: %lld groups, count %lld, sum %lld\n", (long long)g,
+                    (long long)tc, (long long)ts);
+        ++bad;
+      }
+      tga_batch_destroy(held);
+    }
     for (size_t i = 0; i < nq; ++i) {
       const int64_t n = specs[i].n;
       // oracle on 64-bit keys (the 32-bit key values widened keep their order and identity)

@@ -93,21 +93,43 @@ def test_batch_matches_oracle(ctx, mode):
 
 
 def test_batch_underestimate_reruns(ctx):
-    # hints far below the truth: table overflow is detected and the query rerun
-    specs = [(1 << 18, 100_000, 32, True)]
+    # hints far below the truth: table overflow is detected and the query
+    # rerun alone, at every replay.  A second batch built between replays and
+    # held across the next one takes its small buffers from the arena's free
+    # lists: the first batch must never write into memory it released.
+    specs = [(1 << 18, 100_000, 32, True), (20_000, 6000, 64, True), (70_000, 64, 32, True)]
     qs, keep, host = _queries(ctx, specs)
     qs[0].est_groups = 10
+    qs[1].est_groups = 10
+    refs = [O.groupby(k, [v], AGGS4) for k, v in host]
     b = ctypes.c_void_p()
-    ctx._check(lib().tga_batch_create(ctx.h, qs, 1, 0, ctypes.byref(b)))
-    ctx._check(lib().tga_batch_run(b))
-    G = ctypes.c_int64()
-    ctx._check(lib().tga_batch_num_groups(b, 0, ctypes.byref(G)))
-    ref = O.groupby(host[0][0], [host[0][1]], AGGS4)
-    assert G.value == ref["n_groups"]
-    k, outs = _batch_result(b, 0, np.uint32, G.value)
-    np.testing.assert_array_equal(k, ref["keys"])
-    np.testing.assert_array_equal(outs[1], ref["aggs"][1])
-    lib().tga_batch_destroy(b)
+    ctx._check(lib().tga_batch_create(ctx.h, qs, 2, 0, ctypes.byref(b)))
+    try:
+        for rep in range(3):
+            held = ctypes.c_void_p()
+            if rep:
+                ctx._check(lib().tga_batch_create(ctx.h, ctypes.byref(qs[2]), 1, 0, ctypes.byref(held)))
+                ctx._check(lib().tga_batch_run(held))
+            ctx._check(lib().tga_batch_run(b))
+            for i in range(2):
+                G = ctypes.c_int64()
+                ctx._check(lib().tga_batch_num_groups(b, i, ctypes.byref(G)))
+                assert G.value == refs[i]["n_groups"], (rep, i)
+                k, outs = _batch_result(b, i, np.uint32 if specs[i][2] == 32 else np.uint64, G.value)
+                np.testing.assert_array_equal(k, refs[i]["keys"])
+                for x, r in zip(outs, refs[i]["aggs"]):
+                    np.testing.assert_array_equal(x, r)
+            if rep:
+                G = ctypes.c_int64()
+                ctx._check(lib().tga_batch_num_groups(held, 0, ctypes.byref(G)))
+                assert G.value == refs[2]["n_groups"]
+                k, outs = _batch_result(held, 0, np.uint32, G.value)
+                np.testing.assert_array_equal(k, refs[2]["keys"])
+                for x, r in zip(outs, refs[2]["aggs"]):
+                    np.testing.assert_array_equal(x, r)
+                lib().tga_batch_destroy(held)
+    finally:
+        lib().tga_batch_destroy(b)
 
 
 def _result_arrays(r, key_dtype):
