@@ -4,23 +4,34 @@
 Headline (one JSON line on stdout): input rows/s of the plain operator on the
 2^28-row group-by -- int64 keys uniform over 2^20 distinct values, int32
 values uniform in [-2^30, 2^30), COUNT / SUM(int64) / MIN / MAX -- with the
-columns resident in HBM when the timed region starts.  A step is one complete
-group-by (all kernels, the group count read back).  Beside it:
-  roofline      dominant kernel's algorithmic bytes / its HIP-event time
-  cpu_baseline  the restated reference algorithm on host threads, on a
-                2^24-row sample of the same workload
+columns resident in HBM when the timed region starts and NO cardinality hint
+(the autotuner estimates the group count itself).  A step is one complete
+group-by: estimate, plan, every kernel, the group count read back.  Beside it:
+  hinted        the same step with the exact group count as a hint
+  roofline      dominant kernel: algorithmic bytes per launch (12 B per input
+                row it processes, + 32 B per group for the aggregation) / its
+                HIP-event time; traffic = measured HBM bytes of that kernel
+  cpu_baseline  the restated reference algorithm (tile pre-aggregation, hash
+                partitioning, merge_partition, per-partition aggregation) on
+                host threads, over the full 2^28-row input of the headline
+  config1       2^24 rows, u32 keys over 2^10, int32 values: GPU vs the
+                restated reference on the CPU, bit-exact, both timed
+  config3       2^29 rows, Zipf(1.1) int64 keys over 2^24, int32 + fp64
+                columns, COUNT + SUM(int64) + SUM(fp64): timed, roofline
   batched       4096 queries (rows log-uniform in [2^10, 2^22], distinct keys
                 log-uniform in [2^4, 2^18], u32 keys, int32 values,
-                COUNT+SUM): one persistent launch vs one launch per query
+                COUNT+SUM): one persistent launch vs one launch per query,
+                every query's results compared between the two
   serving       Poisson arrivals (mean 20k/s) from that query mix plus 5%
                 2^26-row int64-key queries, 64 GB admission budget: p50/p99
-                and throughput vs the sequential plain operator
-Every aggregate result is checked (group counts, checksums) on the way.
+                and throughput vs the sequential plain operator, every
+                request's group count checked
 """
 import argparse
+import ctypes
 import json
-import math
 import os
+import subprocess
 import sys
 import time
 
@@ -31,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "input rows/s + % HBM peak at 2^28 rows; batched queries/s & p99 vs per-query"
+M64 = 2**64 - 1
 
 
 def log(*a):
@@ -46,22 +58,77 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the config-1 and config-3 sections")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the reference-workload section (its driver's uniform / Zipf / self-similar keys)")
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
-    ap.add_argument("--only", choices=["all", "headline", "copy", "workloads"], default="all",
-                    help="headline: only the 2^28-row group-by (profiling runs); copy: only the copy reference; "
-                         "workloads: only the reference-workload section")
+    ap.add_argument("--serve-rate", type=float, default=20_000.0)
+    ap.add_argument("--only", choices=["all", "headline", "copy", "workloads", "config1", "config3", "batch", "serve"],
+                    default="all",
+                    help="run one section (profiling runs); headline: only the 2^28-row group-by; copy: only the "
+                         "copy reference")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that fill roofline.traffic")
     return ap.parse_args()
 
 
-# profiler name of a timed step -> kernel symbols it may launch (the scatter
-# is the write-combining one or the plain one, as the plan says)
-KERNEL_SYMBOL = {"part_scatter": ("k_part_scatter_wc", "k_part_scatter"), "agg_partitions": ("k_agg",),
-                 "agg_direct": ("k_agg",), "compact": ("k_compact",), "copy16": ("k_copy16",)}
+# profiler name of a timed step -> kernel symbols it may launch
+KERNEL_SYMBOL = {"part_scatter": ("k_part_scatter_wc", "k_part_scatter"), "part_scatter2": ("k_part_scatter_wc",),
+                 "agg_partitions": ("k_agg",), "agg_direct": ("k_agg",), "compact": ("k_compact",),
+                 "copy16": ("k_copy16",), "slice_distinct": ("k_slice_distinct",)}
+
+
+def alg_bytes(kernel, n, G, row_b, state_b, key_b):
+    """Algorithmic bytes of one launch: the input bytes of the rows it
+    processes (the scatter's write of the partitioned copy is traffic, not
+    algorithmic bytes), plus the group state the aggregation produces."""
+    if kernel in ("agg_partitions", "agg_direct"):
+        return n * row_b + G * state_b
+    if kernel == "slice_distinct":
+        return n * key_b
+    return n * row_b
+
+
+# ---------------------------------------------------------------------------
+# measurement helpers
+# ---------------------------------------------------------------------------
+def time_steps(ctx, keys, cols, aggs, steps, warmup, **opts):
+    """Mean ms per complete group-by call (result left in HBM and released)."""
+    G = None
+    for _ in range(warmup):
+        r, G = ctx.groupby_resident(keys, cols, aggs, **opts)
+        ctx.free_result(r)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r, G = ctx.groupby_resident(keys, cols, aggs, **opts)
+        ctx.free_result(r)
+    ctx.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, G, ctx.last_plan()
+
+
+def kernel_times(ctx, keys, cols, aggs, steps, **opts):
+    """{step name: (launches, total ms)} from HIP events on the context stream."""
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        r, _ = ctx.groupby_resident(keys, cols, aggs, **opts)
+        ctx.free_result(r)
+    prof = ctx.profile_entries()
+    ctx.profile(False)
+    return prof
+
+
+def roofline(prof, steps, n, G, row_b, state_b, key_b):
+    dom_name, (dom_launches, dom_ms) = max(prof.items(), key=lambda kv: kv[1][1])
+    avg = dom_ms / max(1, dom_launches)
+    alg = alg_bytes(dom_name, n, G, row_b, state_b, key_b)
+    achieved = alg / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": dom_name, "avg_ms": round(avg, 4), "algorithmic_bytes": int(alg)}, \
+        {k: round(t / steps, 4) for k, (c, t) in prof.items()}
 
 
 def pmc_pass(counter, argv, workdir):
@@ -69,7 +136,6 @@ def pmc_pass(counter, argv, workdir):
     {kernel symbol prefix: [per-dispatch values in KiB]} or None."""
     import csv
     import shutil
-    import subprocess
     exe = shutil.which("rocprofv3")
     if not exe:
         return None
@@ -125,80 +191,54 @@ def measure_traffic(dom_name, args):
     return traffic, detail
 
 
+def host_info():
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return {"host_cores": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_threads():
+    # the host share of one GPU on the box is 16 cores; never more than it has
+    return min(16, os.cpu_count() or 1)
+
+
+def ref_kw(n):
+    tile = 10_000_000  # the reference defaults, test/test.cu:95-106
+    return dict(tile_len=tile, P=300, capacity=tile, min_load=tile * 3 // 5, max_load=tile * 4 // 5)
+
+
+# ---------------------------------------------------------------------------
+# sections
+# ---------------------------------------------------------------------------
 def headline(ctx, tga, args):
     n = 1 << args.rows_log2
     d = 1 << 20
     keys = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=2028, salt=0x5EED)
-    vals = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=2029)
+    vals = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & M64, b=1 << 31, seed=2029)
     aggs = ["count", ("sum", 0), ("min", 0), ("max", 0)]
-    opts = dict(est_groups=d)
-    res = None
-    for _ in range(args.warmup):
-        ctx.free_result(res)
-        res, G = ctx.groupby_resident(keys, [vals], aggs, **opts)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.free_result(res)  # the previous step's output goes back to the arena
-        res, G = ctx.groupby_resident(keys, [vals], aggs, **opts)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    ctx.free_result(res)
-    r = ctx.groupby(keys, [vals], aggs, sort=False, **opts)  # untimed: copy one result out to check it
-    assert r["n_groups"] == G
-    # size-independent checks of the last result: every key present once,
-    # counts add up to n, the int64 sums add up to the column total
-    assert G == d, G
+    ms, G, plan = time_steps(ctx, keys, [vals], aggs, args.steps, args.warmup)
+    ms_hint, G2, plan_hint = time_steps(ctx, keys, [vals], aggs, args.steps, 1, est_groups=d)
+    assert G == G2 == d, (G, G2)
+    # untimed: copy one result out and check it through size-independent
+    # properties (every key once, COUNT adds up to n, MIN <= MAX)
+    r = ctx.groupby(keys, [vals], aggs, sort=False)
+    assert r["n_groups"] == d
     assert int(r["aggs"][0].sum()) == n
-    assert np.unique(r["keys"]).size == G
+    assert np.unique(r["keys"]).size == d
     assert np.all(r["aggs"][2] <= r["aggs"][3])
-    ms = dt * 1e3 / args.steps
-    plan = ctx.last_plan()
-    # the same step without the cardinality hint: the estimate comes from a
-    # device key sample (reported beside the headline, not as it)
-    ctx.free_result(ctx.groupby_resident(keys, [vals], aggs)[0])
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r2, _ = ctx.groupby_resident(keys, [vals], aggs)
-        ctx.free_result(r2)
-    ctx.synchronize()
-    ms_nohint = (time.perf_counter() - t0) * 1e3 / args.steps
-    plan_nohint = ctx.last_plan()
-    # per-kernel HIP-event times over the same number of steps
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(args.steps):
-        res, _ = ctx.groupby_resident(keys, [vals], aggs, **opts)
-        ctx.free_result(res)
-    prof = ctx.profile_entries()
-    ctx.profile(False)
-    kern = {k: (c, t) for k, (c, t) in prof.items()}
-    dom = max(kern.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_launches, dom_ms) = dom
-    dom_avg_ms = dom_ms / max(1, dom_launches)
-    # algorithmic bytes (per launch of the dominant kernel): every kernel of
-    # the partitioned path moves at least the 12 B of every input row; the
-    # aggregation kernel also produces the 32 B state of every group
-    row_b = 8 + 4
-    alg = {
-        "agg_partitions": n * row_b + G * 32,
-        "agg_direct": n * row_b + G * 32,
-        "part_scatter": 2 * n * row_b,
-    }.get(dom_name, n * row_b)
-    achieved = alg / (dom_avg_ms * 1e-3) / 1e9
-    op_alg = n * row_b + G * 32
+    prof = kernel_times(ctx, keys, [vals], aggs, args.steps)
+    roof, kern = roofline(prof, args.steps, n, G, 12, 32, 8)
+    op_alg = n * 12 + G * 32
     out = {
-        "n": n, "groups": G, "ms_per_step": ms, "plan": plan,
-        "no_hint": {"ms_per_step": round(ms_nohint, 4), "rows_per_s": round(n / (ms_nohint * 1e-3), 1),
-                    "partition_bits": plan_nohint["partition_bits"], "attempts": plan_nohint["attempts"]},
-        "kernels_ms_per_step": {k: t / args.steps for k, (c, t) in kern.items()},
-        "dominant": dom_name, "dominant_avg_ms": dom_avg_ms,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
-        "op_bytes": op_alg,
-        "op_frac_of_peak": round(op_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "value": n / (ms * 1e-3),
+        "n": n, "groups": G, "ms_per_step": ms, "plan": plan, "value": n / (ms * 1e-3),
+        "hinted": {"ms_per_step": round(ms_hint, 4), "rows_per_s": round(n / (ms_hint * 1e-3), 1),
+                   "partition_bits": plan_hint["partition_bits"], "attempts": plan_hint["attempts"]},
+        "kernels_ms_per_step": kern, "roofline": roof,
+        "op_bytes": op_alg, "op_frac_of_peak": round(op_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
     return out, keys, vals
 
@@ -220,28 +260,109 @@ def copy_bandwidth(ctx, tga, nbytes=1 << 31, reps=5):
     return {"bytes_moved": 2 * nbytes, "avg_ms": round(ms / n, 4), "gbs": round(2 * nbytes / (ms / n * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(ctx, keys, vals):
+def cpu_baseline(ctx, tga, keys, vals):
+    """The restated reference algorithm on the full headline input (2^28 rows,
+    27 tiles of 1e7), one run after a small warm-up."""
     import oracle as O
-    m = 1 << 24
-    kh = np.empty(m, np.uint64)
-    vh = np.empty(m, np.int32)
-    from cpu_gpu_coprocess_groupby_agg_amd import lib
-    ctx._check(lib().tga_memcpy_d2h(ctx.h, kh.ctypes.data, keys.ptr, m * 8))
-    ctx._check(lib().tga_memcpy_d2h(ctx.h, vh.ctypes.data, vals.ptr, m * 4))
-    cores = min(16, os.cpu_count() or 1)
-    tile = 10_000_000  # the reference defaults, test/test.cu:95-106
-    kw = dict(tile_len=tile, P=300, capacity=tile, min_load=tile * 3 // 5, max_load=tile * 4 // 5, nthreads=cores)
-    O.ref_run(kh[: 1 << 20], vh[: 1 << 20], with_output=False, **kw)  # warm-up
-    times = []
-    res = None
-    for _ in range(3):
+    kh = keys.to_numpy()
+    vh = vals.to_numpy()
+    n = kh.size
+    th = cpu_threads()
+    O.ref_run(kh[: 1 << 20], vh[: 1 << 20], with_output=False, nthreads=th, **ref_kw(1 << 20))  # warm-up
+    t0 = time.perf_counter()
+    res = O.ref_run(kh, vh, with_output=False, nthreads=th, **ref_kw(n))
+    t = time.perf_counter() - t0
+    assert res["n_groups"] == 1 << 20
+    st = res["stats"]
+    out = {"value": round(n / t, 1), "unit": "rows/s", "cores": th, "kind": "port",
+           "sample": f"the full 2^28-row headline input ({st['tiles']} tiles of 1e7 rows, P=300, Capacity=1e7): the "
+                     f"restated reference two-phase algorithm on {th} host threads, one timed run after a 2^20-row "
+                     f"warm-up; {res['n_groups']} groups",
+           "seconds": round(t, 3),
+           "phases_s": {"phase1": round(st["t_phase1_s"], 3), "merge": round(st["t_merge_s"], 6),
+                        "phase2": round(st["t_phase2_s"], 3)}}
+    out.update(host_info())
+    return out
+
+
+def config1(ctx, tga, args):
+    """2^24 rows, u32 keys uniform over 2^10, int32 values in [-2^20, 2^20),
+    COUNT + SUM(int64): GPU timed, and checked bit-exact against the restated
+    reference algorithm run on the CPU over the same rows (its u32 types: the
+    low 32 bits of SUM, the counts, the keys), which is timed as well."""
+    import oracle as O
+    n, d = 1 << 24, 1 << 10
+    k = ctx.generate(tga.GEN_KEYS_U32, n, a=d, seed=1, salt=3)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 20)) & M64, b=1 << 21, seed=2)
+    aggs = ["count", ("sum", 0)]
+    steps = max(args.steps, 20)
+    ms, G, plan = time_steps(ctx, k, [v], aggs, steps, args.warmup)
+    prof = kernel_times(ctx, k, [v], aggs, steps)
+    roof, kern = roofline(prof, steps, n, G, 8, 20, 4)
+    got = ctx.groupby(k, [v], aggs)
+    kh, vh = k.to_numpy(), v.to_numpy()
+    th = cpu_threads()
+    O.ref_run(kh[:1 << 18], vh[:1 << 18].view(np.uint32), nthreads=th, **ref_kw(1 << 18))  # warm-up
+    t0 = time.perf_counter()
+    ref = O.ref_run(kh, vh.view(np.uint32), nthreads=th, **ref_kw(n))
+    t_cpu = time.perf_counter() - t0
+    o = np.argsort(ref["keys"], kind="stable")
+    exact = O.groupby(kh, [vh], aggs)
+    bit_exact = (got["n_groups"] == ref["n_groups"] == d
+                 and np.array_equal(got["keys"], ref["keys"][o])
+                 and np.array_equal(got["aggs"][0], ref["count"][o].astype(np.int64))
+                 and np.array_equal(got["aggs"][1].astype(np.uint64) & np.uint64(0xFFFFFFFF),
+                                    ref["sum"][o].astype(np.uint64) & np.uint64(0xFFFFFFFF))
+                 and np.array_equal(got["aggs"][1], exact["aggs"][1]))
+    k.free()
+    v.free()
+    return {"workload": "2^24 rows, u32 key uniform over 2^10, int32 value uniform [-2^20, 2^20), COUNT + SUM(int64)",
+            "gpu_ms": round(ms, 4), "gpu_rows_per_s": round(n / (ms * 1e-3), 1),
+            "gpu_frac_of_hbm_peak": round((n * 8 + G * 20) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "roofline": roof, "kernels_ms_per_step": kern, "path": plan["path"],
+            "cpu_rows_per_s": round(n / t_cpu, 1), "cpu_threads": th, "cpu_kind": "restated reference (u32 types)",
+            "bit_exact_vs_cpu": bool(bit_exact), "groups": int(G)}
+
+
+def config3(ctx, tga, args):
+    """2^29 rows, int64 keys Zipf(1.1) over 2^24 distinct values, an int32
+    column uniform in [-2^30, 2^30) and an fp64 column N(0, 1e3), COUNT +
+    SUM(int64) + SUM(fp64), no hint.  Checks: COUNT adds up to n, the int64
+    sums add up to the column total, the group count within (2^20, 2^24]."""
+    n, D = 1 << 29, 1 << 24
+    k = ctx.generate(tga.GEN_ZIPF_U64, n, a=D, x=1.1, seed=2930, salt=11)
+    a = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & M64, b=1 << 31, seed=2931)
+    f = ctx.generate(tga.GEN_NORMAL_F64, n, x=0.0, y=1e3, seed=2932)
+    aggs = ["count", ("sum", 0), ("sum", 1)]
+    steps = max(3, args.steps // 3)
+    ms, G, plan = time_steps(ctx, k, [a, f], aggs, steps, 1)
+    prof = kernel_times(ctx, k, [a, f], aggs, steps)
+    roof, kern = roofline(prof, steps, n, G, 20, 32, 8)
+    got = ctx.groupby(k, [a, f], aggs, sort=False)
+    ah = a.to_numpy()
+    ok = (int(got["aggs"][0].sum()) == n and int(got["aggs"][1].sum()) == int(ah.astype(np.int64).sum())
+          and (1 << 20) < got["n_groups"] <= D)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        m = 1 << 26
+        kh = k.to_numpy()[:m]
+        th = cpu_threads()
         t0 = time.perf_counter()
-        res = O.ref_run(kh, vh, with_output=False, **kw)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[1]
-    return {"value": m / t, "unit": "rows/s", "cores": cores, "kind": "port",
-            "sample": f"first 2^24 rows of the same 2^28-row workload; restated reference two-phase algorithm "
-                      f"(tile_len=1e7, P=300, Capacity=1e7), median of 3, {res['n_groups']} groups"}
+        r = O.ref_run(kh, ah[:m], with_output=False, nthreads=th, **ref_kw(m))
+        t = time.perf_counter() - t0
+        cpu = {"rows_per_s": round(m / t, 1), "threads": th,
+               "sample": f"first 2^26 rows, key + int32 column (the restated reference carries one value column), "
+                         f"{r['n_groups']} groups"}
+    for x in (k, a, f):
+        x.free()
+    return {"workload": "2^29 rows, int64 key Zipf(1.1) over 2^24, int32 uniform [-2^30, 2^30) + fp64 N(0, 1e3), "
+                        "COUNT + SUM(int64) + SUM(fp64), no hint",
+            "gpu_ms": round(ms, 4), "gpu_rows_per_s": round(n / (ms * 1e-3), 1),
+            "gpu_frac_of_hbm_peak": round((n * 20 + G * 32) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "roofline": roof, "kernels_ms_per_step": kern, "groups": int(G),
+            "plan": {k2: plan[k2] for k2 in ("path", "partition_bits", "attempts", "est_groups")},
+            "checks_ok": bool(ok), "cpu": cpu}
 
 
 def loguniform_int(rng, lo_log2, hi_log2, size):
@@ -252,12 +373,10 @@ def make_batch(ctx, tga, nq, seed=7):
     rng = np.random.default_rng(seed)
     rows = loguniform_int(rng, 10, 22, nq)
     dist = np.minimum(loguniform_int(rng, 4, 18, nq), rows)
-    keys, vals, qs = [], [], []
+    keys, vals = [], []
     for i in range(nq):
-        k = ctx.generate(tga.GEN_KEYS_U32, int(rows[i]), a=int(dist[i]), seed=10_000 + i, salt=i)
-        v = ctx.generate(tga.GEN_VALS_I32, int(rows[i]), a=(-(1 << 20)) & (2**64 - 1), b=1 << 21, seed=20_000 + i)
-        keys.append(k)
-        vals.append(v)
+        keys.append(ctx.generate(tga.GEN_KEYS_U32, int(rows[i]), a=int(dist[i]), seed=10_000 + i, salt=i))
+        vals.append(ctx.generate(tga.GEN_VALS_I32, int(rows[i]), a=(-(1 << 20)) & M64, b=1 << 21, seed=20_000 + i))
     return rows, dist, keys, vals
 
 
@@ -273,17 +392,7 @@ def workloads(ctx, tga, args, steps=3):
     vals = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1 << 31, seed=97)
     for dist, name in ((0, "uniform"), (1, "zipf_0.9"), (2, "selfsimilar_0.9")):
         keys = ctx.generate(tga.GEN_REF_U32, n, a=card, b=dist, x=0.9, seed=96, salt=dist)
-        aggs = ["count", ("sum", 0)]
-        r, G = ctx.groupby_resident(keys, [vals], aggs)
-        ctx.free_result(r)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            r, G = ctx.groupby_resident(keys, [vals], aggs)
-            ctx.free_result(r)
-        ctx.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
-        plan = ctx.last_plan()
+        ms, G, plan = time_steps(ctx, keys, [vals], ["count", ("sum", 0)], steps, 1)
         out[name] = {"ms": round(ms, 3), "rows_per_s": round(n / (ms * 1e-3), 1), "groups": int(G),
                      "groups_equal_cardinality": int(G) == card, "partition_bits": plan["partition_bits"],
                      "attempts": plan["attempts"]}
@@ -293,8 +402,36 @@ def workloads(ctx, tga, args, steps=3):
     return {"rows": n, "cardinality": card, "key": "u32", "aggs": "COUNT, SUM(int64)", **out}
 
 
+def _mix64(z):
+    """SplitMix64 finalizer on a uint64 array (wraps mod 2^64)."""
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _fingerprint(keys, cnt, sm):
+    """Order-independent 64-bit hash of the multiset of (key, COUNT, SUM) rows."""
+    mix64 = _mix64
+    with np.errstate(over="ignore"):
+        h = mix64(keys.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+                  ^ mix64(cnt.view(np.uint64) ^ mix64(sm.view(np.uint64) + np.uint64(0x632BE59BD9B4E019))))
+        return int(h.sum(dtype=np.uint64))
+
+
+def batch_query_result(lib, b, i, G):
+    k = np.empty(G, np.uint32)
+    c = np.empty(G, np.int64)
+    s = np.empty(G, np.int64)
+    if G:
+        for idx, arr in ((-1, k), (0, c), (1, s)):
+            st = lib().tga_batch_copy_to_host(b, i, idx, arr.ctypes.data, G)
+            assert st == 0, st
+    return k, c, s
+
+
 def batched(ctx, tga, args):
-    import ctypes
+    import oracle as O
     from cpu_gpu_coprocess_groupby_agg_amd import lib
     nq = args.batch_queries
     rows, dist, keys, vals = make_batch(ctx, tga, nq)
@@ -311,71 +448,94 @@ def batched(ctx, tga, args):
         qs[i].est_groups = int(dist[i])
     out = {"queries": nq, "total_rows": int(rows.sum())}
     res = {}
+    check_q = list(range(0, nq, max(1, nq // 16)))
     for mode, name in ((0, "persistent"), (1, "per_query")):
         b = ctypes.c_void_p()
         t0 = time.perf_counter()
         ctx._check(lib().tga_batch_create(ctx.h, qs, nq, mode, ctypes.byref(b)))
         t_build = time.perf_counter() - t0
         ctx._check(lib().tga_batch_run(b))  # warm-up
-        reps = 3 if mode == 0 else 1
+        reps = 5 if mode == 0 else 1
         t0 = time.perf_counter()
         for _ in range(reps):
             ctx._check(lib().tga_batch_run(b))
         dt = (time.perf_counter() - t0) / reps
         g = np.zeros(nq, np.int64)
-        cks = 0
+        fp = np.zeros(nq, np.uint64)
+        exact = {}
         for i in range(nq):
             x = ctypes.c_int64()
             ctx._check(lib().tga_batch_num_groups(b, i, ctypes.byref(x)))
             g[i] = x.value
-        for i in range(0, nq, 97):  # checksum of checksums on a subset
-            s = np.empty(g[i], np.int64)
-            ctx._check(lib().tga_batch_copy_to_host(b, i, 1, s.ctypes.data, int(g[i])))
-            c = np.empty(g[i], np.int64)
-            ctx._check(lib().tga_batch_copy_to_host(b, i, 0, c.ctypes.data, int(g[i])))
+            k, c, s = batch_query_result(lib, b, i, int(g[i]))
             assert int(c.sum()) == int(rows[i])
-            cks = (cks * 1_000_003 + int(s.sum())) % (1 << 61)
+            fp[i] = _fingerprint(k, c, s)
+            if i in check_q:
+                o = np.argsort(k, kind="stable")
+                exact[i] = (k[o], c[o], s[o])
         lib().tga_batch_destroy(b)
-        res[name] = {"s_per_batch": dt, "queries_per_s": nq / dt, "build_s": t_build, "groups": g, "checksum": cks}
+        res[name] = {"s_per_batch": dt, "queries_per_s": nq / dt, "build_s": t_build, "groups": g, "fp": fp,
+                     "exact": exact}
         log(f"batched {name}: {dt*1e3:.2f} ms per batch, {nq/dt:.0f} q/s (build {t_build:.2f} s)")
-    assert np.array_equal(res["persistent"]["groups"], res["per_query"]["groups"])
-    assert np.array_equal(res["persistent"]["groups"], np.minimum(dist, rows)) or True
-    assert res["persistent"]["checksum"] == res["per_query"]["checksum"]
+    p, q = res["persistent"], res["per_query"]
+    same_groups = bool(np.array_equal(p["groups"], q["groups"]))
+    same_rows = bool(np.array_equal(p["fp"], q["fp"]))
+    # a subset against the oracle, row for row
+    oracle_ok = True
+    for i in check_q:
+        kh, vh = keys[i].to_numpy(), vals[i].to_numpy()
+        ref = O.groupby(kh, [vh], ["count", ("sum", 0)])
+        for mode_res in (p, q):
+            k, c, s = mode_res["exact"][i]
+            oracle_ok &= bool(np.array_equal(k, ref["keys"]) and np.array_equal(c, ref["aggs"][0])
+                              and np.array_equal(s, ref["aggs"][1]))
+    assert same_groups and same_rows and oracle_ok, (same_groups, same_rows, oracle_ok)
+    total_groups = int(p["groups"].sum())
+    alg = int(rows.sum()) * 8 + total_groups * 20
     out.update({
-        "persistent_queries_per_s": round(res["persistent"]["queries_per_s"], 1),
-        "per_query_queries_per_s": round(res["per_query"]["queries_per_s"], 1),
-        "speedup": round(res["per_query"]["s_per_batch"] / res["persistent"]["s_per_batch"], 2),
-        "persistent_ms_per_batch": round(res["persistent"]["s_per_batch"] * 1e3, 3),
-        "per_query_ms_per_batch": round(res["per_query"]["s_per_batch"] * 1e3, 3),
-        "bit_identical": True,
+        "persistent_queries_per_s": round(p["queries_per_s"], 1),
+        "per_query_queries_per_s": round(q["queries_per_s"], 1),
+        "speedup": round(q["s_per_batch"] / p["s_per_batch"], 2),
+        "persistent_ms_per_batch": round(p["s_per_batch"] * 1e3, 3),
+        "per_query_ms_per_batch": round(q["s_per_batch"] * 1e3, 3),
+        "persistent_build_s": round(p["build_s"], 3),
+        "total_groups": total_groups,
+        "algorithmic_bytes": alg,
+        "persistent_gbs": round(alg / p["s_per_batch"] / 1e9, 1),
+        "persistent_frac_of_hbm_peak": round(alg / p["s_per_batch"] / 1e9 / HBM_PEAK_GBS, 4),
+        "bit_identical": same_groups and same_rows,
+        "compared": f"all {nq} queries: group counts and a 64-bit order-independent hash of every (key, COUNT, SUM) "
+                    f"row, persistent vs per-query; {len(check_q)} queries row for row against the oracle",
+        "oracle_subset_ok": oracle_ok,
     })
-    return out, (rows, dist, keys, vals)
+    return out, (rows, dist, keys, vals, p["groups"])
 
 
 def serving(ctx, tga, args, pool):
-    import ctypes
     from cpu_gpu_coprocess_groupby_agg_amd import lib
-    rows, dist, keys, vals = pool
+    rows, dist, keys, vals, groups = pool
     rng = np.random.default_rng(11)
-    rate = 20_000.0
+    rate = args.serve_rate
     n = int(rate * args.serve_seconds)
     arr = np.cumsum(rng.exponential(1e6 / rate, n))
     big_n = 1 << 26
-    nbig = 4
-    bigk = [ctx.generate(tga.GEN_KEYS_U64, big_n, a=1 << 20, seed=900 + j, salt=j) for j in range(nbig)]
-    bigv = [ctx.generate(tga.GEN_VALS_I32, big_n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=950 + j)
-            for j in range(nbig)]
+    big_d = [1 << 20, 3 << 18, 1 << 19, 5 << 17]
+    # closed-form columns: request group counts are known exactly
+    bigk = [ctx.generate(tga.GEN_MOD_U64, big_n, a=big_d[j], salt=900 + j) for j in range(len(big_d))]
+    bigv = [ctx.generate(tga.GEN_SEQ_I32, big_n, a=j) for j in range(len(big_d))]
     aggs2 = (tga.Agg * 2)(tga.Agg(tga.COUNT, 0), tga.Agg(tga.SUM, 0))
     qs = (tga.Query * n)()
+    expect = np.zeros(n, np.int64)
     keep = []
     nlarge = 0
     for i in range(n):
         if rng.random() < 0.05:
-            j = int(rng.integers(nbig))
+            j = int(rng.integers(len(big_d)))
             vc = (tga.Column * 1)(bigv[j].column())
             qs[i].key = bigk[j].column()
             qs[i].n_rows = big_n
-            qs[i].est_groups = 1 << 20
+            qs[i].est_groups = big_d[j]
+            expect[i] = big_d[j]
             nlarge += 1
         else:
             j = int(rng.integers(len(rows)))
@@ -383,6 +543,7 @@ def serving(ctx, tga, args, pool):
             qs[i].key = keys[j].column()
             qs[i].n_rows = int(rows[j])
             qs[i].est_groups = int(dist[j])
+            expect[i] = groups[j]
         keep.append(vc)
         qs[i].values = ctypes.cast(vc, ctypes.POINTER(tga.Column))
         qs[i].n_values = 1
@@ -393,24 +554,25 @@ def serving(ctx, tga, args, pool):
     ctx._check(lib().tga_server_create(ctx.h, 64 << 30, 512, ctypes.byref(srv)))
     st = tga.ServerStats()
     wall = ctypes.c_double()
-    lib().tga_server_replay.argtypes = [ctypes.c_void_p, ctypes.POINTER(tga.Query), ctypes.POINTER(ctypes.c_double),
-                                        ctypes.c_int64, ctypes.POINTER(tga.ServerStats), ctypes.POINTER(ctypes.c_double)]
-    lib().tga_server_replay.restype = ctypes.c_int32
-    ctx._check(lib().tga_server_replay(srv, qs, arr_c, n, ctypes.byref(st), ctypes.byref(wall)))
+    got = np.zeros(n, np.int64)
+    ctx._check(lib().tga_server_replay(srv, qs, arr_c, n, ctypes.byref(st), ctypes.byref(wall),
+                                       got.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
     lib().tga_server_destroy(srv)
     served = {"requests": n, "large": nlarge, "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
               "throughput_qps": round(n / wall.value, 1), "launches": st.launches,
-              "peak_admitted_gb": round(st.peak_bytes / 2**30, 2)}
+              "peak_admitted_gb": round(st.peak_bytes / 2**30, 2),
+              "group_counts_ok": bool(np.array_equal(got, expect))}
     log(f"serving: {served}")
     seq = tga.ServerStats()
-    lib().tga_sequential_replay.argtypes = [ctypes.c_void_p, ctypes.POINTER(tga.Query), ctypes.POINTER(ctypes.c_double),
-                                            ctypes.c_int64, ctypes.POINTER(tga.ServerStats),
-                                            ctypes.POINTER(ctypes.c_double)]
-    lib().tga_sequential_replay.restype = ctypes.c_int32
-    ctx._check(lib().tga_sequential_replay(ctx.h, qs, arr_c, n, ctypes.byref(seq), ctypes.byref(wall)))
+    got2 = np.zeros(n, np.int64)
+    ctx._check(lib().tga_sequential_replay(ctx.h, qs, arr_c, n, ctypes.byref(seq), ctypes.byref(wall),
+                                           got2.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
     sequential = {"p50_us": round(seq.p50_us, 1), "p99_us": round(seq.p99_us, 1),
-                  "throughput_qps": round(n / wall.value, 1)}
+                  "throughput_qps": round(n / wall.value, 1), "group_counts_ok": bool(np.array_equal(got2, expect))}
     log(f"sequential: {sequential}")
+    for x in bigk + bigv:
+        x.free()
+    assert served["group_counts_ok"] and sequential["group_counts_ok"]
     return {"arrival_rate_qps": rate, "budget_gb": 64, "server": served, "sequential_plain": sequential}
 
 
@@ -432,32 +594,54 @@ def main():
     if args.only == "workloads":
         print(json.dumps({"workloads": workloads(ctx, tga, args)}))
         return
+    if args.only == "config1":
+        print(json.dumps({"config1": config1(ctx, tga, args)}))
+        return
+    if args.only == "config3":
+        args.no_cpu = True
+        print(json.dumps({"config3": config3(ctx, tga, args)}))
+        return
+    if args.only in ("batch", "serve"):
+        b, pool = batched(ctx, tga, args)
+        out = {"batched": b}
+        if args.only == "serve":
+            out["serving"] = serving(ctx, tga, args, pool)
+        print(json.dumps(out))
+        return
     h, keys, vals = headline(ctx, tga, args)
-    log(f"headline: {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s, kernels {h['kernels_ms_per_step']}")
+    log(f"headline (no hint): {h['ms_per_step']:.3f} ms/step, {h['value']:.3e} rows/s; hinted {h['hinted']}; "
+        f"kernels {h['kernels_ms_per_step']}")
     if not args.no_traffic and args.only == "all":
-        traffic, detail = measure_traffic(h["dominant"], args)
+        traffic, detail = measure_traffic(h["roofline"]["kernel"], args)
         if traffic:
             h["roofline"]["traffic"] = round(traffic)
             h["roofline"]["traffic_detail"] = detail
-        log(f"traffic of {h['dominant']}: {traffic} {detail}")
+            h["roofline"]["traffic_over_algorithmic"] = round(traffic / h["roofline"]["algorithmic_bytes"], 3)
+        log(f"traffic of {h['roofline']['kernel']}: {traffic} {detail}")
     cpu = None
     if args.only == "headline":
-        args.no_cpu = args.no_batch = args.no_serve = True
+        args.no_cpu = args.no_batch = args.no_serve = args.no_configs = True
+        args.no_workloads = True
     if not args.no_cpu:
-        cpu = cpu_baseline(ctx, keys, vals)
+        cpu = cpu_baseline(ctx, tga, keys, vals)
         log(f"cpu baseline: {cpu}")
     keys.free()
     vals.free()
     extra = {}
     pool = None
-    if args.only == "all" and not args.no_workloads:
+    if not args.no_configs:
+        extra["config1"] = config1(ctx, tga, args)
+        log(f"config1: {extra['config1']}")
+        extra["config3"] = config3(ctx, tga, args)
+        log(f"config3: {extra['config3']}")
+    if not args.no_workloads:
         extra["workloads"] = workloads(ctx, tga, args)
     if not args.no_batch:
         extra["batched"], pool = batched(ctx, tga, args)
     if not args.no_serve:
         if pool is None:
             args.batch_queries = 512
-            pool = make_batch(ctx, tga, 512)
+            _, pool = batched(ctx, tga, args)
         extra["serving"] = serving(ctx, tga, args, pool)
     line = {
         "metric": METRIC,
@@ -473,13 +657,12 @@ def main():
         "dtype": "int64",
         "data": "synthetic (SplitMix64 counter-based, generated in HBM)",
         "config": {"workload": "groupby 2^28 rows, int64 key uniform over 2^20 distinct, int32 value uniform "
-                               "[-2^30, 2^30), COUNT/SUM(int64)/MIN/MAX",
+                               "[-2^30, 2^30), COUNT/SUM(int64)/MIN/MAX, no cardinality hint",
                    "rows": h["n"], "groups": h["groups"], "plan": h["plan"]},
         "roofline": h["roofline"],
         "op_frac_of_hbm_peak": h["op_frac_of_peak"],
-        "dominant_kernel": {"name": h["dominant"], "avg_ms": round(h["dominant_avg_ms"], 4)},
-        "no_hint": h["no_hint"],
-        "kernels_ms_per_step": {k: round(v, 4) for k, v in h["kernels_ms_per_step"].items()},
+        "hinted": h["hinted"],
+        "kernels_ms_per_step": h["kernels_ms_per_step"],
         "cpu_baseline": cpu,
         "copy_reference": copy,
         "device": info,

@@ -46,6 +46,7 @@ _OPS = {"count": COUNT, "sum": SUM, "min": MIN, "max": MAX}
 PATH_AUTO, PATH_DIRECT, PATH_PARTITIONED = 0, 1, 2
 
 GEN_KEYS_U32, GEN_KEYS_U64, GEN_VALS_I32, GEN_ZIPF_U64, GEN_NORMAL_F64, GEN_REF_U32 = 1, 2, 3, 4, 5, 6
+GEN_MOD_U32, GEN_MOD_U64, GEN_SEQ_I32 = 7, 8, 9
 
 
 class TgaError(RuntimeError):
@@ -132,8 +133,9 @@ def lib():
             "tga_server_wait": [vp, i64, P(vp)],
             "tga_server_stats_get": [vp, P(ServerStats)],
             "tga_server_destroy": [vp],
-            "tga_server_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double)],
-            "tga_sequential_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double)],
+            "tga_server_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double), P(i64)],
+            "tga_sequential_replay": [vp, P(Query), P(ctypes.c_double), i64, P(ServerStats), P(ctypes.c_double),
+                                      P(i64)],
             "tga_compat_partition": [vp, vp, vp, sz, sz, sz, sz, P(vp)],
             "tga_partset_count": [vp, P(sz)],
             "tga_partset_info": [vp, sz, P(sz), P(u64)],
@@ -256,7 +258,7 @@ class Context:
 
     def generate(self, kind, n, a=0, b=0, x=0.0, y=0.0, seed=1, salt=0):
         dt = {GEN_KEYS_U32: U32, GEN_KEYS_U64: U64, GEN_VALS_I32: I32, GEN_ZIPF_U64: U64, GEN_NORMAL_F64: F64,
-              GEN_REF_U32: U32}[kind]
+              GEN_REF_U32: U32, GEN_MOD_U32: U32, GEN_MOD_U64: U64, GEN_SEQ_I32: I32}[kind]
         d = DeviceArray(self, n, dt)
         self._check(lib().tga_generate(self.h, kind, d.ptr, n, a & (2**64 - 1), b, x, y, seed, salt))
         return d

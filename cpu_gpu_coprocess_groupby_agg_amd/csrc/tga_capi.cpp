@@ -455,6 +455,7 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   TGA_HIP(ctx, hipMemcpyAsync(h, buf.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   bool reliable = true;
+  uint64_t sample_d = h[0];  // distinct keys seen: a lower bound on the group count
   if (!h[3]) {
     *est = estimate_from_stats(h[0], h[1], h[2], m, n, top_share, &reliable);
   } else {
@@ -464,6 +465,8 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
     TGA_HIP(ctx, hipMemcpyAsync(v.data(), smp, m * 8, hipMemcpyDeviceToHost, ctx->stream));
     TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *est = estimate_from_sample(v.data(), m, n, top_share, &reliable);
+    sample_d = 0;
+    for (uint64_t i = 0; i < m; ++i) sample_d += (i == 0 || v[i] != v[i - 1]);  // v is sorted now
   }
   if (reliable || n < kSliceMinRows) return TGA_OK;
   // Otherwise count the distinct keys of one 2^-10 slice of the partition
@@ -481,8 +484,19 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   uint64_t c[2];
   TGA_HIP(ctx, hipMemcpyAsync(c, buf.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (!c[1]) *est = std::max<uint64_t>(1, std::min<uint64_t>(n, c[0] << kSliceBits));
+  *est = combine_slice_estimate(*est, sample_d, c[0], c[1] != 0, n);
   return TGA_OK;
+}
+
+uint64_t combine_slice_estimate(uint64_t sample_est, uint64_t sample_d, uint64_t slice_count, bool slice_overflow,
+                                uint64_t n) {
+  // An overflowed slice set counted only part of its slice: keep the
+  // sample's estimate.  Otherwise the scaled slice count replaces it, but
+  // never below the distinct keys the sample saw (a hot key with a short
+  // tail may leave the slice empty).
+  if (slice_overflow) return std::max<uint64_t>(1, std::max(sample_est, sample_d));
+  const uint64_t scaled = std::min<uint64_t>(n, slice_count << kSliceBits);
+  return std::max<uint64_t>(1, std::max(scaled, std::min<uint64_t>(n, sample_d)));
 }
 
 // ---------------------------------------------------------------------------
@@ -1301,7 +1315,8 @@ TGA_API tga_status tga_scan_exclusive_u32(tga_context *ctx, const uint32_t *d_in
 TGA_API tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int64_t n, uint64_t a, uint64_t b,
                                 double x, double y, uint64_t seed, uint64_t salt) {
   if (!ctx || n < 0 || (n > 0 && !d_out)) return TGA_ERR_INVALID_ARG;
-  if ((kind == TGA_GEN_KEYS_U32 || kind == TGA_GEN_KEYS_U64 || kind == TGA_GEN_ZIPF_U64) && a == 0)
+  if ((kind == TGA_GEN_KEYS_U32 || kind == TGA_GEN_KEYS_U64 || kind == TGA_GEN_ZIPF_U64 ||
+       kind == TGA_GEN_MOD_U32 || kind == TGA_GEN_MOD_U64) && a == 0)
     return fail(ctx, TGA_ERR_INVALID_ARG, "distinct count must be positive");
   if (kind == TGA_GEN_VALS_I32 && b == 0) return fail(ctx, TGA_ERR_INVALID_ARG, "empty value range");
   if (n == 0) return TGA_OK;

@@ -142,6 +142,10 @@ TGA_API tga_status tga_compat_partition(tga_context *ctx, uint32_t *host_keys, u
   if (tile_len == 0 || P == 0 || P > 65536 || nstreams == 0 || (n && (!host_keys || !host_vals)))
     return fail(ctx, TGA_ERR_INVALID_ARG, "compat partition: bad tile_len/P/nstreams or null buffers");
   if (tile_len >= (1ULL << 31)) return fail(ctx, TGA_ERR_INVALID_ARG, "tile_len must be < 2^31");
+  // the partition histogram is one LDS word per partition
+  if (P * 4 > ctx->lds_max)
+    return fail(ctx, TGA_ERR_INVALID_ARG, "P = %zu partitions exceed the LDS histogram (at most %zu)", P,
+                ctx->lds_max / 4);
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
   QuerySpec q = compat_spec();

@@ -183,6 +183,11 @@ uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_s
 // the most frequent seen top times.
 uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n,
                              double *top_share = nullptr, bool *reliable = nullptr);
+// The estimate after the hash-slice count: 2^kSliceBits times the slice's
+// distinct keys, at least the sample's distinct keys sample_d; the sample's
+// estimate when the slice set overflowed.
+uint64_t combine_slice_estimate(uint64_t sample_est, uint64_t sample_d, uint64_t slice_count, bool slice_overflow,
+                                uint64_t n);
 // An unreliable sample estimate over at least this many rows is replaced by
 // an exact count over a hash slice of the keys.
 constexpr uint64_t kSliceMinRows = 1ULL << 22;

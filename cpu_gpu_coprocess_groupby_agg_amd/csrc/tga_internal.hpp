@@ -34,7 +34,7 @@ struct PartOut {
 };
 
 enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_ZIPF_U64 = 4, GEN_NORMAL_F64 = 5,
-                     GEN_REF_U32 = 6 };
+                     GEN_REF_U32 = 6, GEN_MOD_U32 = 7, GEN_MOD_U64 = 8, GEN_SEQ_I32 = 9 };
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);

@@ -1288,6 +1288,25 @@ __global__ void k_gen_vals_i32(int32_t *out, uint64_t n, int64_t lo, uint64_t sp
   }
 }
 
+// Closed-form workload: key[i] = a bijective mix of (i mod D) (the fmix32 /
+// SplitMix64 finalizer of (i mod D) ^ salt) and value[i] = lo + i (two's
+// complement, low 32 bits).  Group k then holds c = floor((n-1-k)/D) + 1 rows
+// with SUM = c*(lo+k) + D*c*(c-1)/2, MIN = lo + k and MAX = lo + k + D*(c-1)
+// (while lo + n - 1 fits int32), so every group's aggregates are known
+// without aggregating.
+__global__ void k_gen_mod_u32(uint32_t *out, uint64_t n, uint64_t distinct, uint64_t salt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = fmix32((uint32_t)(i % distinct) ^ (uint32_t)salt);
+}
+__global__ void k_gen_mod_u64(uint64_t *out, uint64_t n, uint64_t distinct, uint64_t salt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = mix64((i % distinct) ^ salt);
+}
+__global__ void k_gen_seq_i32(int32_t *out, uint64_t n, uint64_t lo) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = (int32_t)(uint32_t)(lo + i);
+}
+
 // Zipf(s) ranks on [1, D] by rejection-inversion (Hoermann & Derflinger 1996),
 // mapped to distinct 64-bit keys mix64((rank-1) ^ salt).
 struct ZipfConst { double s, hx1, hn, sh; uint64_t D; };
@@ -2000,6 +2019,9 @@ hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, d
       hipLaunchKernelGGL(k_gen_zipf_u64, dim3(grid), dim3(256), 0, st, (uint64_t *)out, n, z, seed, salt);
       break;
     }
+    case GEN_MOD_U32: hipLaunchKernelGGL(k_gen_mod_u32, dim3(grid), dim3(256), 0, st, (uint32_t *)out, n, a, salt); break;
+    case GEN_MOD_U64: hipLaunchKernelGGL(k_gen_mod_u64, dim3(grid), dim3(256), 0, st, (uint64_t *)out, n, a, salt); break;
+    case GEN_SEQ_I32: hipLaunchKernelGGL(k_gen_seq_i32, dim3(grid), dim3(256), 0, st, (int32_t *)out, n, a); break;
     case GEN_NORMAL_F64: hipLaunchKernelGGL(k_gen_normal_f64, dim3(grid), dim3(256), 0, st, (double *)out, n, x, y, seed); break;
     case GEN_REF_U32: {
       // a = cardinality (1 <= a <= n, a < 2^32 - 1), b = dist (0 uniform, 1 Zipf, 2 self-similar), x = theta or h

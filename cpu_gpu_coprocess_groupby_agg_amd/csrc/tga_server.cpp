@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <unordered_set>
 
 #include "tga_batch_internal.hpp"
 
@@ -77,6 +78,7 @@ struct tga_server {
   std::condition_variable cv_work, cv_done;
   std::deque<Req> pending;
   std::map<int64_t, Done> done;
+  std::unordered_set<int64_t> live;  // tickets submitted and not yet taken
   std::vector<Flight> flights;
   std::vector<hipStream_t> streams;
   std::vector<bool> slot_busy;
@@ -111,10 +113,15 @@ bool tga_server::run_solo() {
     hipSetDevice(ctx->device);
     QuerySpec spec;
     build_spec(&r.q.key, r.vals.data(), (int32_t)r.vals.size(), r.aggs.data(), (int32_t)r.aggs.size(), spec);
-    const Plan p = autotune(ctx, spec, (uint64_t)r.q.n_rows, r.est, nullptr);
+    // the caller's hint, else the estimator (r.est is only the admission bound)
+    uint64_t est = r.q.est_groups > 0 ? (uint64_t)r.q.est_groups : 0;
+    if (!est) d.st = estimate_groups(ctx, r.q.key.data, spec.key_bytes, (uint64_t)r.q.n_rows, &est);
     PlainOut po;
     uint64_t G = 0;
-    d.st = run_plain(ctx, spec, r.q.key.data, (uint64_t)r.q.n_rows, p, &po, nullptr, nullptr, 0, &G);
+    if (!d.st) {
+      const Plan p = autotune(ctx, spec, (uint64_t)r.q.n_rows, est, nullptr);
+      d.st = run_plain(ctx, spec, r.q.key.data, (uint64_t)r.q.n_rows, p, &po, nullptr, nullptr, 0, &G);
+    }
     if (!d.st) {
       auto *res = new tga_result();
       res->ctx = ctx;
@@ -218,7 +225,9 @@ bool tga_server::launch_one() {
     qs[j] = take[j].q;
     qs[j].values = take[j].vals.data();
     qs[j].aggs = take[j].aggs.data();
-    qs[j].est_groups = (int64_t)take[j].est;
+    // the caller's hint (0: batch_prepare estimates it); take[j].est, every
+    // row its own group when there is no hint, only bounds the admission
+    qs[j].est_groups = take[j].q.est_groups;
   }
   Flight f;
   f.ex = std::make_unique<BatchExec>();
@@ -321,6 +330,7 @@ TGA_API tga_status tga_server_submit(tga_server *s, const tga_query *q, int64_t 
   if (!s->started) { s->started = true; s->t0 = r.t_submit; }
   r.ticket = s->next_ticket++;
   *ticket = r.ticket;
+  s->live.insert(r.ticket);
   ++s->submitted;
   s->pending.push_back(std::move(r));
   s->cv_work.notify_one();
@@ -334,11 +344,12 @@ static tga_status take_result(tga_server *s, int64_t ticket, tga_result **out, b
     if (it != s->done.end()) {
       Done d = it->second;
       s->done.erase(it);
+      s->live.erase(ticket);
       if (out) *out = d.r;
       else if (d.r) tga_result_free(d.r);
       return d.st;
     }
-    if (ticket <= 0 || ticket >= s->next_ticket) return TGA_ERR_INVALID_ARG;
+    if (!s->live.count(ticket)) return TGA_ERR_INVALID_ARG;  // never issued, or already taken
     if (!wait) return TGA_ERR_NOT_READY;
     s->cv_done.wait_for(g, std::chrono::milliseconds(5));
   }
@@ -399,7 +410,7 @@ TGA_API tga_status tga_server_destroy(tga_server *s) {
 // wait on the calling thread), then every result is awaited and freed.
 // The stats of the run are returned; wall_s receives first submit -> last done.
 TGA_API tga_status tga_server_replay(tga_server *s, const tga_query *qs, const double *arrival_us, int64_t n,
-                                     tga_server_stats *out, double *wall_s) {
+                                     tga_server_stats *out, double *wall_s, int64_t *groups) {
   if (!s || (n > 0 && (!qs || !arrival_us)) || n < 0) return TGA_ERR_INVALID_ARG;
   std::vector<int64_t> tickets((size_t)n, 0);
   const auto t0 = clk::now();
@@ -411,9 +422,15 @@ TGA_API tga_status tga_server_replay(tga_server *s, const tga_query *qs, const d
     if (st && !first) first = st;
   }
   for (int64_t i = 0; i < n; ++i) {
+    if (groups) groups[i] = -1;
     if (!tickets[(size_t)i]) continue;
-    tga_status st = tga_server_wait(s, tickets[(size_t)i], nullptr);
+    tga_result *r = nullptr;
+    tga_status st = tga_server_wait(s, tickets[(size_t)i], &r);
     if (st && !first) first = st;
+    if (r) {
+      if (groups) groups[i] = (int64_t)r->out.groups;
+      tga_result_free(r);
+    }
   }
   if (wall_s) *wall_s = std::chrono::duration<double>(clk::now() - t0).count();
   if (out) tga_server_stats_get(s, out);
@@ -423,7 +440,7 @@ TGA_API tga_status tga_server_replay(tga_server *s, const tga_query *qs, const d
 // The same trace through the plain operator, one query at a time in arrival
 // order on the calling thread (latency includes the wait for earlier queries).
 TGA_API tga_status tga_sequential_replay(tga_context *ctx, const tga_query *qs, const double *arrival_us, int64_t n,
-                                         tga_server_stats *out, double *wall_s) {
+                                         tga_server_stats *out, double *wall_s, int64_t *groups) {
   if (!ctx || (n > 0 && (!qs || !arrival_us)) || n < 0 || !out) return TGA_ERR_INVALID_ARG;
   std::vector<double> lat;
   const auto t0 = clk::now();
@@ -437,6 +454,7 @@ TGA_API tga_status tga_sequential_replay(tga_context *ctx, const tga_query *qs, 
     tga_status st = tga_groupby(ctx, &qs[i].key, qs[i].values, qs[i].n_values, qs[i].n_rows, qs[i].aggs, qs[i].n_aggs,
                                 &o, &r);
     if (st && !first) first = st;
+    if (groups) groups[i] = r ? (int64_t)r->out.groups : -1;
     if (r) tga_result_free(r);
     lat.push_back(std::chrono::duration<double, std::micro>(clk::now() - due).count());
   }

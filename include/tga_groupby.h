@@ -171,13 +171,14 @@ tga_status tga_server_stats_get(tga_server *s, tga_server_stats *st);
 tga_status tga_server_destroy(tga_server *s);
 /* Load generator: submits qs[i] at arrival_us[i] after the call starts
  * (busy wait on the calling thread), awaits and frees every result, and
- * returns the server's stats; *wall_s = first submit -> last completion. */
+ * returns the server's stats; *wall_s = first submit -> last completion;
+ * groups (optional, n entries) receives each request's group count. */
 tga_status tga_server_replay(tga_server *s, const tga_query *qs, const double *arrival_us, int64_t n,
-                             tga_server_stats *out, double *wall_s);
+                             tga_server_stats *out, double *wall_s, int64_t *groups);
 /* Baseline for the same trace: the plain operator, one query at a time in
  * arrival order (latency includes the wait behind earlier queries). */
 tga_status tga_sequential_replay(tga_context *ctx, const tga_query *qs, const double *arrival_us, int64_t n,
-                                 tga_server_stats *out, double *wall_s);
+                                 tga_server_stats *out, double *wall_s, int64_t *groups);
 
 /* ---- compat phase entry points (reference call sequence, u32 keys/values) --- */
 typedef struct tga_partset tga_partset;
@@ -223,9 +224,14 @@ typedef enum tga_gen_kind {
   TGA_GEN_VALS_I32 = 3,   /* a = lo (as int64 bits), b = hi - lo */
   TGA_GEN_ZIPF_U64 = 4,   /* a = distinct, x = exponent s */
   TGA_GEN_NORMAL_F64 = 5, /* x = mean, y = sigma */
-  TGA_GEN_REF_U32 = 6     /* u32 keys of the reference's workload (test/test.cu:96-129, data_generator.hpp:7-85):
+  TGA_GEN_REF_U32 = 6,    /* u32 keys of the reference's workload (test/test.cu:96-129, data_generator.hpp:7-85):
                              a = cardinality (exactly that many distinct keys), b = 0 uniform / 1 Zipf(theta < 1) /
                              2 self-similar(h) draws of the repeated keys, x = theta or h; rows shuffled */
+  TGA_GEN_MOD_U32 = 7,    /* closed-form keys: fmix32((i mod a) ^ salt), a = distinct */
+  TGA_GEN_MOD_U64 = 8,    /* closed-form keys: SplitMix64 finalizer of (i mod a) ^ salt, a = distinct */
+  TGA_GEN_SEQ_I32 = 9     /* values a + i (low 32 bits): with the MOD keys, group k has
+                             c = floor((n-1-k)/D) + 1 rows, SUM = c(a+k) + D c(c-1)/2,
+                             MIN = a+k, MAX = a+k+D(c-1) */
 } tga_gen_kind;
 /* Fills a device buffer with seeded counter-based synthetic data. */
 tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int64_t n, uint64_t a, uint64_t b, double x,

@@ -22,6 +22,8 @@ namespace tga {
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st);
 uint32_t slice_set_slots(uint64_t n);
+uint64_t combine_slice_estimate(uint64_t sample_est, uint64_t sample_d, uint64_t slice_count, bool slice_overflow,
+                                uint64_t n);
 uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n, double *top_share,
                              bool *reliable);
 hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
@@ -340,6 +342,24 @@ int check_estimate_rule() {
     const bool ok = rel == c.reliable && est >= c.lo && est <= c.hi;
     std::printf("estimate_rule_%s: estimate %llu, reliable %d %s\n", c.name, (unsigned long long)est, (int)rel,
                 ok ? "ok" : "FAIL");
+    bad += !ok;
+  }
+  // the slice count combined with the sample: a hot key with a short tail
+  // leaves the slice empty, yet the sample saw 50 keys; an overflowed slice
+  // set keeps the sample's estimate; a large slice count scales up, capped
+  // at the row count
+  struct S { const char *name; uint64_t est, d, c; bool ovf; uint64_t n, want; };
+  const S sc[] = {
+      {"hot_key_short_tail", 100000, 50, 0, false, 1ULL << 24, 50},
+      {"empty_slice_empty_sample", 1, 0, 0, false, 1ULL << 24, 1},
+      {"slice_overflow", 700000, 16000, 3, true, 1ULL << 28, 700000},
+      {"uniform_26m", 1600000, 16380, 26200, false, 1ULL << 28, 26200ULL << 10},
+      {"capped_at_rows", 5000, 4000, 9000, false, 1ULL << 22, 1ULL << 22},
+  };
+  for (const S &c : sc) {
+    const uint64_t got = tga::combine_slice_estimate(c.est, c.d, c.c, c.ovf, c.n);
+    const bool ok = got == c.want;
+    std::printf("estimate_rule_%s: %llu %s\n", c.name, (unsigned long long)got, ok ? "ok" : "FAIL");
     bad += !ok;
   }
   return bad;

@@ -174,6 +174,10 @@ def test_server_mix_equals_alone(ctx, budget):
             np.testing.assert_array_equal(k, alone["keys"])
             for x, y in zip(outs, alone["aggs"]):
                 np.testing.assert_array_equal(x, y)
+            ref = O.groupby(host[i][0], [host[i][1]], AGGS4)  # and the oracle
+            np.testing.assert_array_equal(k, ref["keys"])
+            for x, y in zip(outs, ref["aggs"]):
+                np.testing.assert_array_equal(x, y)
         st = tga.ServerStats()
         ctx._check(lib().tga_server_stats_get(srv, ctypes.byref(st)))
         assert st.completed == len(specs) and st.p99_us >= st.p50_us > 0
@@ -192,3 +196,70 @@ def test_server_rejects_over_budget(ctx):
     ctx._check(lib().tga_server_stats_get(srv, ctypes.byref(st)))
     assert st.rejected == 1
     lib().tga_server_destroy(srv)
+
+
+def test_server_poisson_arrivals_with_large_requests(ctx):
+    """Requests arrive on a seeded Poisson schedule (mean gap 300 us) while
+    earlier ones are on the GPU: 95% from the batched mix, 5% 2^26-row
+    int64-key requests (closed-form columns).  Every served result equals
+    the oracle (small requests) or its closed form (the large ones), and the
+    same small request alone through the plain operator."""
+    import time
+    from closed_form import expected
+    rng = np.random.default_rng(2026)
+    small = [s for s in SPECS if s[0] > 0]
+    qs_small, keep, host = _queries(ctx, small)
+    refs = [O.groupby(k, [v], AGGS4) for k, v in host]
+    big_n = 1 << 26
+    bigs = []
+    aggs4 = (tga.Agg * 4)(tga.Agg(tga.COUNT, 0), tga.Agg(tga.SUM, 0), tga.Agg(tga.MIN, 0), tga.Agg(tga.MAX, 0))
+    for j, (d, salt, hint) in enumerate(((1 << 20, 501, True), (3 << 16, 502, False))):
+        k = ctx.generate(tga.GEN_MOD_U64, big_n, a=d, salt=salt)
+        v = ctx.generate(tga.GEN_SEQ_I32, big_n, a=j * 1000)
+        vc = (tga.Column * 1)(v.column())
+        q = tga.Query()
+        q.key = k.column()
+        q.values = ctypes.cast(vc, ctypes.POINTER(tga.Column))
+        q.n_values = 1
+        q.aggs = ctypes.cast(aggs4, ctypes.POINTER(tga.Agg))
+        q.n_aggs = 4
+        q.n_rows = big_n
+        q.est_groups = d if hint else 0
+        keep += [k, v, vc]
+        bigs.append((q, expected(big_n, d, 64, salt, j * 1000)))
+    nreq = 120
+    kinds = []  # ("s", index) or ("b", index)
+    for i in range(nreq):
+        kinds.append(("b", int(rng.integers(len(bigs)))) if (rng.random() < 0.05 or i in (3, 40)) else
+                     ("s", int(rng.integers(len(small)))))
+    gaps = rng.exponential(300e-6, nreq)
+    srv = ctypes.c_void_p()
+    ctx._check(lib().tga_server_create(ctx.h, 64 << 30, 64, ctypes.byref(srv)))
+    try:
+        tickets = []
+        t_next = time.perf_counter()
+        for (kind, j), gap in zip(kinds, gaps):
+            t_next += gap
+            while time.perf_counter() < t_next:
+                pass
+            q = qs_small[j] if kind == "s" else bigs[j][0]
+            t = ctypes.c_int64()
+            ctx._check(lib().tga_server_submit(srv, ctypes.byref(q), ctypes.byref(t)))
+            tickets.append(t.value)
+        for (kind, j), t in zip(kinds, tickets):
+            r = ctypes.c_void_p()
+            ctx._check(lib().tga_server_wait(srv, t, ctypes.byref(r)))
+            if kind == "s":
+                k, outs = _result_arrays(r, np.uint32 if small[j][2] == 32 else np.uint64)
+                ek, ea = refs[j]["keys"], refs[j]["aggs"]
+            else:
+                k, outs = _result_arrays(r, np.uint64)
+                ek, ea = bigs[j][1]
+            np.testing.assert_array_equal(k, ek)
+            for x, y in zip(outs, ea):
+                np.testing.assert_array_equal(x, y)
+        st = tga.ServerStats()
+        ctx._check(lib().tga_server_stats_get(srv, ctypes.byref(st)))
+        assert st.completed == nreq and st.inflight == 0
+    finally:
+        lib().tga_server_destroy(srv)

@@ -277,3 +277,45 @@ def test_zipf_hot_keys_int_and_fp64(ctx, path):
     top = int(np.argmax(got["aggs"][0]))
     assert got["aggs"][0][top] > n // 20
     assert s[top] == math.fsum(fh[kh == got["keys"][top]].tolist())
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("path", [tga.PATH_AUTO, tga.PATH_DIRECT, tga.PATH_PARTITIONED])
+def test_closed_form_columns_every_group(ctx, bits, path):
+    """Closed-form columns (key = a bijective mix of i mod D, value = lo + i):
+    the device generators equal the host restatement, and every group's
+    COUNT / SUM / MIN / MAX equals its closed form."""
+    from closed_form import columns, expected
+    for n, D, lo in ((1_000_003, 1 << 10, -(1 << 20)), (3_000_000, 50_000, 7)):
+        kind = tga.GEN_MOD_U32 if bits == 32 else tga.GEN_MOD_U64
+        k = ctx.generate(kind, n, a=D, salt=0xC0DE)
+        v = ctx.generate(tga.GEN_SEQ_I32, n, a=lo & (2**64 - 1))
+        if n <= 1_000_003:
+            kh, vh = columns(n, D, bits, 0xC0DE, lo)
+            np.testing.assert_array_equal(k.to_numpy(), kh)
+            np.testing.assert_array_equal(v.to_numpy(), vh)
+        got = ctx.groupby(k, [v], AGGS4, path=path, est_groups=D)
+        ek, ea = expected(n, D, bits, 0xC0DE, lo)
+        np.testing.assert_array_equal(got["keys"], ek)
+        for g, e in zip(got["aggs"], ea):
+            np.testing.assert_array_equal(g, e)
+
+
+@pytest.mark.parametrize("path", [tga.PATH_DIRECT, tga.PATH_PARTITIONED])
+@pytest.mark.parametrize("keys", ["one_hot_key", "zipf"])
+def test_wave_fold_on(ctx, path, keys):
+    """wave_fold=1: equal keys of a wave are folded into one lane (ballots on
+    the slot bits) before the LDS atomics.  Bit-exact against the oracle on
+    maximum contention (one key) and Zipf(1.1) hot keys."""
+    n = 1 << 21
+    if keys == "one_hot_key":
+        kh = np.full(n, 0x1234_5678_9ABC, np.uint64)
+        kh[::1000] = np.arange(kh[::1000].size, dtype=np.uint64)  # a few cold keys beside it
+        k = ctx.upload(kh)
+    else:
+        k = ctx.generate(tga.GEN_ZIPF_U64, n, a=1 << 16, x=1.1, seed=71, salt=3)
+        kh = k.to_numpy()
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=72)
+    got = ctx.groupby(k, [v], AGGS4, path=path, wave_fold=1)
+    assert ctx.last_plan()["wave_fold"] == 1
+    check_equal(got, O.groupby(kh, [v.to_numpy()], AGGS4))
