@@ -390,7 +390,7 @@ tga_status batch_prepare(tga_context *ctx, const tga_query *queries, int32_t n, 
     b.rows[i] = (uint64_t)q.n_rows;
     b.keys[i] = q.key.data;
   }
-  // estimates: the caller's hint, else up to 1024 strided keys per query
+  // estimates: the caller's hint, else up to 1024 keys per query at random rows
   // copied in one transfer and fed to the GEE estimator
   b.est.assign(n, 0);
   std::vector<int32_t> todo;

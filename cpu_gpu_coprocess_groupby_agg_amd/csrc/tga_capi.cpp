@@ -474,9 +474,19 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   // frequencies, 2^10 times that count is within ~(D/2^10)^-1/2 of D.
   // 2^28 rows of Zipf(0.9) keys over 26.8M groups: the sample says 1.6M,
   // and the plan for it is abandoned and run again (7.5 ms instead of 5.3).
+  uint64_t count = 0;
+  bool overflow = false;
+  if (tga_status e = slice_count(ctx, d_keys, key_bytes, n, &count, &overflow)) return e;
+  *est = combine_slice_estimate(*est, sample_d, count, overflow, n);
+  return TGA_OK;
+}
+
+tga_status slice_count(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *count,
+                       bool *overflow) {
   const uint32_t slots = slice_set_slots(n);
-  DBuf set;
-  if (!set.alloc(ctx, (uint64_t)slots * key_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "slice set");
+  DBuf set, buf;
+  if (!set.alloc(ctx, (uint64_t)slots * key_bytes) || !buf.alloc(ctx, 64))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "slice set");
   TGA_HIP(ctx, timed(ctx, "slice_distinct", [&] {
     return launch_slice_distinct(key_bytes, d_keys, n, set.p, slots, buf.as<uint64_t>(), (unsigned)ctx->cus * 2,
                                  ctx->stream);
@@ -484,7 +494,8 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   uint64_t c[2];
   TGA_HIP(ctx, hipMemcpyAsync(c, buf.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  *est = combine_slice_estimate(*est, sample_d, c[0], c[1] != 0, n);
+  *count = c[0];
+  *overflow = c[1] != 0;
   return TGA_OK;
 }
 
@@ -1016,6 +1027,7 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
     }
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table kept overflowing");
   }
+  bool sliced = false;
   for (int attempt = 0; attempt < 6; ++attempt) {
     GTable g{};
     std::vector<DBuf> gbufs;
@@ -1060,8 +1072,24 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
                   (unsigned long long)ext_cap);
     }
     if (hflags[FLAG_OVERFLOW]) {
-      // the estimate was too low: grow the global table and run again
-      plan.gslots *= 4;
+      // The estimate was too low: the table held more keys than half its
+      // slots.  Plan again for at least 4x as many groups -- over 2^22 rows
+      // from the exact count of a hash slice of the keys, which no row order
+      // or key frequency misleads -- possibly on the partitioned path.
+      uint64_t est2 = plan.gslots * 2;
+      if (!sliced && n >= kSliceMinRows) {
+        uint64_t c = 0;
+        bool ovf = false;
+        if (tga_status e = slice_count(ctx, d_keys, q.key_bytes, n, &c, &ovf)) return e;
+        est2 = std::max(est2, combine_slice_estimate(est2, 0, c, ovf, n));
+        sliced = true;
+      }
+      const uint64_t grown = plan.gslots * 4;
+      Plan np = autotune(ctx, q, n, std::min<uint64_t>(est2, n), &plan.opt);
+      if (np.path == TGA_PATH_PARTITIONED)
+        return run_plain(ctx, q, d_keys, n, np, out, ext_keys, ext_cols, ext_cap, groups);
+      plan = np;
+      plan.gslots = std::max(plan.gslots, grown);
       continue;
     }
     if (hflags[FLAG_RANGE]) return fail(ctx, TGA_ERR_RANGE, "fp64 SUM input outside [2^-96, 2^64) or not finite");

@@ -130,6 +130,14 @@ TGA_HD uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
   return z ^ (z >> 31);
 }
+// Row of the i-th of m sample keys over n rows: every row when m >= n,
+// else rows drawn uniformly at random (with replacement, a fixed SplitMix64
+// stream).  Evenly strided rows alias with periodic inputs: keys repeating
+// every 2^20 rows, sampled every 2^14 rows, show only 64 keys.
+TGA_HD uint64_t sample_row(uint64_t i, uint64_t m, uint64_t n) {
+  if (m >= n) return i;
+  return (uint64_t)(((unsigned __int128)mix64(0x6a09e667f3bcc909ULL + (i + 1) * 0x9e3779b97f4a7c15ULL) * n) >> 64);
+}
 // MurmurHash3 fmix32: a bijection on 32 bits.
 TGA_HD uint32_t fmix32(uint32_t h) {
   h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;

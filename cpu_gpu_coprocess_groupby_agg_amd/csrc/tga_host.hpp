@@ -167,7 +167,7 @@ struct Plan {
 Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint64_t est_groups,
               const tga_options *opt);
 
-// Cardinality estimate from a strided key sample (GEE estimator,
+// Cardinality estimate from a random key sample (GEE estimator,
 // Charikar et al. 2000, or the equal-frequency model), replaced by an exact
 // count over a 2^-10 hash slice of all keys when the sample leaves the
 // count open.  Needs the context lock.
@@ -183,6 +183,11 @@ uint64_t estimate_from_sample(uint64_t *v, uint64_t m, uint64_t n, double *top_s
 // the most frequent seen top times.
 uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, uint64_t n,
                              double *top_share = nullptr, bool *reliable = nullptr);
+// Exact distinct count of the keys in one 2^-kSliceBits slice of the
+// partition hash, over all n rows (*overflow: the set filled up, a lower
+// bound).  Needs the context lock.
+tga_status slice_count(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *count,
+                       bool *overflow);
 // The estimate after the hash-slice count: 2^kSliceBits times the slice's
 // distinct keys, at least the sample's distinct keys sample_d; the sample's
 // estimate when the slice set overflowed.

@@ -124,7 +124,7 @@ hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, vo
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 // out[0..3] = {distinct keys, keys seen once, largest count of one key,
-// overflow} of the m <= 16384 keys sampled at rows i * n / m (gathered into
+// overflow} of the m <= 16384 keys sampled at random rows (all rows when m >= n; gathered into
 // smp[m], then counted in an LDS hash table by one workgroup; overflow 1:
 // the counts are incomplete, smp still holds the sample)
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,

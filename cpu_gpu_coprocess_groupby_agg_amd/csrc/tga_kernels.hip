@@ -1406,11 +1406,13 @@ __global__ void k_gen_ref_u32(uint32_t *out, RefGenConst c) {
   }
 }
 
-// strided key sample for the cardinality estimate
+// key sample for the cardinality estimate
 template <typename K>
 __global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = (uint64_t)keys[(uint64_t)(((unsigned __int128)i * n) / m)];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t row = sample_row(i, m, n);
+    out[i] = (uint64_t)keys[row];
+  }
 }
 
 // Statistics of a key sample for the cardinality estimate, in one

@@ -213,7 +213,7 @@ int check_sample_stats() {
   for (const S &c : sets) {
     const uint64_t n = c.k.size();
     std::vector<uint64_t> smp(c.m);
-    for (uint32_t i = 0; i < c.m; ++i) smp[i] = c.k[(uint64_t)(((unsigned __int128)i * n) / c.m)];
+    for (uint32_t i = 0; i < c.m; ++i) smp[i] = c.k[tga::sample_row(i, c.m, n)];
     std::sort(smp.begin(), smp.end());
     uint64_t d = 0, f1 = 0, top = 0;
     for (uint64_t i = 0; i < c.m;) {
@@ -426,6 +426,21 @@ int main(int argc, char **argv) {
     for (int64_t i = 0; i < N; i += 2) k[i] = 77;
     cases.push_back({"replan_skew_split", 64, k, {col_i32(N, -9, 9, 32)}, {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]},
                      opts(TGA_PATH_PARTITIONED, 2, 0, 1024, 1000)});
+  }
+  {
+    // Periodic rows, no hint: the key of row i is a mix of (i mod 2^15)
+    // over 2^22 rows.  A sample strided by n/m = 2^8 rows would see only 128
+    // keys; the random-row sample sees the true spread and plans for it.
+    const int64_t n2 = 1LL << 22;
+    std::vector<uint64_t> k(n2);
+    for (int64_t i = 0; i < n2; ++i) k[i] = (uint64_t)(i % 32768) * 0x9E3779B97F4A7C15ULL + 11;
+    cases.push_back({"periodic_no_hint", 64, k, {col_i32(n2, -9, 9, 35)}, {TGA_I32}, {a4[0], a4[1]},
+                     opts(TGA_PATH_AUTO, 2)});
+    // The same rows with a hint of 10 groups: the direct path's table
+    // overflows, the slice count of the keys gives the group count, and the
+    // call is planned again on the partitioned path.
+    cases.push_back({"direct_overflow_replan", 64, k, {col_i32(n2, -9, 9, 36)}, {TGA_I32}, {a4[0], a4[1]},
+                     opts(TGA_PATH_AUTO, 2, 0, 0, 10)});
   }
   // 2^10 partitions (more than one per scatter thread) over enough tiles
   // that the waves refill their bucket reserves several times
