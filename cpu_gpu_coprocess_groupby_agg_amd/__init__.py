@@ -104,6 +104,8 @@ def lib():
             "tga_device_info": [vp, ctypes.c_char_p, sz, P(i32), P(i64)],
             "tga_groupby": [vp, P(Column), P(Column), i32, i64, P(Agg), i32, P(Options), P(vp)],
             "tga_groupby_into": [vp, P(Column), P(Column), i32, i64, P(Agg), i32, P(Options), vp, P(vp), i64, P(i64)],
+            "tga_groupby_into_noalloc": [vp, P(Column), P(Column), i32, i64, P(Agg), i32, P(Options), vp, P(vp), i64,
+                                         P(i64)],
             "tga_result_num_groups": [vp, P(i64)],
             "tga_result_column": [vp, i32, P(vp), P(i32)],
             "tga_result_copy_to_host": [vp, i32, vp, i64],

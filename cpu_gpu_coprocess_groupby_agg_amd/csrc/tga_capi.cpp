@@ -1194,6 +1194,9 @@ TGA_API tga_status tga_context_create(int32_t device, uint64_t mem_budget_bytes,
     budget = (uint64_t)(fr * 0.9);
   }
   c->arena = new Arena(budget);
+  // the no-alloc entry point's scratch address on this device (once per
+  // device; a failure only disables that entry point)
+  (void)noalloc_scratch_init(device);
   if (hipHostMalloc((void **)&c->status_host, kStatusWords * 4, hipHostMallocDefault) != hipSuccess) {
     delete c->arena;
     hipStreamDestroy(c->stream);

@@ -81,6 +81,10 @@ struct tga_context {
 
 namespace tga {
 
+// Looks up, once per device, the device address of the no-alloc entry
+// point's scratch array (the device must be current).
+hipError_t noalloc_scratch_init(int device);
+
 // Sets the context's error text (printf-style) and returns `st`.
 tga_status fail(tga_context *ctx, tga_status st, const char *fmt, ...);
 

@@ -114,6 +114,25 @@ tga_status tga_groupby_into(tga_context *ctx, const tga_column *key, const tga_c
                             int64_t n_rows, const tga_agg *aggs, int32_t n_aggs, const tga_options *opt,
                             void *out_keys, void *const *out_aggs, int64_t capacity, int64_t *n_groups);
 
+/* Same as tga_groupby_into, allocating nothing: no hipMalloc, no arena, no
+ * allocating library call.  No counterpart in the reference; it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  Its temporaries live in a fixed 32 MiB
+ * device array that each GPU holds until the process ends, so:
+ *   - key and value columns must be device-resident (TGA_DEVICE);
+ *   - it runs the direct path (LDS tables flushed into one global table)
+ *     and holds at most half the table's slots in groups: 2^18 groups for a
+ *     64-bit key with COUNT/SUM/MIN/MAX of an int32 column (fewer with fp64
+ *     SUM); a larger est_groups hint or a path of TGA_PATH_PARTITIONED is
+ *     rejected, and more groups than that return TGA_ERR_OUT_OF_MEMORY;
+ *   - every call on one GPU shares the array: calls on one context
+ *     serialise, calls that overlap on one GPU from different contexts race.
+ * Results equal tga_groupby_into's. */
+tga_status tga_groupby_into_noalloc(tga_context *ctx, const tga_column *key, const tga_column *values,
+                                    int32_t n_values, int64_t n_rows, const tga_agg *aggs, int32_t n_aggs,
+                                    const tga_options *opt, void *out_keys, void *const *out_aggs, int64_t capacity,
+                                    int64_t *n_groups);
+
 tga_status tga_result_num_groups(const tga_result *r, int64_t *n_groups);
 /* index -1 = key column, else aggregate index.  Device pointer + dtype. */
 tga_status tga_result_column(const tga_result *r, int32_t index, const void **device_ptr, int32_t *dtype);

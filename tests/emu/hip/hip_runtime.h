@@ -145,6 +145,9 @@ hipError_t hipHostFree(void *p);
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t n, hipMemcpyKind k, hipStream_t s = nullptr);
 hipError_t hipMemcpy(void *dst, const void *src, size_t n, hipMemcpyKind k);
 hipError_t hipMemGetInfo(size_t *fr, size_t *tot);
+// file-scope __device__ arrays are host arrays here: their address is the symbol's
+#define HIP_SYMBOL(x) (&(x))
+template <class T> inline hipError_t hipGetSymbolAddress(void **p, T *sym) { *p = (void *)sym; return hipSuccess; }
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int dev);
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
 hipError_t hipSetDevice(int d);

@@ -365,6 +365,88 @@ int check_estimate_rule() {
   return bad;
 }
 
+// tga_groupby_into_noalloc (temporaries in the fixed scratch array) against
+// tga_groupby_into, two calls in a row with the second on fewer rows, both
+// against the oracle; and its limits: a partitioned path is refused.
+int check_noalloc(tga_context *ctx) {
+  int bad = 0;
+  const int64_t n = 300000;
+  const std::vector<uint64_t> k = keys_u64(n, 20000, 61);
+  const std::vector<unsigned char> v = col_i32(n, -(1LL << 31), (1LL << 31) - 1, 62);
+  void *dk = nullptr, *dv = nullptr;
+  if (tga_device_alloc(ctx, n * 8, &dk) || tga_device_alloc(ctx, n * 4, &dv) ||
+      tga_memcpy_h2d(ctx, dk, k.data(), n * 8) || tga_memcpy_h2d(ctx, dv, v.data(), n * 4)) {
+    std::printf("noalloc: setup failed\n");
+    return 1;
+  }
+  const tga_column key{dk, TGA_U64, TGA_DEVICE}, val{dv, TGA_I32, TGA_DEVICE};
+  const tga_agg aggs[4] = {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}};
+  const int64_t cap = 20000;
+  for (int64_t rows : {n, n / 3 + 17}) {
+    for (int hint : {0, 1}) {
+      tga_options o = opts(TGA_PATH_AUTO, 2, 0, 0, hint ? 20000 : 0);
+      std::vector<void *> out[2];
+      int64_t g[2] = {-1, -1};
+      tga_status st[2];
+      for (int w = 0; w < 2; ++w) {
+        out[w].resize(5);
+        tga_device_alloc(ctx, cap * 8, &out[w][0]);
+        for (int a = 0; a < 4; ++a) tga_device_alloc(ctx, cap * 8, &out[w][1 + a]);
+        void *cols[4] = {out[w][1], out[w][2], out[w][3], out[w][4]};
+        st[w] = (w == 0 ? tga_groupby_into_noalloc : tga_groupby_into)(ctx, &key, &val, 1, rows, aggs, 4, &o,
+                                                                        out[w][0], cols, cap, &g[w]);
+      }
+      // oracle
+      std::vector<uint64_t> ok(rows);
+      std::vector<int64_t> oc(rows), os(rows);
+      std::vector<int32_t> omin(rows), omax(rows);
+      const void *cp[1] = {v.data()};
+      const int cdt[1] = {TGA_I32}, ops[4] = {TGA_COUNT, TGA_SUM, TGA_MIN, TGA_MAX}, acs[4] = {0, 0, 0, 0};
+      void *outs[4] = {oc.data(), os.data(), omin.data(), omax.data()};
+      const int64_t G = tga_o_groupby(k.data(), 8, rows, cp, cdt, 1, ops, acs, 4, ok.data(), outs, rows);
+      int mism = 0;
+      if (st[0] || st[1] || g[0] != G || g[1] != G) mism = 1;
+      for (int w = 0; w < 2 && !mism; ++w) {
+        std::vector<uint64_t> gk(G);
+        std::vector<int64_t> gc(G), gs(G);
+        std::vector<int32_t> gmin(G), gmax(G);
+        tga_memcpy_d2h(ctx, gk.data(), out[w][0], G * 8);
+        tga_memcpy_d2h(ctx, gc.data(), out[w][1], G * 8);
+        tga_memcpy_d2h(ctx, gs.data(), out[w][2], G * 8);
+        tga_memcpy_d2h(ctx, gmin.data(), out[w][3], G * 4);
+        tga_memcpy_d2h(ctx, gmax.data(), out[w][4], G * 4);
+        std::vector<int64_t> ord(G);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return gk[x] < gk[y]; });
+        for (int64_t i = 0; i < G; ++i) {
+          const int64_t j = ord[i];
+          if (gk[j] != ok[i] || gc[j] != oc[i] || gs[j] != os[i] || gmin[j] != omin[i] || gmax[j] != omax[i]) ++mism;
+        }
+      }
+      for (int w = 0; w < 2; ++w)
+        for (void *p : out[w]) tga_device_free(ctx, p);
+      std::printf("noalloc_rows%lld_hint%d: status %d/%d, %lld/%lld groups, oracle %lld %s\n", (long long)rows, hint,
+                  (int)st[0], (int)st[1], (long long)g[0], (long long)g[1], (long long)G, mism ? "FAIL" : "ok");
+      bad += mism != 0;
+    }
+  }
+  tga_options part = opts(TGA_PATH_PARTITIONED, 2);
+  void *ok1 = nullptr, *oc1 = nullptr;
+  tga_device_alloc(ctx, cap * 8, &ok1);
+  tga_device_alloc(ctx, cap * 8, &oc1);
+  void *c1[1] = {oc1};
+  int64_t g1 = 0;
+  const tga_agg cnt[1] = {{TGA_COUNT, 0}};
+  const tga_status sp = tga_groupby_into_noalloc(ctx, &key, &val, 1, n, cnt, 1, &part, ok1, c1, cap, &g1);
+  std::printf("noalloc_refuses_partitioned: status %d %s\n", (int)sp, sp == TGA_ERR_UNSUPPORTED ? "ok" : "FAIL");
+  bad += sp != TGA_ERR_UNSUPPORTED;
+  tga_device_free(ctx, ok1);
+  tga_device_free(ctx, oc1);
+  tga_device_free(ctx, dk);
+  tga_device_free(ctx, dv);
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -503,7 +585,6 @@ int main(int argc, char **argv) {
     bad += run_case(ctx, c);
     ++ran;
   }
-  tga_context_destroy(ctx);
   if (filter.empty() || std::string("sample_stats").find(filter) != std::string::npos) {
     bad += check_sample_stats();
     ++ran;
@@ -516,6 +597,11 @@ int main(int argc, char **argv) {
     bad += check_estimate_rule();
     ++ran;
   }
+  if (filter.empty() || std::string("noalloc").find(filter) != std::string::npos) {
+    bad += check_noalloc(ctx);
+    ++ran;
+  }
+  tga_context_destroy(ctx);
   std::printf(bad ? "FAIL %d of %d\n" : "OK %d\n", bad ? bad : ran, ran);
   return bad ? 1 : 0;
 }
