@@ -160,7 +160,11 @@ __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uin
   if (threadIdx.x == 0) *spilled = FX ? 1u : 0u;  // fp64 limbs: always through the global table
   __syncthreads();
   if constexpr (FX) {
-    agg_rows<K, FX, NC>((const K *)keys, s, b, e, t, Q.g, Q.nbits, Q.max_probe);
+    const void *cols[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
+    agg_rows_int<K, NC, 0, false, 0, true>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
+                                           nullptr, spilled);
   } else {
     const void *cols[NC];
 #pragma unroll

@@ -257,6 +257,7 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   const size_t lds_max = ctx->lds_max - 1024;
   if (est < 1) est = 1;
   if (est > n && n > 0) est = n;
+  p.est = est;
   // largest power-of-two table fitting one workgroup's LDS
   uint32_t tmax = 64;
   while ((uint64_t)(tmax * 2 + 2) * sb + 256 <= lds_max) tmax *= 2;
@@ -990,7 +991,11 @@ tga_status run_plain(tga_context *ctx, const QuerySpec &q, const void *d_keys, u
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
   if (plan.path == TGA_PATH_PARTITIONED && n > 0 && n < (1ULL << 31)) {
-    bool grown = false;
+    // Partitions expected to fill their LDS tables past 3/4 (a fanout capped
+    // below the estimate's) spill rows to the global table, which the
+    // split-partition sizing does not count: the full-size table from the
+    // start, instead of a run that overflows a reduced one.
+    bool grown = plan.est * 4 > ((uint64_t)plan.table_slots << plan.pbits) * 3;
     for (int attempt = 0; attempt < 8; ++attempt) {
       bool overflow = false, reduced = false;
       uint64_t replan = 0;

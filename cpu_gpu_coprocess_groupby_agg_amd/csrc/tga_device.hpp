@@ -726,7 +726,15 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
   const int64_t gs = g_find_or_insert<K>(*g, key, key_hash((uint64_t)key));
   if (gs < 0) return;
   const uint64_t cell[kMaxCols] = {raw0, raw1, raw2, raw3};
-  for (int a = 0; a < s->nacc; ++a) g_apply(*g, a, s->acc[a].kind, gs, acc_operand(*s, a, cell));
+  for (int a = 0; a < s->nacc; ++a) {
+    if (s->acc[a].kind == ACC_SUM_F) {
+      int64_t lv[kLimbs];
+      fx_expand(cell[s->acc[a].col], lv, g->flags);
+      g_apply_limbs(*g, a, gs, lv);
+    } else {
+      g_apply(*g, a, s->acc[a].kind, gs, acc_operand(*s, a, cell));
+    }
+  }
 }
 
 // Aggregates rows [b, e) into the LDS table for schemas without fp64 SUM.
@@ -850,7 +858,11 @@ __device__ __forceinline__ uint32_t prog_map(const AccProg &ap, int a) {
   else return (ap.maps >> (2 * a)) & 3u;
 }
 
-template <typename K, int NC, int VW, bool FOLD, uint32_t SIG = 0>
+// FXA: the schema may hold exact fp64 SUM accumulators (never with FOLD):
+// a row's double is split into at most three signed 32-bit chunks of its
+// fixed-point image (fx_split) and each nonzero chunk is added to its limb
+// with one return-less LDS atomic, as the integer accumulators are.
+template <typename K, int NC, int VW, bool FOLD, uint32_t SIG = 0, bool FXA = false>
 __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const void *const *cols, uint64_t b0,
                                              uint64_t e0, const LdsTable<K> &t, const AccProg &ap, int nbits,
                                              int max_probe, const GTable *gsp, const Schema *ssp,
@@ -1051,6 +1063,26 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
 #pragma unroll
             for (int u = 0; u < U; ++u)
               atomicMax((long long *)arr + sl[u], (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col)));
+            break;
+          case ACC_SUM_F:
+            if constexpr (FXA) {
+              // limb l of slot s at l * (slots + 1) + s; rows without a slot
+              // add nothing (the discard entry has no limb rows)
+              unsigned long long *lp = (unsigned long long *)arr;
+#pragma unroll
+              for (int u = 0; u < U; ++u) {
+                if (slot[u] < 0) continue;
+                int64_t c[3];
+                int range = 0;
+                const int i0 = fx_split(pick_col<NC>(raw[u], col), c, &range);
+                if (range) atomicOr(&gsp->flags[FLAG_RANGE], 1u);
+                if (i0 < 0) continue;
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                  if (c[j] != 0 && i0 + j < kLimbs)
+                    atomicAdd(lp + (uint32_t)(i0 + j) * (S + 1) + (uint32_t)slot[u], (unsigned long long)c[j]);
+              }
+            }
             break;
           default: break;
         }

@@ -152,6 +152,7 @@ struct Plan {
   uint32_t block = 256;
   uint64_t chunk_rows = 0;
   uint64_t gslots = 0;  // global table slots
+  uint64_t est = 0;     // the group estimate the plan was made for
   Layout layout{};
   int nbits = 0;
   int max_probe = 0;

@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     // abandoned and recorded in ovf = {count, capacity, (rows consumed, rows)...}
     // instead of spilling: the host then plans a finer partitioning
     const bool whole = emit && nch == 1;
-    const bool abandon = !FX && ovf != nullptr;
+    const bool abandon = ovf != nullptr;
     lds_init(t, s);
     if (threadIdx.x == 0) {
       s_abort[0] = 0;
@@ -137,14 +137,12 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
     }
     __syncthreads();
     const BucketList bl{plan.blist};
-    if constexpr (FX) {
-      agg_rows<K, FX, NC>(keys, s, b, e, t, g, nbits, max_probe, bl);
-    } else {
+    {
       const void *cols[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
-      agg_rows_int<K, NC, VW, FOLD, SIG>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
-                                         abandon ? s_abort : nullptr, nullptr, bl);
+      agg_rows_int<K, NC, VW, FX ? false : FOLD, SIG, FX>(keys, cols, b, e, t, ap, nbits, max_probe, &s_g, &s_s,
+                                                          abandon ? s_abort : nullptr, nullptr, bl);
     }
     // every wave has read s_item / s_seg (before the barrier after lds_init)
     if (threadIdx.x == 0) take_item();
@@ -1670,8 +1668,11 @@ static uint32_t fixed_signature(const Schema &s) {
 
 // One kernel per (key width, fp64 SUM present, value columns, wave folding):
 // each variant gets its own register allocation, and a single value column
-// of a known width is read without branches.  fp64 SUM accumulators always
-// fold (their limb vectors make per-row atomics costly).
+// of a known width is read without branches.  fp64 SUM accumulators never
+// fold: a row adds at most three 32-bit chunks to its limbs with return-less
+// LDS atomics (folding the limb vectors across a wave -- ten 64-bit shuffles
+// per duplicate lane -- cost ~6000 VALU instructions per 64 rows under
+// Zipf(1.1) keys).
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
                       int emit, const OutDesc *od, uint64_t *emit_count, int fold, uint32_t *ovf, int rec) {
@@ -1734,11 +1735,20 @@ size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t til
   return b + row * tile_rows;
 }
 
-template <typename K, int NC, int VW, bool BK, bool REC = false>
-static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
-                                   uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs *bb,
-                                   const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
-  constexpr int ITEMS = 8;
+// Rows per thread of the plain scatter: 8 down to 1, so a tile of 1024 to
+// 8192 rows always has 1024 threads.  At high fanouts the per-partition LDS
+// arrays leave room for 1024-row tiles only; with 8 rows per thread those ran
+// on 128 threads, and the per-tile work over all partitions (scan, bucket
+// chains) fell on 2 waves (2^29 rows of a three-column schema at 2^12
+// partitions: 80 ms).
+static int scatter_items(uint32_t tile_rows) {
+  return tile_rows >= 8192 ? 8 : tile_rows >= 4096 ? 4 : tile_rows >= 2048 ? 2 : 1;
+}
+
+template <typename K, int NC, int VW, bool BK, bool REC, int ITEMS>
+static hipError_t launch_scatter_items(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                       uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs *bb,
+                                       const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
   const unsigned block = tile_rows / ITEMS;
   const size_t lds = scatter_lds_bytes(s, sizeof(K), pbits, tile_rows);
   hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter<K, ITEMS, NC, VW, BK, REC>, lds);
@@ -1749,13 +1759,25 @@ static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n
   return hipGetLastError();
 }
 
+template <typename K, int NC, int VW, bool BK, bool REC = false>
+static hipError_t launch_scatter_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
+                                   uint32_t ntiles, void *okeys, const PartOut &o, const BucketBufs *bb,
+                                   const uint32_t *offs, const uint32_t *pbase, unsigned grid, hipStream_t st) {
+  switch (scatter_items(tile_rows)) {
+    case 8: return launch_scatter_items<K, NC, VW, BK, REC, 8>(keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st);
+    case 4: return launch_scatter_items<K, NC, VW, BK, REC, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st);
+    case 2: return launch_scatter_items<K, NC, VW, BK, REC, 2>(keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st);
+    default: return launch_scatter_items<K, NC, VW, BK, REC, 1>(keys, s, n, pbits, tile_rows, ntiles, okeys, o, bb, offs, pbase, grid, st);
+  }
+}
+
 unsigned bucket_grid(uint32_t ntiles, unsigned max_grid) {
   // at least 8 tiles per workgroup, so its buckets fill up
   unsigned g = std::max<unsigned>(1, std::min<unsigned>(max_grid, ntiles / 8));
   return xcd_grid(g);
 }
 
-unsigned scatter_block(uint32_t tile_rows, int wc) { return wc ? 1024u : tile_rows / 8; }
+unsigned scatter_block(uint32_t tile_rows, int wc) { return wc ? 1024u : tile_rows / (unsigned)scatter_items(tile_rows); }
 
 uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block) {
   return (n + kBucketRows - 1) / kBucketRows + (uint64_t)grid * ((1ULL << pbits) + block) + 1;

@@ -524,6 +524,20 @@ int main(int argc, char **argv) {
     cases.push_back({"direct_overflow_replan", 64, k, {col_i32(n2, -9, 9, 36)}, {TGA_I32}, {a4[0], a4[1]},
                      opts(TGA_PATH_AUTO, 2, 0, 0, 10)});
   }
+  {
+    // exact fp64 SUM beside an int32 SUM: half the rows on one key (its
+    // partition split into chunks that flush to the global table), a far
+    // too small hint (partitions abandoned and planned again), a tiny LDS
+    // table in a second run (rows spill to the global table)
+    std::vector<uint64_t> k = keys_u64(N, 30000, 37);
+    for (int64_t i = 0; i < N; i += 2) k[i] = 4242;
+    cases.push_back({"fp64_skew_replan", 64, k, {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 38), col_f64(N, 39)},
+                     {TGA_I32, TGA_F64}, {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_SUM, 1}},
+                     opts(TGA_PATH_PARTITIONED, 2, 0, 0, 300)});
+    cases.push_back({"fp64_spill", 64, k, {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 40), col_f64(N, 41)},
+                     {TGA_I32, TGA_F64}, {{TGA_COUNT, 0}, {TGA_SUM, 1}, {TGA_SUM, 0}},
+                     opts(TGA_PATH_PARTITIONED, 2, 3, 256, 30000)});
+  }
   // 2^10 partitions (more than one per scatter thread) over enough tiles
   // that the waves refill their bucket reserves several times
   cases.push_back({"buckets_p1024", 64, keys_u64(3 * N, 200000, 33), {col_i32(3 * N, -99, 99, 34)}, {TGA_I32},
