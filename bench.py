@@ -64,7 +64,8 @@ def parse():
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-rate", type=float, default=20_000.0)
-    ap.add_argument("--only", choices=["all", "headline", "copy", "workloads", "config1", "config3", "batch", "serve"],
+    ap.add_argument("--only", choices=["all", "headline", "copy", "workloads", "config1", "config3", "batch", "serve",
+                                       "stream"],
                     default="all",
                     help="run one section (profiling runs); headline: only the 2^28-row group-by; copy: only the "
                          "copy reference")
@@ -365,6 +366,58 @@ def config3(ctx, tga, args):
             "checks_ok": bool(ok), "cpu": cpu}
 
 
+def host_streaming(ctx, tga, args, reps=3):
+    """The headline workload from pinned host memory: the operator streams
+    the columns through HBM tile by tile, copies overlapped with the
+    aggregation (PCIe-inclusive rows/s, not the headline).  Beside it the
+    copy of the same bytes alone (the PCIe floor) and the operator on the
+    device-resident columns (the compute floor)."""
+    from cpu_gpu_coprocess_groupby_agg_amd import lib
+    n = 1 << args.rows_log2
+    d = 1 << 20
+    keys = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=2028, salt=0x5EED)
+    vals = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & M64, b=1 << 31, seed=2029)
+    kh = ctx.pinned_empty(n, np.uint64)
+    vh = ctx.pinned_empty(n, np.int32)
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, kh.ctypes.data, keys.ptr, n * 8))
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, vh.ctypes.data, vals.ptr, n * 4))
+    aggs = ["count", ("sum", 0), ("min", 0), ("max", 0)]
+    al = [(tga._OPS[a] if isinstance(a, str) else tga._OPS[a[0]], 0 if isinstance(a, str) else a[1]) for a in aggs]
+    ag = (tga.Agg * 4)(*[tga.Agg(o, c) for o, c in al])
+    kc = tga.Column(kh.ctypes.data, tga.U64, 1)
+    vc = (tga.Column * 1)(tga.Column(vh.ctypes.data, tga.I32, 1))
+
+    def call():
+        r = ctypes.c_void_p()
+        ctx._check(lib().tga_groupby(ctx.h, ctypes.byref(kc), vc, 1, n, ag, 4, None, ctypes.byref(r)))
+        g = ctypes.c_int64()
+        ctx._check(lib().tga_result_num_groups(r.value, ctypes.byref(g)))
+        lib().tga_result_free(r.value)
+        return g.value
+
+    G = call()
+    plan = ctx.last_plan()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        G = call()
+    t = (time.perf_counter() - t0) / reps
+    assert G == d
+    # the copies alone, same bytes, pinned -> HBM
+    t0 = time.perf_counter()
+    ctx._check(lib().tga_memcpy_h2d(ctx.h, keys.ptr, kh.ctypes.data, n * 8))
+    ctx._check(lib().tga_memcpy_h2d(ctx.h, vals.ptr, vh.ctypes.data, n * 4))
+    t_copy = time.perf_counter() - t0
+    ms_dev, _, _ = time_steps(ctx, keys, [vals], aggs, 3, 1)
+    keys.free()
+    vals.free()
+    return {"workload": "the headline workload in pinned host memory (2^28 rows, 3.2 GB), no hint",
+            "ms": round(t * 1e3, 2), "rows_per_s_pcie_inclusive": round(n / t, 1),
+            "host_to_device_gbs": round(n * 12 / t / 1e9, 1),
+            "copy_alone_ms": round(t_copy * 1e3, 2), "device_resident_ms": round(ms_dev, 3),
+            "overlap_efficiency": round((t_copy + ms_dev * 1e-3) / t, 3),
+            "tiles": plan["streamed_tiles"], "pinned": plan["stream_pinned"], "groups": int(G)}
+
+
 def loguniform_int(rng, lo_log2, hi_log2, size):
     return np.floor(2.0 ** (lo_log2 + (hi_log2 - lo_log2) * rng.random(size))).astype(np.int64)
 
@@ -601,6 +654,9 @@ def main():
         args.no_cpu = True
         print(json.dumps({"config3": config3(ctx, tga, args)}))
         return
+    if args.only == "stream":
+        print(json.dumps({"host_streaming": host_streaming(ctx, tga, args)}))
+        return
     if args.only in ("batch", "serve"):
         b, pool = batched(ctx, tga, args)
         out = {"batched": b}
@@ -634,6 +690,9 @@ def main():
         log(f"config1: {extra['config1']}")
         extra["config3"] = config3(ctx, tga, args)
         log(f"config3: {extra['config3']}")
+    if not args.no_configs:
+        extra["host_streaming"] = host_streaming(ctx, tga, args)
+        log(f"host streaming: {extra['host_streaming']}")
     if not args.no_workloads:
         extra["workloads"] = workloads(ctx, tga, args)
     if not args.no_batch:

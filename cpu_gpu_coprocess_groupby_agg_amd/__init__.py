@@ -68,7 +68,8 @@ class Options(ctypes.Structure):
                 ("table_slots", ctypes.c_int32), ("tile_rows", ctypes.c_int32), ("workgroups", ctypes.c_int32),
                 ("block_threads", ctypes.c_int32), ("chunk_rows", ctypes.c_int64),
                 ("wave_fold", ctypes.c_int32), ("attempts", ctypes.c_int32), ("scatter_layout", ctypes.c_int32),
-                ("write_combine", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("write_combine", ctypes.c_int32), ("streamed_tiles", ctypes.c_int32),
+                ("stream_pinned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Query(ctypes.Structure):
@@ -114,6 +115,7 @@ def lib():
             "tga_scan_exclusive_u32": [vp, vp, vp, i64],
             "tga_generate": [vp, i32, vp, i64, u64, u64, ctypes.c_double, ctypes.c_double, u64, u64],
             "tga_device_alloc": [vp, u64, P(vp)],
+            "tga_host_alloc": [vp, u64, P(vp)],
             "tga_device_free": [vp, vp],
             "tga_memcpy_h2d": [vp, vp, vp, u64],
             "tga_memcpy_d2h": [vp, vp, vp, u64],
@@ -250,6 +252,19 @@ class Context:
     # -- memory ---------------------------------------------------------------
     def empty(self, n, dtype):
         return DeviceArray(self, n, dtype)
+
+    def pinned_empty(self, n, dtype):
+        """A numpy array in pinned host memory (tga_host_alloc), freed with
+        the array.  Host columns in pinned memory are streamed to the GPU with
+        the copies overlapping the aggregation."""
+        import weakref
+        dt = np.dtype(dtype)
+        p = ctypes.c_void_p()
+        self._check(lib().tga_host_alloc(self.h, max(1, int(n) * dt.itemsize), ctypes.byref(p)))
+        buf = (ctypes.c_char * max(1, int(n) * dt.itemsize)).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=dt, count=int(n))
+        weakref.finalize(buf, lib().tga_host_free, p.value)
+        return arr
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)

@@ -372,7 +372,7 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   return p;
 }
 
-static void plan_to_options(const Plan &p, tga_options &o) {
+void plan_to_options(const Plan &p, tga_options &o) {
   o = tga_options{};
   o.path = p.path;
   o.partition_bits = p.pbits;
@@ -1148,14 +1148,21 @@ tga_status stage(tga_context *ctx, const tga_column *key, const tga_column *valu
   return TGA_OK;
 }
 
-tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values, int64_t n_rows,
-                   const tga_agg *aggs, int32_t n_aggs, const tga_options *opt, Staged &sg, QuerySpec &q, Plan &plan) {
+tga_status validate(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values,
+                    int64_t n_rows, const tga_agg *aggs, int32_t n_aggs, QuerySpec &q) {
   if (n_rows < 0) return fail(ctx, TGA_ERR_INVALID_ARG, "negative row count");
   std::string e = build_spec(key, values, n_values, aggs, n_aggs, q);
   if (!e.empty()) return fail(ctx, TGA_ERR_INVALID_ARG, "%s", e.c_str());
   if (n_rows > 0 && !key->data) return fail(ctx, TGA_ERR_INVALID_ARG, "key data is null");
   for (int c = 0; c < n_values; ++c)
     if (n_rows > 0 && !values[c].data) return fail(ctx, TGA_ERR_INVALID_ARG, "value column %d is null", c);
+  return TGA_OK;
+}
+
+// Stages host columns whole, estimates the group count (hint, else a device
+// sample) and plans; q comes from validate().
+tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *values, int32_t n_values, int64_t n_rows,
+                   const tga_options *opt, Staged &sg, QuerySpec &q, Plan &plan) {
   tga_status st = stage(ctx, key, values, n_values, n_rows, sg);
   if (st) return st;
   for (int c = 0; c < n_values; ++c) q.schema.cols[c] = sg.vals[c].data;
@@ -1249,15 +1256,18 @@ TGA_API tga_status tga_groupby(tga_context *ctx, const tga_column *key, const tg
   Staged sg;
   QuerySpec q;
   Plan plan;
-  tga_status st = prepare(ctx, key, values, n_values, n_rows, aggs, n_aggs, opt, sg, q, plan);
+  tga_status st = validate(ctx, key, values, n_values, n_rows, aggs, n_aggs, q);
   if (st) return st;
+  const bool streamed = should_stream(key, values, n_values, (uint64_t)n_rows);
+  if (!streamed && (st = prepare(ctx, key, values, n_values, n_rows, opt, sg, q, plan))) return st;
   auto *r = new tga_result();
   r->ctx = ctx;
   r->key_dtype = q.key_dtype;
   r->naggs = n_aggs;
   for (int a = 0; a < n_aggs; ++a) r->dtype[a] = q.out_dtype[a];
   uint64_t G = 0;
-  st = run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, &r->out, nullptr, nullptr, 0, &G);
+  st = streamed ? run_streamed(ctx, q, key, values, n_values, (uint64_t)n_rows, opt, &r->out, nullptr, nullptr, 0, &G)
+                : run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, &r->out, nullptr, nullptr, 0, &G);
   if (st) { delete r; return st; }
   *out = r;
   return TGA_OK;
@@ -1274,10 +1284,16 @@ TGA_API tga_status tga_groupby_into(tga_context *ctx, const tga_column *key, con
   Staged sg;
   QuerySpec q;
   Plan plan;
-  tga_status st = prepare(ctx, key, values, n_values, n_rows, aggs, n_aggs, opt, sg, q, plan);
+  tga_status st = validate(ctx, key, values, n_values, n_rows, aggs, n_aggs, q);
   if (st) return st;
   uint64_t G = 0;
-  st = run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, nullptr, out_keys, out_aggs, (uint64_t)capacity, &G);
+  if (should_stream(key, values, n_values, (uint64_t)n_rows)) {
+    st = run_streamed(ctx, q, key, values, n_values, (uint64_t)n_rows, opt, nullptr, out_keys, out_aggs,
+                      (uint64_t)capacity, &G);
+  } else {
+    if ((st = prepare(ctx, key, values, n_values, n_rows, opt, sg, q, plan))) return st;
+    st = run_plain(ctx, q, sg.key.data, (uint64_t)n_rows, plan, nullptr, out_keys, out_aggs, (uint64_t)capacity, &G);
+  }
   *n_groups = (int64_t)G;
   return st;
 }
@@ -1360,6 +1376,16 @@ TGA_API tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int
   hipSetDevice(ctx->device);
   TGA_HIP(ctx, launch_gen(kind, d_out, (uint64_t)n, a, b, x, y, seed, salt, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_host_alloc(tga_context *ctx, uint64_t bytes, void **out) {
+  if (!ctx || !out) return TGA_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  hipSetDevice(ctx->device);
+  void *p = nullptr;
+  TGA_HIP(ctx, hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault));
+  *out = p;
   return TGA_OK;
 }
 

@@ -172,6 +172,9 @@ struct Plan {
 Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n_rows, uint64_t est_groups,
               const tga_options *opt);
 
+// The plan as the options tga_last_plan reports.
+void plan_to_options(const Plan &p, tga_options &o);
+
 // Cardinality estimate from a random key sample (GEE estimator,
 // Charikar et al. 2000, or the equal-frequency model), replaced by an exact
 // count over a 2^-10 hash slice of all keys when the sample leaves the
@@ -201,6 +204,20 @@ uint64_t combine_slice_estimate(uint64_t sample_est, uint64_t sample_d, uint64_t
 // An unreliable sample estimate over at least this many rows is replaced by
 // an exact count over a hash slice of the keys.
 constexpr uint64_t kSliceMinRows = 1ULL << 22;
+
+inline uint64_t next_pow2_u64(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Host-resident columns of at least kStreamMinRows rows are streamed tile by
+// tile through HBM (run_streamed) instead of staged whole.
+bool should_stream(const tga_column *key, const tga_column *values, int nv, uint64_t n);
+struct PlainOut;
+tga_status run_streamed(tga_context *ctx, const struct QuerySpec &q, const tga_column *key, const tga_column *values,
+                        int nv, uint64_t n, const tga_options *opt, PlainOut *out, void *ext_keys,
+                        void *const *ext_cols, uint64_t ext_cap, uint64_t *groups);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

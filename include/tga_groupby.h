@@ -88,7 +88,11 @@ typedef struct tga_options {
   int32_t write_combine;  /* bucket layout with one 4-byte value column: 0 or 1 = keep each partition's
                              unfinished cache line in LDS and write whole lines (when the LDS holds it),
                              2 = never */
-  int32_t reserved[4];
+  int32_t streamed_tiles; /* tga_last_plan only: host tiles the input was streamed in (host-resident
+                             columns of >= 2^22 rows), 0 = device-resident or staged whole */
+  int32_t stream_pinned;  /* tga_last_plan only: 1 = the streamed host columns were pinned, so the copies
+                             overlapped the aggregation */
+  int32_t reserved[2];
 } tga_options;
 
 typedef struct tga_context tga_context;
@@ -257,6 +261,9 @@ tga_status tga_generate(tga_context *ctx, int32_t kind, void *d_out, int64_t n, 
                         double y, uint64_t seed, uint64_t salt);
 
 tga_status tga_device_alloc(tga_context *ctx, uint64_t bytes, void **out);
+/* Pinned host memory (free with tga_host_free): host columns in it are
+ * streamed with the copies overlapping the aggregation. */
+tga_status tga_host_alloc(tga_context *ctx, uint64_t bytes, void **out);
 tga_status tga_device_free(tga_context *ctx, void *p);
 tga_status tga_memcpy_h2d(tga_context *ctx, void *dst, const void *src, uint64_t bytes);
 tga_status tga_memcpy_d2h(tga_context *ctx, void *dst, const void *src, uint64_t bytes);

@@ -174,6 +174,12 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+// streams run their work in submission order on the host: a wait is a no-op
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+enum hipMemoryType { hipMemoryTypeHost = 0, hipMemoryTypeDevice = 1 };
+struct hipPointerAttribute_t { hipMemoryType type; int device; void *devicePointer; void *hostPointer; };
+// every host pointer reads as pageable here
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t *, const void *) { return hipErrorInvalidValue; }
 
 namespace emu {
 // Runs `grid` workgroups of `block` work-items, each calling body(); on a

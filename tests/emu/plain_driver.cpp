@@ -447,6 +447,46 @@ int check_noalloc(tga_context *ctx) {
   return bad;
 }
 
+// Host-resident columns of more rows than a small context budget holds in
+// two tiles: the call streams them tile by tile into one global table; the
+// result must equal the oracle's (integer aggregates bit-exact, the exact
+// fp64 SUM within 1e-12).
+int check_streaming() {
+  int bad = 0;
+  tga_context *sctx = nullptr;
+  if (tga_context_create(0, 96ULL << 20, &sctx)) { std::printf("streaming: context failed\n"); return 1; }
+  const int64_t n = (1LL << 22) + 12345;
+  std::vector<uint64_t> k = keys_u64(n, 40000, 71);
+  for (int64_t i = 0; i < n; i += 5) k[i] = 99;  // a hot key
+  Case c{"stream_u64_i32_f64", 64, k, {col_i32(n, -(1LL << 31), (1LL << 31) - 1, 72), col_f64(n, 73)},
+         {TGA_I32, TGA_F64}, {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}, {TGA_SUM, 1}},
+         opts(TGA_PATH_AUTO, 2)};
+  bad += run_case(sctx, c);
+  tga_options plan;
+  tga_last_plan(sctx, &plan);
+  std::printf("stream_tiles: %d tiles, pinned %d %s\n", plan.streamed_tiles, plan.stream_pinned,
+              plan.streamed_tiles > 1 ? "ok" : "FAIL");
+  bad += plan.streamed_tiles > 1 ? 0 : 1;
+  // integer aggregates: per-tile partials (20000 groups in tiles of ~0.2M
+  // rows), merged as the partial buffer fills and at the end
+  Case m{"stream_merge_u64", 64, keys_u64(n, 20000, 76), {col_i32(n, -(1LL << 31), (1LL << 31) - 1, 77)},
+         {TGA_I32}, {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}}, opts(TGA_PATH_AUTO, 2)};
+  bad += run_case(sctx, m);
+  // tiles that hardly reduce (200000 groups per tile): one global table
+  Case m2{"stream_table_u64", 64, keys_u64(n, 200000, 78), {col_i32(n, -(1LL << 31), (1LL << 31) - 1, 79)},
+          {TGA_I32}, {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_MIN, 0}, {TGA_MAX, 0}}, opts(TGA_PATH_AUTO, 2)};
+  bad += run_case(sctx, m2);
+  tga_last_plan(sctx, &plan);
+  std::printf("stream_merge_tiles: %d tiles %s\n", plan.streamed_tiles, plan.streamed_tiles > 1 ? "ok" : "FAIL");
+  bad += plan.streamed_tiles > 1 ? 0 : 1;
+  std::vector<uint64_t> k32 = keys_u32(n, 3000, 74);
+  Case d{"stream_u32_no_hint", 32, k32, {col_i32(n, -5, 5, 75)}, {TGA_I32}, {{TGA_COUNT, 0}, {TGA_SUM, 0}},
+         opts(TGA_PATH_AUTO, 2)};
+  bad += run_case(sctx, d);
+  tga_context_destroy(sctx);
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -613,6 +653,10 @@ int main(int argc, char **argv) {
   }
   if (filter.empty() || std::string("noalloc").find(filter) != std::string::npos) {
     bad += check_noalloc(ctx);
+    ++ran;
+  }
+  if (filter.empty() || std::string("streaming").find(filter) != std::string::npos) {
+    bad += check_streaming();
     ++ran;
   }
   tga_context_destroy(ctx);
