@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-rate", type=float, default=20_000.0)
     ap.add_argument("--only", choices=["all", "headline", "copy", "workloads", "config1", "config3", "batch", "serve",
-                                       "stream"],
+                                       "stream", "coprocess"],
                     default="all",
                     help="run one section (profiling runs); headline: only the 2^28-row group-by; copy: only the "
                          "copy reference")
@@ -418,6 +418,68 @@ def host_streaming(ctx, tga, args, reps=3):
             "tiles": plan["streamed_tiles"], "pinned": plan["stream_pinned"], "groups": int(G)}
 
 
+def coprocess(ctx, tga, args, reps=2):
+    """CPU + GPU co-processing of the reference's two phases over pinned host
+    buffers (its driver's set-up: u32 keys of cardinality n/10, u32 values,
+    tile_len 1e7, P 300, Capacity 1e7, min/max load 6e6/8e6, 4 streams, 2^26
+    rows): GPU only, the planner's split, and a forced quarter of the rows on
+    16 host threads.  End-to-end time per mode (both phases), the split, and
+    the result (every (key, SUM) row) compared across the modes."""
+    from cpu_gpu_coprocess_groupby_agg_amd import lib
+    n = 1 << 26
+    card = n // 10
+    dk = ctx.generate(tga.GEN_REF_U32, n, a=card, b=0, x=0.9, seed=96, salt=0)
+    dv = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1 << 31, seed=97)
+    k0, v0 = ctx.pinned_empty(n, np.uint32), ctx.pinned_empty(n, np.uint32)
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, k0.ctypes.data, dk.ptr, n * 4))
+    ctx._check(lib().tga_memcpy_d2h(ctx.h, v0.ctypes.data, dv.ptr, n * 4))
+    dk.free()
+    dv.free()
+    k, v = ctx.pinned_empty(n, np.uint32), ctx.pinned_empty(n, np.uint32)
+    P, tile, cap = 300, 10_000_000, 10_000_000
+    out = {"workload": "reference driver shape: 2^26 rows, u32 keys of cardinality n/10 (uniform draws), u32 values, "
+                       "tile_len 1e7, P 300, Capacity 1e7, 4 streams, pinned host buffers", "rows": n,
+           "cardinality": card, "host_threads": cpu_threads()}
+    prints = {}
+    for name, share in (("gpu_only", 0.0), ("planner", -1.0), ("cpu_quarter", 0.25)):
+        ctx.set_coprocess(cpu_threads(), share)
+        times = []
+        for _ in range(reps):
+            k[:] = k0
+            v[:] = v0
+            t0 = time.perf_counter()
+            ps = ctypes.c_void_p()
+            ctx._check(lib().tga_compat_partition(ctx.h, k.ctypes.data, v.ctypes.data, n, tile, P, 4, ctypes.byref(ps)))
+            begin, num, nm = (ctypes.c_size_t * P)(), (ctypes.c_size_t * P)(), ctypes.c_size_t()
+            ok, ov = ctypes.c_void_p(), ctypes.c_void_p()
+            ctx._check(lib().tga_compat_intra_partition(ctx.h, ps, cap, cap * 3 // 5, cap * 4 // 5, 4, ctypes.byref(ok),
+                                                        ctypes.byref(ov), begin, num, ctypes.byref(nm)))
+            times.append(time.perf_counter() - t0)
+            lib().tga_partset_destroy(ps)
+            extent = max(begin[i] + num[i] for i in range(nm.value))
+            K = np.ctypeslib.as_array(ctypes.cast(ok, ctypes.POINTER(ctypes.c_uint32)), shape=(extent,))
+            V = np.ctypeslib.as_array(ctypes.cast(ov, ctypes.POINTER(ctypes.c_uint32)), shape=(extent,))
+            ko = np.concatenate([K[begin[i]:begin[i] + num[i]] for i in range(nm.value)])
+            vo = np.concatenate([V[begin[i]:begin[i] + num[i]] for i in range(nm.value)])
+            lib().tga_host_free(ok)
+            lib().tga_host_free(ov)
+        s1, s2 = ctx.last_coprocess(1), ctx.last_coprocess(2)
+        prints[name] = (ko.size, _fingerprint(ko, vo.astype(np.int64), vo.astype(np.int64)))
+        t = min(times)
+        out[name] = {"ms": round(t * 1e3, 1), "rows_per_s": round(n / t, 1), "groups": int(ko.size),
+                     "phase1_tiles_cpu_gpu": [s1["cpu_units"], s1["gpu_units"]],
+                     "phase2_partitions_cpu_gpu": [s2["cpu_units"], s2["gpu_units"]],
+                     "cpu_row_share": round((s1["cpu_rows"] + s2["cpu_rows"]) /
+                                            max(1, s1["cpu_rows"] + s1["gpu_rows"] + s2["cpu_rows"] + s2["gpu_rows"]), 4),
+                     "cpu_rows_per_s_per_thread": round(max(s1["cpu_rows_per_s"], s2["cpu_rows_per_s"]), 1)}
+        log(f"coprocess {name}: {out[name]}")
+    ctx.set_coprocess(0, 0.0)
+    out["groups_equal_cardinality"] = all(g == card for g, _ in prints.values())
+    out["identical_across_modes"] = len({fp for _, fp in prints.values()}) == 1
+    assert out["groups_equal_cardinality"] and out["identical_across_modes"], prints
+    return out
+
+
 def loguniform_int(rng, lo_log2, hi_log2, size):
     return np.floor(2.0 ** (lo_log2 + (hi_log2 - lo_log2) * rng.random(size))).astype(np.int64)
 
@@ -654,6 +716,9 @@ def main():
         args.no_cpu = True
         print(json.dumps({"config3": config3(ctx, tga, args)}))
         return
+    if args.only == "coprocess":
+        print(json.dumps({"coprocess": coprocess(ctx, tga, args)}))
+        return
     if args.only == "stream":
         print(json.dumps({"host_streaming": host_streaming(ctx, tga, args)}))
         return
@@ -693,6 +758,7 @@ def main():
     if not args.no_configs:
         extra["host_streaming"] = host_streaming(ctx, tga, args)
         log(f"host streaming: {extra['host_streaming']}")
+        extra["coprocess"] = coprocess(ctx, tga, args)
     if not args.no_workloads:
         extra["workloads"] = workloads(ctx, tga, args)
     if not args.no_batch:

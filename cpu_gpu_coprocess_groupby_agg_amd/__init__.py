@@ -86,6 +86,12 @@ class ServerStats(ctypes.Structure):
                 ("budget_bytes", ctypes.c_uint64), ("launches", ctypes.c_int64)]
 
 
+class CoprocessStats(ctypes.Structure):
+    _fields_ = [("gpu_units", ctypes.c_int64), ("cpu_units", ctypes.c_int64), ("gpu_rows", ctypes.c_int64),
+                ("cpu_rows", ctypes.c_int64), ("wall_s", ctypes.c_double), ("cpu_rows_per_s", ctypes.c_double),
+                ("gpu_rows_per_s", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -149,6 +155,8 @@ def lib():
             "tga_partset_destroy": [vp],
             "tga_compat_intra_partition": [vp, vp, sz, sz, sz, sz, P(vp), P(vp), P(sz), P(sz), P(sz)],
             "tga_host_free": [vp],
+            "tga_set_coprocess": [vp, i32, ctypes.c_double],
+            "tga_last_coprocess": [vp, i32, P(CoprocessStats)],
         }
         for name, argt in list(sigs.items()) + list(optional.items()):
             fn = getattr(L, name, None)
@@ -372,6 +380,16 @@ class Context:
             keys = keys[order]
             outs = [a[order] for a in outs]
         return {"keys": keys, "aggs": outs, "n_groups": G}
+
+    def set_coprocess(self, cpu_threads, cpu_share):
+        """CPU workers for the compat phases: cpu_share 0 = GPU only, (0, 1] =
+        that share of the rows, < 0 = the planner decides."""
+        self._check(lib().tga_set_coprocess(self.h, int(cpu_threads), float(cpu_share)))
+
+    def last_coprocess(self, phase):
+        st = CoprocessStats()
+        self._check(lib().tga_last_coprocess(self.h, int(phase), ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in CoprocessStats._fields_}
 
     def last_plan(self):
         o = Options()

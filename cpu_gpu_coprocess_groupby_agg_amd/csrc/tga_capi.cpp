@@ -1221,6 +1221,8 @@ TGA_API tga_status tga_context_create(int32_t device, uint64_t mem_budget_bytes,
 
 TGA_API tga_status tga_context_destroy(tga_context *ctx) {
   if (!ctx) return TGA_ERR_INVALID_ARG;
+  for (tga_context *l : ctx->lanes) tga_context_destroy(l);
+  ctx->lanes.clear();
   {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     hipSetDevice(ctx->device);

@@ -77,6 +77,12 @@ struct tga_context {
   tga::Profiler prof;
   std::string err;
   tga_options last_plan{};
+  // compat phases: extra GPU lanes (own stream, arena and status words, one
+  // per compat stream beyond the first) and the CPU co-processing settings
+  std::vector<tga_context *> lanes;
+  int32_t cpu_threads = 0;
+  double cpu_share = 0.0;
+  tga_coprocess_stats cop[2]{};
 };
 
 namespace tga {
@@ -218,6 +224,51 @@ struct PlainOut;
 tga_status run_streamed(tga_context *ctx, const struct QuerySpec &q, const tga_column *key, const tga_column *values,
                         int nv, uint64_t n, const tga_options *opt, PlainOut *out, void *ext_keys,
                         void *const *ext_cols, uint64_t ext_cap, uint64_t *groups);
+
+// Host aggregation of u32 keys with u32 wrap-around sums (every key value
+// legal): open addressing over a power-of-two table kept at most half full.
+class CpuAgg32 {
+ public:
+  void reset(uint64_t expect);
+  void add(const uint32_t *k, const uint32_t *v, uint64_t n);
+  uint64_t groups() const { return groups_; }
+  // the groups in table order; returns their count
+  uint64_t emit(uint32_t *ok, uint32_t *ov) const;
+
+ private:
+  void grow();
+  void add_one(uint32_t key, uint32_t val) {
+    uint64_t s = (uint64_t)(key_hash((uint64_t)key) & mask_);
+    for (;;) {
+      if (!used_[s]) {
+        used_[s] = 1;
+        keys_[s] = key;
+        sums_[s] = val;
+        ++groups_;
+        return;
+      }
+      if (keys_[s] == key) {
+        sums_[s] += val;
+        return;
+      }
+      s = (s + 1) & mask_;
+    }
+  }
+  std::vector<uint32_t> keys_, sums_;
+  std::vector<uint8_t> used_;
+  uint64_t mask_ = 0, groups_ = 0;
+};
+// The compat partition of a distinct row (upper hash bits modulo P), as the
+// GPU computes it.
+uint32_t compat_partition_host(uint32_t key, uint32_t P);
+// A phase-1 tile on the host: rows [0, m) of keys/vals pre-aggregated and
+// written back partition by partition; offs[p] = start of partition p,
+// offs[P] = the group count (returned).
+uint64_t cpu_compat_tile(uint32_t *keys, uint32_t *vals, uint64_t m, uint32_t P, uint32_t *offs);
+// A phase-2 merged partition on the host: its m gathered rows aggregated
+// with the reference's chunk rule, the groups written over them.
+tga_status cpu_compat_intra(uint32_t *k, uint32_t *v, uint64_t m, uint64_t min_load, uint64_t max_load,
+                            uint64_t *groups);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

@@ -237,6 +237,28 @@ tga_status tga_compat_intra_partition(tga_context *ctx, tga_partset *ps, size_t 
                                       size_t *par_kv_begin, size_t *par_result_kv_num, size_t *n_merged);
 tga_status tga_host_free(void *p);
 
+/* CPU + GPU co-processing of the compat phases (the reference's open item
+ * "cpu process partition only or groupby_agg_partition in phase_1",
+ * README.md:16): host worker threads take phase-1 tiles (pre-aggregation and
+ * hash partitioning, written back over the caller's buffers like a GPU tile)
+ * and phase-2 merged partitions (aggregated into the same gapped output) while
+ * the GPU takes the others; every unit is done by exactly one side, so the
+ * result is the same.  cpu_threads = 0 or cpu_share = 0: GPU only (the
+ * default).  cpu_share in (0, 1]: the CPU takes that share of the rows (tiles
+ * from the end of the input, partitions from the smallest).  cpu_share < 0:
+ * a planner gives the CPU a unit only when its measured throughput finishes it
+ * before the GPU would finish the rest. */
+tga_status tga_set_coprocess(tga_context *ctx, int32_t cpu_threads, double cpu_share);
+typedef struct tga_coprocess_stats {
+  int64_t gpu_units, cpu_units;  /* tiles (phase 1) or merged partitions (phase 2) */
+  int64_t gpu_rows, cpu_rows;    /* input rows of those units */
+  double wall_s;                 /* the call, end to end */
+  double cpu_rows_per_s;         /* measured per CPU thread (planner input) */
+  double gpu_rows_per_s;         /* measured over the GPU's units */
+} tga_coprocess_stats;
+/* Split of the last compat call of that phase (1 or 2) on this context. */
+tga_status tga_last_coprocess(tga_context *ctx, int32_t phase, tga_coprocess_stats *st);
+
 /* ---- primitives and utilities --------------------------------------------- */
 /* Exclusive prefix sum of n u32 on the device (own decoupled look-back). */
 tga_status tga_scan_exclusive_u32(tga_context *ctx, const uint32_t *d_in, uint32_t *d_out, int64_t n);

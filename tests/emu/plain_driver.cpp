@@ -487,6 +487,67 @@ int check_streaming() {
   return bad;
 }
 
+// The compat phases with three GPU lanes and host workers (a forced half of
+// the rows, then the planner): the gapped output, sorted by key, must equal
+// the oracle's (u32 keys, u32 wrap-around SUM).
+int check_compat_coprocess() {
+  int bad = 0;
+  tga_context *c = nullptr;
+  if (tga_context_create(0, 0, &c)) { std::printf("compat: context failed\n"); return 1; }
+  const int64_t n = 3000000;
+  const std::vector<uint64_t> k64 = keys_u32(n, 50000, 91);
+  const std::vector<unsigned char> vb = col_i32(n, -(1LL << 31), (1LL << 31) - 1, 92);
+  // oracle: keys widened, values as u32 bits (the low 32 bits of the sum)
+  std::vector<uint64_t> ok(n);
+  std::vector<int64_t> os(n);
+  {
+    std::vector<uint64_t> vw(n);
+    for (int64_t i = 0; i < n; ++i) { uint32_t x; std::memcpy(&x, &vb[i * 4], 4); vw[i] = x; }
+    const void *cols[1] = {vw.data()};
+    const int cdt[1] = {TGA_U64}, ops[1] = {TGA_SUM}, acs[1] = {0};
+    void *outs[1] = {os.data()};
+    const int64_t G = tga_o_groupby(k64.data(), 8, n, cols, cdt, 1, ops, acs, 1, ok.data(), outs, n);
+    ok.resize(G);
+    os.resize(G);
+  }
+  for (double share : {0.5, -1.0, 0.0}) {
+    std::vector<uint32_t> hk(n), hv(n);
+    for (int64_t i = 0; i < n; ++i) { hk[i] = (uint32_t)k64[i]; std::memcpy(&hv[i], &vb[i * 4], 4); }
+    tga_set_coprocess(c, 4, share);
+    tga_partset *ps = nullptr;
+    tga_status st = tga_compat_partition(c, hk.data(), hv.data(), n, 700000, 37, 3, &ps);
+    tga_coprocess_stats s1{}, s2{};
+    tga_last_coprocess(c, 1, &s1);
+    size_t P = 0;
+    if (!st) tga_partset_count(ps, &P);
+    std::vector<size_t> begin(P ? P : 1), num(P ? P : 1);
+    size_t nm = 0;
+    uint32_t *rk = nullptr, *rv = nullptr;
+    if (!st) st = tga_compat_intra_partition(c, ps, 200000, 60000, 80000, 2, &rk, &rv, begin.data(), num.data(), &nm);
+    tga_last_coprocess(c, 2, &s2);
+    int mism = st ? 1 : 0;
+    if (!st) {
+      std::vector<std::pair<uint32_t, uint32_t>> rows;
+      for (size_t i = 0; i < nm; ++i)
+        for (size_t j = 0; j < num[i]; ++j) rows.push_back({rk[begin[i] + j], rv[begin[i] + j]});
+      std::sort(rows.begin(), rows.end());
+      if (rows.size() != ok.size()) mism = 1;
+      for (size_t i = 0; i < rows.size() && !mism; ++i)
+        if (rows[i].first != (uint32_t)ok[i] || rows[i].second != (uint32_t)os[i]) mism = 1;
+      tga_host_free(rk);
+      tga_host_free(rv);
+    }
+    if (ps) tga_partset_destroy(ps);
+    const bool split_ok = share <= 0 || (s1.cpu_units > 0 && s2.cpu_units > 0 && s2.gpu_units > 0);
+    std::printf("compat_coprocess_share%.1f: status %d, phase 1 cpu %lld/gpu %lld tiles, phase 2 cpu %lld/gpu %lld "
+                "partitions %s\n", share, (int)st, (long long)s1.cpu_units, (long long)s1.gpu_units,
+                (long long)s2.cpu_units, (long long)s2.gpu_units, mism || !split_ok ? "FAIL" : "ok");
+    bad += mism || !split_ok;
+  }
+  tga_context_destroy(c);
+  return bad;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -653,6 +714,10 @@ int main(int argc, char **argv) {
   }
   if (filter.empty() || std::string("noalloc").find(filter) != std::string::npos) {
     bad += check_noalloc(ctx);
+    ++ran;
+  }
+  if (filter.empty() || std::string("compat_coprocess").find(filter) != std::string::npos) {
+    bad += check_compat_coprocess();
     ++ran;
   }
   if (filter.empty() || std::string("streaming").find(filter) != std::string::npos) {

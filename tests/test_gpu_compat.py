@@ -150,3 +150,47 @@ def test_cpp_compat_header_driver(ctx, tmp_path):
     ref = O.groupby(keys, [vals], [("sum", 0)])
     np.testing.assert_array_equal(k[o], ref["keys"])
     np.testing.assert_array_equal(v[o], (ref["aggs"][0] & 0xFFFFFFFF).astype(np.uint32))
+
+
+@pytest.mark.parametrize("share", [0.5, -1.0])
+@pytest.mark.parametrize("nstreams", [1, 4])
+def test_cpu_gpu_coprocessing_split_matches_oracle(ctx, share, nstreams):
+    """Host workers take a share of the phase-1 tiles (from the end of the
+    input) and of the phase-2 merged partitions (the smallest) beside
+    `nstreams` GPU lanes; the gapped output, sorted by key, equals the oracle
+    (u32 keys, u32 wrap-around SUM), and the forced share is honoured."""
+    n, d = 6_000_000, 150_000
+    keys = O.gen_keys_u32(n, d, 95, 3)
+    vals = O.gen_vals_i32(n, -(1 << 31), (1 << 31) - 1, 96).view(np.uint32)
+    ref = O.groupby(keys, [vals.astype(np.uint64)], [("sum", 0)])
+    ref_sum = (ref["aggs"][0].astype(np.uint64) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    ctx.set_coprocess(8, share)
+    try:
+        k, v = keys.copy(), vals.copy()
+        ps = ctypes.c_void_p()
+        ctx._check(lib().tga_compat_partition(ctx.h, k.ctypes.data, v.ctypes.data, n, 1_000_000, 61, nstreams,
+                                              ctypes.byref(ps)))
+        s1 = ctx.last_coprocess(1)
+        P = 61
+        begin, num, nm = (ctypes.c_size_t * P)(), (ctypes.c_size_t * P)(), ctypes.c_size_t()
+        ok, ov = ctypes.c_void_p(), ctypes.c_void_p()
+        ctx._check(lib().tga_compat_intra_partition(ctx.h, ps, 400_000, 240_000, 320_000, nstreams, ctypes.byref(ok),
+                                                    ctypes.byref(ov), begin, num, ctypes.byref(nm)))
+        lib().tga_partset_destroy(ps)
+        s2 = ctx.last_coprocess(2)
+        extent = max(begin[i] + num[i] for i in range(nm.value))
+        K = np.ctypeslib.as_array(ctypes.cast(ok, ctypes.POINTER(ctypes.c_uint32)), shape=(max(extent, 1),))
+        V = np.ctypeslib.as_array(ctypes.cast(ov, ctypes.POINTER(ctypes.c_uint32)), shape=(max(extent, 1),))
+        ko = np.concatenate([K[begin[i]:begin[i] + num[i]] for i in range(nm.value)])
+        vo = np.concatenate([V[begin[i]:begin[i] + num[i]] for i in range(nm.value)])
+        lib().tga_host_free(ok)
+        lib().tga_host_free(ov)
+    finally:
+        ctx.set_coprocess(0, 0.0)
+    o = np.argsort(ko, kind="stable")
+    np.testing.assert_array_equal(ko[o], ref["keys"])
+    np.testing.assert_array_equal(vo[o], ref_sum)
+    assert s1["gpu_units"] + s1["cpu_units"] == 6 and nm.value > 1
+    if share > 0:
+        assert s1["cpu_units"] == 3 and s2["cpu_units"] >= 1 and s2["gpu_units"] >= 1, (s1, s2)
+        assert s2["cpu_rows"] <= 0.5 * (s2["cpu_rows"] + s2["gpu_rows"]) + 1
