@@ -264,9 +264,15 @@ __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t 
   }
 }
 
+// Device clock for the optional per-item-type profile (100 MHz).
+__device__ __forceinline__ uint64_t batch_clock() { return __builtin_amdgcn_s_memrealtime(); }
+
+// prof (optional, 3 * kItemTypes words): per item type the clock ticks its
+// items spent working, the ticks spent waiting for the items they depend
+// on, and the item count.
 template <bool FX, int NC>
 __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
-                                                         uint32_t nitems, uint32_t *ticket) {
+                                                         uint32_t nitems, uint32_t *ticket, uint64_t *prof) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t s_item, s_sentinel, s_spilled;
   __shared__ uint32_t scratch[kBatchThreads / kWave + 1];
@@ -285,6 +291,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
     if (i >= nitems) break;
     const BItem it = items[i];
     const BQuery &Q = qs[it.q];
+    const uint64_t c0 = prof && threadIdx.x == 0 ? batch_clock() : 0;
     if (threadIdx.x == 0) {
       switch (it.type) {
         case IT_DIRECT: wait_done(Q, IT_INIT); break;
@@ -296,19 +303,27 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
       }
     }
     __syncthreads();
+    const uint64_t c1 = prof && threadIdx.x == 0 ? batch_clock() : 0;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
     else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
+    if (prof && threadIdx.x == 0) {
+      const uint64_t c2 = batch_clock();
+      atomicAdd((unsigned long long *)&prof[it.type], (unsigned long long)(c2 - c1));
+      atomicAdd((unsigned long long *)&prof[kItemTypes + it.type], (unsigned long long)(c1 - c0));
+      atomicAdd((unsigned long long *)&prof[2 * kItemTypes + it.type], 1ull);
+    }
     prev_q = it.q;
     prev_type = it.type;
   }
 }
 
 __global__ void k_batch_reset(uint32_t *done, uint64_t ndone, uint64_t *counts, uint64_t nq, uint32_t *ticket,
-                              uint32_t *flags) {
+                              uint32_t *flags, uint64_t *prof) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ndone; i += (uint64_t)gridDim.x * blockDim.x) done[i] = 0;
+  if (prof && blockIdx.x == 0 && threadIdx.x < 3 * kItemTypes) prof[threadIdx.x] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq * kFlagWords; i += (uint64_t)gridDim.x * blockDim.x)
     flags[i] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) counts[i] = 0;
@@ -438,6 +453,8 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
   if (!b.ddone.alloc(ctx, total_done * 4) || !b.dcounts.alloc(ctx, (uint64_t)n * 8) || !b.dticket.alloc(ctx, 64) ||
       !b.dflags.alloc(ctx, (uint64_t)n * kFlagWords * 4))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch counters");
+  // the per-item-type profile, when kernel timing is on for the context
+  if (ctx->prof.on && !b.dprof.alloc(ctx, 3 * kItemTypes * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch profile");
   if (hipHostMalloc((void **)&b.h_counts, std::max(1, n) * 8) != hipSuccess ||
       hipHostMalloc((void **)&b.h_flags, std::max(1, n) * kFlagWords * 4) != hipSuccess)
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch status");
@@ -553,15 +570,16 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
 tga_status batch_enqueue(BatchExec &b, hipStream_t st) {
   tga_context *ctx = b.ctx;
   const uint64_t n = (uint64_t)b.nq;
+  uint64_t *parg = b.dprof.p ? b.dprof.as<uint64_t>() : nullptr;
   hipLaunchKernelGGL(k_batch_reset, dim3(64), dim3(256), 0, st, b.ddone.as<uint32_t>(), n * kItemTypes,
-                     b.dcounts.as<uint64_t>(), n, b.dticket.as<uint32_t>(), b.dflags.as<uint32_t>());
+                     b.dcounts.as<uint64_t>(), n, b.dticket.as<uint32_t>(), b.dflags.as<uint32_t>(), parg);
   TGA_HIP(ctx, hipGetLastError());
   if (b.nitems) {
     const BQuery *qarg = b.dq.as<BQuery>();
     const BItem *iarg = b.ditems.as<BItem>();
     uint32_t narg = b.nitems;
     uint32_t *targ = b.dticket.as<uint32_t>();
-    void *args[] = {&qarg, &iarg, &narg, &targ};
+    void *args[] = {&qarg, &iarg, &narg, &targ, &parg};
     TGA_HIP(ctx, hipLaunchKernel(b.kernel, dim3(b.grid), dim3(kBatchThreads), args, b.lds, st));
   }
   if (n) {
@@ -712,6 +730,23 @@ TGA_API tga_status tga_batch_copy_to_host(const tga_batch *b, int32_t q, int32_t
   const void *src = index < 0 ? b->ex.okeys[q].p : b->ex.ocols[q][index].p;
   TGA_HIP(ctx, hipMemcpyAsync(dst, src, rows * dtype_bytes(dt), hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return TGA_OK;
+}
+
+TGA_API tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items) {
+  if (!b || !work_ms || !wait_ms || !items) return TGA_ERR_INVALID_ARG;
+  if (!b->ex.dprof.p) return TGA_ERR_UNSUPPORTED;
+  tga_context *ctx = b->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  uint64_t h[3 * kItemTypes];
+  TGA_HIP(ctx, hipMemcpyAsync(h, b->ex.dprof.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int t = 0; t < kItemTypes; ++t) {
+    work_ms[t] = (double)h[t] * 1e-5;  // 100 MHz ticks
+    wait_ms[t] = (double)h[kItemTypes + t] * 1e-5;
+    items[t] = (int64_t)h[2 * kItemTypes + t];
+  }
   return TGA_OK;
 }
 

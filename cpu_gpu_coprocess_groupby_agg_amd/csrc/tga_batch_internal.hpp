@@ -52,7 +52,7 @@ struct BatchExec {
   std::vector<const void *> keys;
   std::vector<uint64_t> est;
   std::vector<BQuery> hq;
-  DBuf dq, ditems, ddone, dcounts, dticket, dflags;
+  DBuf dq, ditems, ddone, dcounts, dticket, dflags, dprof;
   std::vector<DBuf> bufs;                          // tables and partition buffers
   std::vector<DBuf> okeys;                         // per query output keys
   std::vector<std::array<DBuf, kMaxAccs>> ocols;   // per query output columns

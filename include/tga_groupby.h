@@ -174,6 +174,12 @@ tga_status tga_batch_run(tga_batch *b);
 tga_status tga_batch_num_groups(const tga_batch *b, int32_t query, int64_t *n_groups);
 tga_status tga_batch_copy_to_host(const tga_batch *b, int32_t query, int32_t index, void *dst, int64_t capacity_rows);
 tga_status tga_batch_destroy(tga_batch *b);
+/* Per item type of the persistent work queue (INIT, COUNT, DIRECT, SCAN,
+ * SCATTER, PAGG, COMPACT: 7 entries each) over the last run: workgroup-ms
+ * spent working, workgroup-ms spent waiting for the items they depend on, and
+ * the item count.  Only for a persistent batch created while kernel timing
+ * (tga_profile_enable) was on; TGA_ERR_UNSUPPORTED otherwise. */
+tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items);
 
 /* ---- serving loop ----------------------------------------------------------- */
 typedef struct tga_server tga_server;

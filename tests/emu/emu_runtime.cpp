@@ -53,6 +53,11 @@ emu_ctx_switch:
 .size emu_ctx_switch,.-emu_ctx_switch
 )");
 
+uint64_t emu_realtime() {
+  return (uint64_t)(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                        std::chrono::steady_clock::now().time_since_epoch()).count() / 10);
+}
+
 // dynamic LDS of the kernels (their `extern __shared__` arrays)
 namespace tga {
 __attribute__((aligned(16))) thread_local char smem[163840];

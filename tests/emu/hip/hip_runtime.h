@@ -174,6 +174,9 @@ hipError_t hipEventRecord(hipEvent_t e, hipStream_t s = nullptr);
 hipError_t hipEventSynchronize(hipEvent_t e);
 hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+// the 100 MHz device clock
+uint64_t emu_realtime();
+#define __builtin_amdgcn_s_memrealtime() ::emu_realtime()
 // streams run their work in submission order on the host: a wait is a no-op
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 enum hipMemoryType { hipMemoryTypeHost = 0, hipMemoryTypeDevice = 1 };
