@@ -137,7 +137,8 @@ def lib():
             "tga_batch_num_groups": [vp, i32, P(i64)],
             "tga_batch_copy_to_host": [vp, i32, i32, vp, i64],
             "tga_batch_destroy": [vp],
-            "tga_batch_profile": [vp, P(ctypes.c_double), P(ctypes.c_double), P(i64)],
+            "tga_batch_profile": [vp, P(ctypes.c_double), P(ctypes.c_double), P(i64), P(ctypes.c_double),
+                                  P(ctypes.c_double)],
             "tga_server_create": [vp, u64, i32, P(vp)],
             "tga_server_submit": [vp, P(Query), P(i64)],
             "tga_server_poll": [vp, i64, P(vp)],

@@ -17,9 +17,12 @@
 // Workgroups pull items from an atomic ticket.  An item may depend only on
 // items earlier in the list (INIT < COUNT < DIRECT < SCAN < SCATTER < PAGG <
 // COMPACT), so every item it waits for is finished or held by a running
-// workgroup: the spin waits always end.  Producers publish with an
-// agent-scope release before bumping their stage counter; consumers acquire
-// after observing it.
+// workgroup: the spin waits always end.  An item's results are complete and
+// visible to every XCD before its stage counter is bumped: INIT writes back
+// its plain stores (an agent-scope release); the others hand on their results
+// through agent-coherent stores or device atomics and wait for them to
+// complete.  PAGG and COMPACT, which read with plain loads, acquire after
+// observing the counter.  Two workgroups share each CU.
 #include <algorithm>
 #include <numeric>
 
@@ -30,6 +33,48 @@ namespace tga {
 
 constexpr int kBatchThreads = 512;
 constexpr uint32_t kCompactSlice = 65536;
+constexpr int kPhases = 8;
+// partition tiles: kSubTiles sub-tiles of kScatterItems rows per thread, each
+// staged through LDS on its own
+constexpr int kScatterItems = 8;
+constexpr uint32_t kSubRows = kBatchThreads * kScatterItems;
+constexpr uint32_t kSubTiles = 4;
+constexpr uint32_t kTileRows = kSubRows * kSubTiles;
+
+// Device clock for the optional profile (100 MHz).
+__device__ __forceinline__ uint64_t batch_clock() { return __builtin_amdgcn_s_memrealtime(); }
+
+// Optional per-phase clock of an item (thread 0's view; every phase ends at
+// a barrier, so it is the workgroup's): mark(k) adds the ticks since the
+// previous mark to phase k.
+struct PhaseClock {
+  uint64_t *p;
+  uint64_t t;
+  __device__ explicit PhaseClock(uint64_t *p_) : p(p_), t(p_ && threadIdx.x == 0 ? batch_clock() : 0) {}
+  __device__ void mark(int k) {
+    if (!p) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t n = batch_clock();
+      atomicAdd((unsigned long long *)&p[k], (unsigned long long)(n - t));
+      t = n;
+    }
+  }
+};
+
+// Data one workgroup writes and another reads inside the launch (tile
+// counts, offsets, the partitioned rows) is stored and loaded coherently at
+// agent scope, access by access.  With every such store complete before its
+// item is counted done, a consumer needs no cache-wide acquire, and a
+// producer no cache-wide release (which writes back the whole L2 of its XCD).
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ void wait_done(const BQuery &Q, uint32_t type) {
   if (Q.need[type] == 0) return;
@@ -38,7 +83,7 @@ __device__ __forceinline__ void wait_done(const BQuery &Q, uint32_t type) {
 }
 
 template <typename K>
-__device__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
+__device__ __noinline__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
   const GTable &g = Q.g;
   const uint64_t n = g.S + 1;
   K *keys = (K *)g.keys;
@@ -60,105 +105,156 @@ __device__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
   }
 }
 
+// Partition histogram of one tile; each thread has 16 key loads in flight.
 template <typename K>
-__device__ void item_count(const BQuery &Q, uint32_t tile, uint32_t *hist) {
+__device__ __noinline__ void item_count(const BQuery &Q, uint32_t tile, uint32_t *hist) {
+  constexpr int R = 16;
   const uint32_t P = 1u << Q.pbits;
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
   __syncthreads();
   const K *keys = (const K *)Q.keys;
   const uint64_t b = (uint64_t)tile * Q.tile_rows, e = min(Q.n, b + Q.tile_rows);
-  for (uint64_t r = b + threadIdx.x; r < e; r += blockDim.x)
-    atomicAdd(&hist[hash_partition(key_hash((uint64_t)keys[r]), Q.pbits)], 1u);
+  for (uint64_t r0 = b; r0 < e; r0 += (uint64_t)R * blockDim.x) {
+    K kk[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint64_t r = r0 + (uint64_t)i * blockDim.x + threadIdx.x;
+      kk[i] = r < e ? keys[r] : KeyTraits<K>::kEmpty;
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (r0 + (uint64_t)i * blockDim.x + threadIdx.x < e)
+        atomicAdd(&hist[hash_partition(key_hash((uint64_t)kk[i]), Q.pbits)], 1u);
+  }
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) Q.cnt[(uint64_t)p * Q.ntiles + tile] = hist[p];
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) st_agent(&Q.cnt[(uint64_t)p * Q.ntiles + tile], hist[p]);
 }
 
 // One workgroup scans the whole [P][ntiles] histogram of a query.
-__device__ void item_scan(const BQuery &Q, uint32_t *buf, uint32_t *scratch) {
+__device__ __noinline__ void item_scan(const BQuery &Q, uint32_t *buf, uint32_t *scratch) {
   const uint32_t P = 1u << Q.pbits;
   const uint64_t cells = (uint64_t)P * Q.ntiles;
   const uint32_t chunk = blockDim.x * 8;
   uint32_t run = 0;
   for (uint64_t base = 0; base < cells; base += chunk) {
     const uint32_t m = (uint32_t)min<uint64_t>(chunk, cells - base);
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = Q.cnt[base + i];
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = ld_agent(&Q.cnt[base + i]);
     __syncthreads();
     const uint32_t tot = block_exclusive_scan_lds(buf, m, scratch);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
       const uint64_t idx = base + i;
-      Q.offs[idx] = run + buf[i];
-      if (idx % Q.ntiles == 0) Q.pstart[idx / Q.ntiles] = run + buf[i];  // first tile of partition p
+      st_agent(&Q.offs[idx], run + buf[i]);
+      if (idx % Q.ntiles == 0) st_agent(&Q.pstart[idx / Q.ntiles], (uint64_t)(run + buf[i]));  // first tile of p
     }
     run += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) Q.pstart[P] = Q.n;
+  if (threadIdx.x == 0) st_agent(&Q.pstart[P], Q.n);
 }
 
+// Rows [b, b + kSubRows) of a tile into registers (past e: empty / 0).
 template <typename K>
-__device__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_t *scratch) {
-  constexpr int ITEMS = 16;
-  const uint32_t P = 1u << Q.pbits;
-  const uint32_t TR = Q.tile_rows;  // == blockDim.x * ITEMS
-  uint32_t *hist = (uint32_t *)smem;
-  uint32_t *goff = hist + P;
-  K *skey = (K *)(goff + P);
-  uint32_t *sval = (uint32_t *)(skey + TR);  // one 32-bit value column
-  uint16_t *spid = (uint16_t *)(sval + TR);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-  __syncthreads();
-  const K *keys = (const K *)Q.keys;
-  const uint32_t *vals = (const uint32_t *)Q.s.cols[0];
-  const uint64_t b = (uint64_t)tile * TR;
-  K kk[ITEMS];
-  uint32_t vv[ITEMS], pp[ITEMS], rk[ITEMS];
+__device__ __forceinline__ void load_sub(const K *keys, const uint32_t *vals, uint64_t b, uint64_t e, K *kk,
+                                         uint32_t *vv) {
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
+  for (int i = 0; i < kScatterItems; ++i) {
     const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    const bool act = row < Q.n;
+    const bool act = row < e;
     kk[i] = act ? keys[row] : KeyTraits<K>::kEmpty;
     vv[i] = act ? vals[row] : 0u;
   }
+}
+
+// A tile of rows into the query's partitioned copy, one sub-tile at a time:
+// LDS counting sort by partition, then each partition's run written
+// contiguously at its cursor.  The next sub-tile's loads are issued before
+// the current one is sorted.  A destination outside [0, n) (offsets that do
+// not add up) is dropped and reported as an overflow, never written.
+template <typename K>
+__device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_t *scratch,
+                                          uint64_t *ph) {
+  PhaseClock pc(ph);
+  constexpr int IT = kScatterItems;
+  const uint32_t P = 1u << Q.pbits;
+  uint32_t *hist = (uint32_t *)smem;
+  uint32_t *goff = hist + P;  // output cursor per partition
+  K *skey = (K *)(goff + P);
+  uint32_t *sval = (uint32_t *)(skey + kSubRows);  // one 32-bit value column
+  uint16_t *spid = (uint16_t *)(sval + kSubRows);
+  const K *keys = (const K *)Q.keys;
+  const uint32_t *vals = (const uint32_t *)Q.s.cols[0];
+  K *pk = (K *)Q.pkeys;
+  uint32_t *pv = (uint32_t *)Q.pcols[0];
+  const uint64_t tb = (uint64_t)tile * Q.tile_rows, te = min(Q.n, tb + Q.tile_rows);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = ld_agent(&Q.offs[(uint64_t)p * Q.ntiles + tile]);
+  K kk[IT];
+  uint32_t vv[IT];
+  load_sub<K>(keys, vals, tb, te, kk, vv);
+  bool bad = false;
+  for (uint64_t sb = tb; sb < te; sb += kSubRows) {
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
+    __syncthreads();
+    K ck[IT];
+    uint32_t cv[IT], pp[IT], rk[IT];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    pp[i] = hash_partition(key_hash((uint64_t)kk[i]), Q.pbits);
-    rk[i] = row < Q.n ? atomicAdd(&hist[pp[i]], 1u) : 0u;
-  }
-  __syncthreads();
-  block_exclusive_scan_lds(hist, P, scratch);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = Q.offs[(uint64_t)p * Q.ntiles + tile];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    if (row < Q.n) {
-      const uint32_t lp = hist[pp[i]] + rk[i];
-      skey[lp] = kk[i];
-      sval[lp] = vv[i];
-      spid[lp] = (uint16_t)pp[i];
+    for (int i = 0; i < IT; ++i) {
+      const uint64_t row = sb + (uint64_t)i * blockDim.x + threadIdx.x;
+      ck[i] = kk[i];
+      cv[i] = vv[i];
+      pp[i] = hash_partition(key_hash((uint64_t)ck[i]), Q.pbits);
+      rk[i] = row < te ? atomicAdd(&hist[pp[i]], 1u) : 0u;
     }
+    if (sb + kSubRows < te) load_sub<K>(keys, vals, sb + kSubRows, te, kk, vv);
+    __syncthreads();
+    pc.mark(0);
+    block_exclusive_scan_lds(hist, P, scratch);
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const uint64_t row = sb + (uint64_t)i * blockDim.x + threadIdx.x;
+      if (row < te) {
+        const uint32_t lp = hist[pp[i]] + rk[i];
+        skey[lp] = ck[i];
+        sval[lp] = cv[i];
+        spid[lp] = (uint16_t)pp[i];
+      }
+    }
+    __syncthreads();
+    pc.mark(1);
+    const uint32_t rows = (uint32_t)min<uint64_t>(kSubRows, te - sb);
+    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+      const uint32_t p = spid[j];
+      const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
+      if (dst < Q.n) {
+        st_agent(&pk[dst], skey[j]);
+        st_agent(&pv[dst], sval[j]);
+      } else {
+        bad = true;
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] += (p + 1 < P ? hist[p + 1] : rows) - hist[p];
+    __syncthreads();
+    pc.mark(2);
   }
-  __syncthreads();
-  const uint32_t rows = (uint32_t)min<uint64_t>(TR, Q.n - b);
-  for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
-    const uint32_t p = spid[j];
-    const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
-    ((K *)Q.pkeys)[dst] = skey[j];
-    ((uint32_t *)Q.pcols[0])[dst] = sval[j];
-  }
+  if (bad) atomicOr(&Q.g.flags[FLAG_OVERFLOW], 1u);
 }
 
 // DIRECT items (a row range) flush their table into the query's global
-// table.  PAGG items (a whole partition: its keys occur in no other item)
-// emit their table straight into the query's output, unless a row had to
-// spill to the global table; then the table is flushed there as well.
+// table.  Items that see every row of their keys -- PAGG items (a whole
+// partition: its keys occur in no other item) and the DIRECT item of a query
+// small enough for one item -- emit their table straight into the query's
+// output, unless a row had to spill to the global table; then the table is
+// flushed there as well.
 template <typename K, bool FX, int NC>
-__device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e, char *smem,
-                         uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch, uint64_t *s_base) {
+__device__ __noinline__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e,
+                                      char *smem, uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch,
+                                      uint64_t *s_base, uint64_t *ph) {
+  PhaseClock pc(ph);
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
   if (threadIdx.x == 0) *spilled = FX ? 1u : 0u;  // fp64 limbs: always through the global table
   __syncthreads();
+  pc.mark(whole ? 6 : 3);
   if constexpr (FX) {
     const void *cols[NC];
 #pragma unroll
@@ -182,14 +278,17 @@ __device__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uin
     }
   }
   __syncthreads();
+  pc.mark(whole ? 7 : 4);
   if (whole && wave_uniform(*spilled) == 0) lds_emit<K>(t, s, Q.od, Q.count_out, Q.g.flags, scratch, s_base);
   else lds_flush<K>(t, s, Q.g);
+  pc.mark(5);
 }
 
 // Compacts table slots [a, b) of one query; output rows are reserved with an
 // atomic on the query's counter (order inside a query is unspecified).
 template <typename K>
-__device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *scratch, uint64_t *s_base) {
+__device__ __noinline__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *scratch,
+                                          uint64_t *s_base) {
   const GTable &g = Q.g;
   const uint64_t n = g.S + 1;
   const K *keys = (const K *)g.keys;
@@ -243,20 +342,25 @@ __device__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *
 
 template <typename K, bool FX, int NC>
 __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
-                         uint64_t *s_base, uint32_t *spilled) {
+                         uint64_t *s_base, uint32_t *spilled, uint64_t *ph) {
   switch (it.type) {
     case IT_INIT: item_init<K>(Q, it.a, it.b); break;
     case IT_COUNT: item_count<K>(Q, (uint32_t)it.a, (uint32_t *)smem); break;
-    case IT_DIRECT: item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel, false, spilled, scratch, s_base); break;
+    case IT_DIRECT:
+      item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel, it.a == 0 && it.b == Q.n, spilled, scratch,
+                          s_base, ph);
+      break;
     case IT_SCAN: item_scan(Q, (uint32_t *)smem, scratch); break;
-    case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch); break;
+    case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch, ph); break;
     case IT_PAGG: {
       Schema sp = Q.s;
 #pragma unroll
       for (int c = 0; c < kMaxCols; ++c)
         if (c < sp.ncols) sp.cols[c] = Q.pcols[c];
-      item_agg<K, FX, NC>(Q, Q.pkeys, sp, Q.pstart[it.a], Q.pstart[it.a + 1], smem, sentinel, true, spilled, scratch,
-                          s_base);
+      // a range outside [0, n) (offsets that do not add up) is cut to it
+      const uint64_t e = min(ld_agent(&Q.pstart[it.a + 1]), Q.n);
+      const uint64_t b = min(ld_agent(&Q.pstart[it.a]), e);
+      item_agg<K, FX, NC>(Q, Q.pkeys, sp, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
       break;
     }
     case IT_COMPACT: item_compact<K>(Q, it.a, it.b, scratch, s_base); break;
@@ -264,14 +368,17 @@ __device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t 
   }
 }
 
-// Device clock for the optional per-item-type profile (100 MHz).
-__device__ __forceinline__ uint64_t batch_clock() { return __builtin_amdgcn_s_memrealtime(); }
-
-// prof (optional, 3 * kItemTypes words): per item type the clock ticks its
-// items spent working, the ticks spent waiting for the items they depend
-// on, and the item count.
+// prof (optional, 4 * kItemTypes + kPhases words): per item type the clock
+// ticks its items spent working, the ticks spent waiting for the items they
+// depend on, the item count, and the ticks spent acquiring and publishing
+// around each item; then the ticks of the phases inside items
+// (scatter: rank, local sort, write-out; row-range aggregation: table
+// set-up, rows; both aggregations: flush or emission; partition
+// aggregation: table set-up, rows).
+// Two workgroups per CU (each with at most half of the LDS, at most 128
+// VGPRs): one's barriers and memory waits overlap the other's work.
 template <bool FX, int NC>
-__global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
+__global__ void __launch_bounds__(kBatchThreads, 2 * kBatchThreads / kWave / 4) k_batch(const BQuery *__restrict__ qs, const BItem *__restrict__ items,
                                                          uint32_t nitems, uint32_t *ticket, uint64_t *prof) {
   extern __shared__ __align__(16) char smem[];
   __shared__ uint32_t s_item, s_sentinel, s_spilled;
@@ -283,7 +390,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
   for (;;) {
     if (threadIdx.x == 0) {
       if (prev_q != 0xffffffffu)
-        __hip_atomic_fetch_add(&qs[prev_q].done[prev_type], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&qs[prev_q].done[prev_type], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_item = atomicAdd(ticket, 1u);
     }
     __syncthreads();
@@ -304,16 +411,36 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
     }
     __syncthreads();
     const uint64_t c1 = prof && threadIdx.x == 0 ? batch_clock() : 0;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
-    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // PAGG reads the partitioned rows and COMPACT the global table with plain
+    // loads: lines of them cached in this XCD's L2 before their writers
+    // finished are dropped first
+    if (it.type == IT_PAGG || it.type == IT_COMPACT) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint64_t c1f = prof && threadIdx.x == 0 ? batch_clock() : 0;
+    uint64_t *ph = prof ? prof + 4 * kItemTypes : nullptr;
+    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
+    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
+    // profiling: every wave's work ends before the release is timed
+    if (prof) __syncthreads();
+    const uint64_t c2w = prof && threadIdx.x == 0 ? batch_clock() : 0;
+    // Publication before the item is counted done (by a relaxed increment
+    // after the barrier).  INIT's plain stores are written back from L2.
+    // Every other item hands on its results through agent-coherent stores
+    // or device-scope atomics, which are visible to every XCD once complete:
+    // the wave waits for all of its memory operations.
+    if (it.type == IT_INIT) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    } else {
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
     __syncthreads();
     if (prof && threadIdx.x == 0) {
       const uint64_t c2 = batch_clock();
-      atomicAdd((unsigned long long *)&prof[it.type], (unsigned long long)(c2 - c1));
+      atomicAdd((unsigned long long *)&prof[it.type], (unsigned long long)(c2w - c1f));
       atomicAdd((unsigned long long *)&prof[kItemTypes + it.type], (unsigned long long)(c1 - c0));
       atomicAdd((unsigned long long *)&prof[2 * kItemTypes + it.type], 1ull);
+      atomicAdd((unsigned long long *)&prof[3 * kItemTypes + it.type], (unsigned long long)((c1f - c1) + (c2 - c2w)));
     }
     prev_q = it.q;
     prev_type = it.type;
@@ -323,7 +450,7 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch(const BQuery *__restric
 __global__ void k_batch_reset(uint32_t *done, uint64_t ndone, uint64_t *counts, uint64_t nq, uint32_t *ticket,
                               uint32_t *flags, uint64_t *prof) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < ndone; i += (uint64_t)gridDim.x * blockDim.x) done[i] = 0;
-  if (prof && blockIdx.x == 0 && threadIdx.x < 3 * kItemTypes) prof[threadIdx.x] = 0;
+  if (prof && blockIdx.x == 0 && threadIdx.x < 4 * kItemTypes + kPhases) prof[threadIdx.x] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq * kFlagWords; i += (uint64_t)gridDim.x * blockDim.x)
     flags[i] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) counts[i] = 0;
@@ -350,9 +477,19 @@ struct QShape {
   uint32_t tile_rows = 0, ntiles = 0;
 };
 
+// LDS per workgroup: two workgroups share a CU (less the kernel's static
+// LDS, a few hundred bytes).
+size_t batch_lds_budget(const tga_context *ctx) { return ctx->lds_max / 2 - 1024; }
+
+// LDS of a scatter item: histogram and cursors per partition, one sub-tile
+// of keys, values and partition ids.
+size_t scatter_lds(int pbits, int key_bytes) {
+  return ((size_t)8 << pbits) + (size_t)kSubRows * (key_bytes + 4 + 2);
+}
+
 QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est) {
   QShape z;
-  const size_t lds_max = ctx->lds_max - 2048;
+  const size_t lds_max = batch_lds_budget(ctx);
   const uint64_t G = std::max<uint64_t>(1, std::min<uint64_t>(est, std::max<uint64_t>(n, 1)));
   const uint32_t sb = lds_slot_bytes(q.schema, q.key_bytes);
   uint32_t tmax = 256;
@@ -365,8 +502,12 @@ QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t
   // only takes the rows of partitions that spill (a miss is rerun alone)
   if (z.part) z.S = std::max<uint64_t>(4096, z.S / 8);
   if (z.part) {
-    z.pbits = std::max(1, std::min(12, lg2(np2((2 * G + z.T - 1) / z.T))));
-    z.tile_rows = kBatchThreads * 16;
+    // partitions past the scatter's LDS are not made: larger partitions
+    // spill part of their rows to the global table instead
+    int pmax = 12;
+    while (pmax > 1 && scatter_lds(pmax, q.key_bytes) > lds_max) --pmax;
+    z.pbits = std::max(1, std::min(pmax, lg2(np2((2 * G + z.T - 1) / z.T))));
+    z.tile_rows = kTileRows;
     z.ntiles = (uint32_t)((n + z.tile_rows - 1) / z.tile_rows);
   }
   return z;
@@ -454,7 +595,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
       !b.dflags.alloc(ctx, (uint64_t)n * kFlagWords * 4))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch counters");
   // the per-item-type profile, when kernel timing is on for the context
-  if (ctx->prof.on && !b.dprof.alloc(ctx, 3 * kItemTypes * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch profile");
+  if (ctx->prof.on && !b.dprof.alloc(ctx, (4 * kItemTypes + kPhases) * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch profile");
   if (hipHostMalloc((void **)&b.h_counts, std::max(1, n) * 8) != hipSuccess ||
       hipHostMalloc((void **)&b.h_flags, std::max(1, n) * kFlagWords * 4) != hipSuccess)
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch status");
@@ -527,7 +668,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
       b.bufs.push_back(std::move(c3));
       b.bufs.push_back(std::move(c4));
       b.bufs.push_back(std::move(c5));
-      lds_need = std::max(lds_need, (size_t)8 * PP + (size_t)Q.tile_rows * (q.key_bytes + 4 + 2));
+      lds_need = std::max(lds_need, scatter_lds(z.pbits, q.key_bytes));
       lds_need = std::max<size_t>(lds_need, (size_t)4 * std::min<uint64_t>(cells, kBatchThreads * 8));
       for (uint32_t t = 0; t < Q.ntiles; ++t) lists[IT_COUNT].push_back({(uint32_t)i, IT_COUNT, t, 0});
       lists[IT_SCAN].push_back({(uint32_t)i, IT_SCAN, 0, 0});
@@ -539,7 +680,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
   }
   for (auto &l : lists)
     for (auto &it : l) b.hq[it.q].need[it.type] += 1;
-  if (lds_need > ctx->lds_max) return fail(ctx, TGA_ERR_UNSUPPORTED, "batch needs %zu B of LDS", lds_need);
+  if (lds_need > batch_lds_budget(ctx)) return fail(ctx, TGA_ERR_UNSUPPORTED, "batch needs %zu B of LDS", lds_need);
   // stage by stage; the largest queries first inside a stage
   std::vector<BItem> items;
   for (int t = 0; t < kItemTypes; ++t) {
@@ -733,20 +874,24 @@ TGA_API tga_status tga_batch_copy_to_host(const tga_batch *b, int32_t q, int32_t
   return TGA_OK;
 }
 
-TGA_API tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items) {
-  if (!b || !work_ms || !wait_ms || !items) return TGA_ERR_INVALID_ARG;
+TGA_API tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items,
+                                     double *fence_ms, double *phase_ms) {
+  if (!b || !work_ms || !wait_ms || !items || !fence_ms) return TGA_ERR_INVALID_ARG;
   if (!b->ex.dprof.p) return TGA_ERR_UNSUPPORTED;
   tga_context *ctx = b->ctx;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   hipSetDevice(ctx->device);
-  uint64_t h[3 * kItemTypes];
+  uint64_t h[4 * kItemTypes + kPhases];
   TGA_HIP(ctx, hipMemcpyAsync(h, b->ex.dprof.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int t = 0; t < kItemTypes; ++t) {
     work_ms[t] = (double)h[t] * 1e-5;  // 100 MHz ticks
     wait_ms[t] = (double)h[kItemTypes + t] * 1e-5;
     items[t] = (int64_t)h[2 * kItemTypes + t];
+    fence_ms[t] = (double)h[3 * kItemTypes + t] * 1e-5;
   }
+  if (phase_ms)
+    for (int k = 0; k < kPhases; ++k) phase_ms[k] = (double)h[4 * kItemTypes + k] * 1e-5;
   return TGA_OK;
 }
 

@@ -1103,35 +1103,70 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   if (giveup && lane == 0) atomicAdd(giveup + 1, consumed);
 }
 
+// Adds LDS slot i's accumulators into global slot gs.
 template <typename K>
-__device__ __forceinline__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g) {
-  const uint32_t S = t.L.slots, n = S + 1;
-  const K *keys = t.keys();
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    K key;
-    bool occ;
-    if (i < S) { key = keys[i]; occ = key != KeyTraits<K>::kEmpty; }
-    else { key = KeyTraits<K>::kEmpty; occ = *t.sentinel != 0; }
-    if (!occ) continue;
-    const int64_t gs = g_find_or_insert<K>(g, key, key_hash((uint64_t)key));
-    if (gs < 0) continue;
-    #pragma unroll
+__device__ __forceinline__ void flush_slot(const LdsTable<K> &t, const Schema &s, const GTable &g, uint32_t i,
+                                           int64_t gs) {
+  const uint32_t n = t.L.slots + 1;
+#pragma unroll
   for (int a = 0; a < kMaxAccs; ++a) {
     if (a >= s.nacc) break;
-      const int kind = s.acc[a].kind;
-      if (kind == ACC_SUM_F) {
-        int64_t lv[kLimbs];
-        const int64_t *p = t.template arr<int64_t>(a);
+    const int kind = s.acc[a].kind;
+    if (kind == ACC_SUM_F) {
+      int64_t lv[kLimbs];
+      const int64_t *p = t.template arr<int64_t>(a);
 #pragma unroll
-        for (int l = 0; l < kLimbs; ++l) lv[l] = p[(size_t)l * n + i];
-        g_apply_limbs(g, a, gs, lv);
-      } else if (acc_lds_bytes(kind) == 4) {
-        const uint32_t v = t.template arr<uint32_t>(a)[i];
-        g_apply(g, a, kind, gs, kind == ACC_COUNT ? (uint64_t)v : (uint64_t)(int64_t)(int32_t)v);
-      } else {
-        g_apply(g, a, kind, gs, t.template arr<uint64_t>(a)[i]);
-      }
+      for (int l = 0; l < kLimbs; ++l) lv[l] = p[(size_t)l * n + i];
+      g_apply_limbs(g, a, gs, lv);
+    } else if (acc_lds_bytes(kind) == 4) {
+      const uint32_t v = t.template arr<uint32_t>(a)[i];
+      g_apply(g, a, kind, gs, kind == ACC_COUNT ? (uint64_t)v : (uint64_t)(int64_t)(int32_t)v);
+    } else {
+      g_apply(g, a, kind, gs, t.template arr<uint64_t>(a)[i]);
     }
+  }
+}
+
+// Adds the LDS table into the global table.  Each lane takes R slots and
+// issues their first global probes together, then their compare-and-swaps
+// together: a group already in the global table costs one load, a new one
+// one compare-and-swap, with R of either in flight per lane.  A key whose
+// first slot holds another key takes the full probe.
+template <typename K>
+__device__ __forceinline__ void lds_flush(const LdsTable<K> &t, const Schema &s, const GTable &g) {
+  constexpr int R = 4;
+  const uint32_t S = t.L.slots;
+  const K *keys = t.keys();
+  K *gk = (K *)g.keys;
+  const K kE = KeyTraits<K>::kEmpty;
+  for (uint32_t i0 = threadIdx.x; i0 < S; i0 += R * blockDim.x) {
+    K key[R], cur[R];
+    uint64_t gs0[R];
+    bool occ[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t i = i0 + (uint32_t)r * blockDim.x;
+      key[r] = i < S ? keys[i] : kE;
+      occ[r] = key[r] != kE;
+      gs0[r] = occ[r] ? hash_global_slot(key_hash((uint64_t)key[r]), g.S) : 0;
+      cur[r] = occ[r] ? gk[gs0[r]] : kE;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (occ[r] && cur[r] == kE) cur[r] = atomicCAS(&gk[gs0[r]], kE, key[r]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!occ[r]) continue;
+      // cur: the key read, or what the compare-and-swap found (empty: ours)
+      const int64_t gs = (cur[r] == key[r] || cur[r] == kE) ? (int64_t)gs0[r]
+                                                            : g_find_or_insert<K>(g, key[r], key_hash((uint64_t)key[r]));
+      if (gs >= 0) flush_slot<K>(t, s, g, i0 + (uint32_t)r * blockDim.x, gs);
+    }
+  }
+  // the sentinel key's slot
+  if (threadIdx.x == 0 && *t.sentinel != 0) {
+    const int64_t gs = g_find_or_insert<K>(g, kE, key_hash((uint64_t)kE));
+    if (gs >= 0) flush_slot<K>(t, s, g, S, gs);
   }
 }
 

@@ -176,10 +176,15 @@ tga_status tga_batch_copy_to_host(const tga_batch *b, int32_t query, int32_t ind
 tga_status tga_batch_destroy(tga_batch *b);
 /* Per item type of the persistent work queue (INIT, COUNT, DIRECT, SCAN,
  * SCATTER, PAGG, COMPACT: 7 entries each) over the last run: workgroup-ms
- * spent working, workgroup-ms spent waiting for the items they depend on, and
- * the item count.  Only for a persistent batch created while kernel timing
- * (tga_profile_enable) was on; TGA_ERR_UNSUPPORTED otherwise. */
-tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items);
+ * spent working, workgroup-ms spent waiting for the items they depend on, the
+ * item count, and workgroup-ms spent in the memory fences that publish and
+ * acquire an item's results.  phase_ms (8 entries, may be NULL) splits the
+ * work: SCATTER rank, local sort, write-out; DIRECT table set-up, rows;
+ * DIRECT and PAGG flush or emission; PAGG table set-up, rows.  Only for a
+ * persistent batch created while kernel timing (tga_profile_enable) was on;
+ * TGA_ERR_UNSUPPORTED otherwise. */
+tga_status tga_batch_profile(const tga_batch *b, double *work_ms, double *wait_ms, int64_t *items,
+                             double *fence_ms, double *phase_ms);
 
 /* ---- serving loop ----------------------------------------------------------- */
 typedef struct tga_server tga_server;
