@@ -83,7 +83,8 @@ class ServerStats(ctypes.Structure):
                 ("inflight", ctypes.c_int64), ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
                 ("mean_us", ctypes.c_double), ("max_us", ctypes.c_double), ("elapsed_s", ctypes.c_double),
                 ("qps", ctypes.c_double), ("bytes_in_use", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
-                ("budget_bytes", ctypes.c_uint64), ("launches", ctypes.c_int64)]
+                ("budget_bytes", ctypes.c_uint64), ("launches", ctypes.c_int64), ("solo_runs", ctypes.c_int64),
+                ("build_s", ctypes.c_double), ("finish_s", ctypes.c_double)]
 
 
 class CoprocessStats(ctypes.Structure):

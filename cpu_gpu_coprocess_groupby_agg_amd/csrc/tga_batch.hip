@@ -515,8 +515,10 @@ QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t
 }  // namespace
 
 BatchExec::~BatchExec() {
-  if (h_counts) hipHostFree(h_counts);
-  if (h_flags) hipHostFree(h_flags);
+  if (ctx) {
+    ctx->pinned->put(h_counts, (uint64_t)std::max(1, nq) * 8);
+    ctx->pinned->put(h_flags, (uint64_t)std::max(1, nq) * kFlagWords * 4);
+  }
 }
 
 uint64_t batch_query_bytes(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est) {
@@ -596,9 +598,9 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch counters");
   // the per-item-type profile, when kernel timing is on for the context
   if (ctx->prof.on && !b.dprof.alloc(ctx, (4 * kItemTypes + kPhases) * 8)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch profile");
-  if (hipHostMalloc((void **)&b.h_counts, std::max(1, n) * 8) != hipSuccess ||
-      hipHostMalloc((void **)&b.h_flags, std::max(1, n) * kFlagWords * 4) != hipSuccess)
-    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch status");
+  b.h_counts = (uint64_t *)ctx->pinned->get((uint64_t)std::max(1, n) * 8);
+  b.h_flags = (uint32_t *)ctx->pinned->get((uint64_t)std::max(1, n) * kFlagWords * 4);
+  if (!b.h_counts || !b.h_flags) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch status");
   for (int32_t i = 0; i < n; ++i) {
     const QuerySpec &q = b.spec[i];
     BQuery &Q = b.hq[i];

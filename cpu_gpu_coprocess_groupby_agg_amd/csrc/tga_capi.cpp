@@ -50,13 +50,17 @@ uint64_t Arena::round(uint64_t bytes) {
 void *Arena::alloc(uint64_t bytes) {
   const uint64_t c = round(bytes);
   std::lock_guard<std::mutex> g(mu_);
-  auto it = free_.find(c);
-  if (it != free_.end() && !it->second.empty()) {
+  // a cached block of this class, or of a class at most a quarter larger
+  // (above 2 MiB classes are close together: a request that varies a little
+  // from call to call still finds the last call's block)
+  for (auto it = free_.lower_bound(c); it != free_.end() && it->first <= c + c / 4; ++it) {
+    if (it->second.empty()) continue;
+    const uint64_t cc = it->first;
     void *p = it->second.back();
     it->second.pop_back();
-    cached_ -= c;
-    in_use_ += c;
-    live_[p] = c;
+    cached_ -= cc;
+    in_use_ += cc;
+    live_[p] = cc;
     peak_ = std::max(peak_, in_use_);
     return p;
   }
@@ -98,6 +102,39 @@ void Arena::release_cached() {
   std::lock_guard<std::mutex> g(mu_);
   for (auto &kv : free_) { for (void *q : kv.second) hipFree(q); kv.second.clear(); }
   cached_ = 0;
+}
+
+uint64_t PinnedPool::cls(uint64_t bytes) {
+  uint64_t c = 4096;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+void *PinnedPool::get(uint64_t bytes) {
+  const uint64_t c = cls(bytes);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.find(c);
+    if (it != free_.end() && !it->second.empty()) {
+      void *p = it->second.back();
+      it->second.pop_back();
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipHostMalloc(&p, c) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  return p;
+}
+
+void PinnedPool::put(void *p, uint64_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu_);
+  free_[cls(bytes)].push_back(p);
+}
+
+PinnedPool::~PinnedPool() {
+  for (auto &kv : free_)
+    for (void *p : kv.second) hipHostFree(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,6 +1243,7 @@ TGA_API tga_status tga_context_create(int32_t device, uint64_t mem_budget_bytes,
     budget = (uint64_t)(fr * 0.9);
   }
   c->arena = new Arena(budget);
+  c->pinned = new PinnedPool();
   // the no-alloc entry point's scratch address on this device (once per
   // device; a failure only disables that entry point)
   (void)noalloc_scratch_init(device);
@@ -1228,8 +1266,10 @@ TGA_API tga_status tga_context_destroy(tga_context *ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->prof.destroy();
-    delete ctx->arena;
+    if (ctx->owns_arena) delete ctx->arena;
     ctx->arena = nullptr;
+    delete ctx->pinned;
+    ctx->pinned = nullptr;
     hipHostFree(ctx->status_host);
     ctx->status_host = nullptr;
     hipStreamDestroy(ctx->stream);

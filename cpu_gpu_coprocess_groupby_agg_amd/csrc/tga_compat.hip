@@ -83,6 +83,32 @@ struct tga_partset {
 
 #define TGA_API extern "C" __attribute__((visibility("default")))
 
+namespace tga {
+
+// The context and nstreams - 1 lane contexts on its device (created once,
+// kept by the context, at most 8 in all).  A lane has its own stream and
+// status words and allocates from the context's arena: one budget and one
+// block cache per device, so a block one lane frees serves the next
+// allocation of any other.
+std::vector<tga_context *> lanes_of(tga_context *ctx, size_t nstreams, tga_status *st) {
+  const size_t L = std::max<size_t>(1, std::min<size_t>(nstreams, 8));
+  *st = TGA_OK;
+  while (ctx->lanes.size() + 1 < L) {
+    tga_context *l = nullptr;
+    const tga_status e = tga_context_create(ctx->device, 1, &l);
+    if (e) { *st = fail(ctx, e, "lane context"); break; }
+    delete l->arena;
+    l->arena = ctx->arena;
+    l->owns_arena = false;
+    ctx->lanes.push_back(l);
+  }
+  std::vector<tga_context *> v{ctx};
+  for (size_t i = 0; i + 1 < L && i < ctx->lanes.size(); ++i) v.push_back(ctx->lanes[i]);
+  return v;
+}
+
+}  // namespace tga
+
 namespace {
 
 using clk = std::chrono::steady_clock;
@@ -97,22 +123,6 @@ QuerySpec compat_spec() {
   return q;
 }
 
-// The context and nstreams - 1 lane contexts on its device (created once,
-// kept by the context, at most 8 lanes); each lane context gets an eighth of
-// the context's budget.
-std::vector<tga_context *> lanes_of(tga_context *ctx, size_t nstreams, tga_status *st) {
-  const size_t L = std::max<size_t>(1, std::min<size_t>(nstreams, 8));
-  *st = TGA_OK;
-  while (ctx->lanes.size() + 1 < L) {
-    tga_context *l = nullptr;
-    const tga_status e = tga_context_create(ctx->device, ctx->arena->budget() / 8, &l);
-    if (e) { *st = fail(ctx, e, "compat lane context"); break; }
-    ctx->lanes.push_back(l);
-  }
-  std::vector<tga_context *> v{ctx};
-  for (size_t i = 0; i + 1 < L && i < ctx->lanes.size(); ++i) v.push_back(ctx->lanes[i]);
-  return v;
-}
 
 // Units [0, n) claimed from both ends: GPU lanes from the front, host
 // workers from the back.  The host takes a unit only while its planned share

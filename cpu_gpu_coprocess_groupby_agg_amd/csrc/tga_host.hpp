@@ -41,6 +41,21 @@ class Arena {
   std::map<void *, uint64_t> live_;
 };
 
+// Pinned host blocks, reused.  hipHostFree waits for the whole device, so
+// the blocks of finished batches are kept for the next ones and freed with
+// the context.  Power-of-two size classes from 4 KiB.
+class PinnedPool {
+ public:
+  ~PinnedPool();
+  void *get(uint64_t bytes);  // nullptr when the host is out of pinned memory
+  void put(void *p, uint64_t bytes);
+
+ private:
+  static uint64_t cls(uint64_t bytes);
+  std::mutex mu_;
+  std::map<uint64_t, std::vector<void *>> free_;
+};
+
 // ---------------------------------------------------------------------------
 // per-kernel HIP-event timing on the context stream
 // ---------------------------------------------------------------------------
@@ -73,6 +88,8 @@ struct tga_context {
   size_t lds_max = 65536;  // dynamic LDS bytes one workgroup may use
   std::recursive_mutex mu;
   tga::Arena *arena = nullptr;
+  bool owns_arena = true;  // false: a lane context using its parent's arena
+  tga::PinnedPool *pinned = nullptr;
   uint32_t *status_host = nullptr;  // pinned, kStatusWords: a call's status words come back in one copy
   tga::Profiler prof;
   std::string err;
@@ -269,6 +286,10 @@ uint64_t cpu_compat_tile(uint32_t *keys, uint32_t *vals, uint64_t m, uint32_t P,
 // with the reference's chunk rule, the groups written over them.
 tga_status cpu_compat_intra(uint32_t *k, uint32_t *v, uint64_t m, uint64_t min_load, uint64_t max_load,
                             uint64_t *groups);
+// The context and nstreams - 1 lane contexts on its device (created once,
+// kept by the context, at most 8 in all; they allocate from the context's
+// arena).  Call with ctx->mu held.
+std::vector<tga_context *> lanes_of(tga_context *ctx, size_t nstreams, tga_status *st);
 
 // Output of one plain group-by execution (arena buffers owned by caller).
 struct PlainOut {

@@ -2,12 +2,16 @@
 //
 // Requests are queued by tga_server_submit.  One scheduler thread
 //   * admits queued requests, oldest first, while their device footprint fits
-//     the memory budget and a stream slot is free,
-//   * turns each admitted group into one batched persistent launch (the work
-//     queue of tga_batch.hip) on its own stream, so batches overlap on the
-//     device while new requests keep arriving; a request of >= 2^24 rows
-//     runs alone through the plain operator instead,
-//   * polls the streams' events, finishes completed batches (overflowed
+//     the memory budget,
+//   * hands each admitted request of >= 2^24 rows to the solo lanes: worker
+//     threads, each with its own lane context (stream and arena), that run
+//     such a request alone through the plain operator (device-wide scans,
+//     direct emission) while the scheduler goes on,
+//   * turns each admitted group of smaller requests into one batched
+//     persistent launch (the work queue of tga_batch.hip) on its own stream,
+//     so flights overlap on the device with each other and with the solo
+//     lanes while new requests keep arriving,
+//   * polls the flights' events, finishes completed flights (overflowed
 //     queries rerun alone) and hands each request its own result,
 // and records latency = completion - submission for p50/p99.
 //
@@ -65,9 +69,12 @@ double pct(std::vector<double> v, double p) {
 // rows per flight before a new flight is started
 constexpr uint64_t kFlightRows = 1ULL << 26;
 // A request this large fills the GPU by itself: it runs alone through the
-// plain operator (device-wide scans, direct emission), while the flights of
-// small requests already in the streams keep running beside it.
+// plain operator (device-wide scans, direct emission) on a solo lane, while
+// the flights of small requests keep running beside it.
 constexpr uint64_t kSoloRows = 1ULL << 24;
+// Solo lanes: while one waits for its request's group count, the next
+// request's kernels run on another.
+constexpr int kSoloLanes = 2;
 
 struct tga_server {
   tga_context *ctx = nullptr;
@@ -84,31 +91,61 @@ struct tga_server {
   std::vector<bool> slot_busy;
   std::vector<double> lat_us;
   uint64_t reserved = 0, peak = 0;
-  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0;
+  int64_t next_ticket = 1, submitted = 0, completed = 0, rejected = 0, launches = 0, solo_runs = 0;
+  double build_s = 0, finish_s = 0;  // scheduler thread: building and finishing flights
   clk::time_point t0;
   bool started = false, stop = false;
   std::thread worker;
+  // solo lanes: admitted large requests wait in `solo`; lane i runs them on
+  // its own context
+  std::deque<Req> solo;
+  std::condition_variable cv_solo;
+  std::vector<tga_context *> solo_ctx;
+  std::vector<std::thread> solo_workers;
 
   void loop();
   bool finish_ready();
   bool launch_one();
-  bool run_solo();
+  bool admit_solo();
+  void solo_loop(tga_context *lc);
+  void run_solo(tga_context *lc, Req &r);
 };
 
-bool tga_server::run_solo() {
-  Req r;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    if (pending.empty()) return false;
-    const Req &f = pending.front();
-    if ((uint64_t)f.q.n_rows < kSoloRows || reserved + f.bytes > budget) return false;
-    r = std::move(pending.front());
-    pending.pop_front();
-    reserved += r.bytes;
-    peak = std::max(peak, reserved);
+// Moves the oldest request to the solo queue when it is a large one and its
+// footprint fits the budget (oldest first: a large request that does not fit
+// yet holds back the requests behind it).
+bool tga_server::admit_solo() {
+  std::lock_guard<std::mutex> g(mu);
+  if (pending.empty()) return false;
+  const Req &f = pending.front();
+  if ((uint64_t)f.q.n_rows < kSoloRows || reserved + f.bytes > budget) return false;
+  reserved += f.bytes;
+  peak = std::max(peak, reserved);
+  solo.push_back(std::move(pending.front()));
+  pending.pop_front();
+  cv_solo.notify_one();
+  return true;
+}
+
+void tga_server::solo_loop(tga_context *lc) {
+  hipSetDevice(lc->device);
+  for (;;) {
+    Req r;
+    {
+      std::unique_lock<std::mutex> g(mu);
+      cv_solo.wait(g, [&] { return stop || !solo.empty(); });
+      if (solo.empty()) return;  // stop, and nothing left
+      r = std::move(solo.front());
+      solo.pop_front();
+    }
+    run_solo(lc, r);
   }
+}
+
+void tga_server::run_solo(tga_context *lc, Req &r) {
   Done d{nullptr, TGA_OK};
   {
+    tga_context *ctx = lc;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     hipSetDevice(ctx->device);
     QuerySpec spec;
@@ -138,9 +175,9 @@ bool tga_server::run_solo() {
   lat_us.push_back(std::chrono::duration<double, std::micro>(now - r.t_submit).count());
   ++completed;
   ++launches;
+  ++solo_runs;
   reserved -= r.bytes;
   cv_done.notify_all();
-  return true;
 }
 
 bool tga_server::finish_ready() {
@@ -149,6 +186,7 @@ bool tga_server::finish_ready() {
     Flight &f = flights[i];
     if (hipEventQuery(f.ev) != hipSuccess) { ++i; continue; }
     any = true;
+    const auto tf = clk::now();
     std::vector<uint64_t> groups;
     tga_status st;
     {
@@ -185,6 +223,7 @@ bool tga_server::finish_ready() {
       f.ex.reset();  // returns tables and buffers to the arena
     }
     flights.erase(flights.begin() + (long)i);
+    finish_s += std::chrono::duration<double>(clk::now() - tf).count();
     cv_done.notify_all();
   }
   return any;
@@ -208,7 +247,7 @@ bool tga_server::launch_one() {
     while (!pending.empty() && (int)take.size() < max_batch) {
       const Req &r = pending.front();
       if (reserved + bytes + r.bytes > budget) break;  // oldest first: wait for memory
-      if ((uint64_t)r.q.n_rows >= kSoloRows) break;      // runs alone (run_solo)
+      if ((uint64_t)r.q.n_rows >= kSoloRows) break;      // a solo lane's (admit_solo)
       if (!take.empty() && rows + (uint64_t)r.q.n_rows > kFlightRows) break;
       bytes += r.bytes;
       rows += (uint64_t)r.q.n_rows;
@@ -229,6 +268,7 @@ bool tga_server::launch_one() {
     // row its own group when there is no hint, only bounds the admission
     qs[j].est_groups = take[j].q.est_groups;
   }
+  const auto tb = clk::now();
   Flight f;
   f.ex = std::make_unique<BatchExec>();
   f.slot = slot;
@@ -246,6 +286,7 @@ bool tga_server::launch_one() {
   }
   std::lock_guard<std::mutex> g(mu);
   ++launches;
+  build_s += std::chrono::duration<double>(clk::now() - tb).count();
   if (st) {
     for (auto &r : take) { done[r.ticket] = Done{nullptr, st}; ++completed; }
     reserved -= bytes;
@@ -263,7 +304,7 @@ void tga_server::loop() {
   for (;;) {
     bool did = false;
     if (!flights.empty()) did |= finish_ready();
-    did |= run_solo();
+    while (admit_solo()) did = true;
     did |= launch_one();
     std::unique_lock<std::mutex> g(mu);
     if (stop && pending.empty() && flights.empty()) break;
@@ -300,7 +341,21 @@ TGA_API tga_status tga_server_create(tga_context *ctx, uint64_t budget_bytes, in
     }
   }
   s->slot_busy.assign(s->nslots, false);
+  {
+    // the solo lanes' contexts belong to ctx (its lane list): results of
+    // solo requests hold their memory and may outlive the server
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    tga_status lst = TGA_OK;
+    const std::vector<tga_context *> lanes = lanes_of(ctx, 1 + kSoloLanes, &lst);
+    if (lst) {
+      for (auto x : s->streams) hipStreamDestroy(x);
+      delete s;
+      return lst;
+    }
+    s->solo_ctx.assign(lanes.begin() + 1, lanes.end());
+  }
   s->t0 = clk::now();
+  for (tga_context *lc : s->solo_ctx) s->solo_workers.emplace_back([s, lc] { s->solo_loop(lc); });
   s->worker = std::thread([s] { s->loop(); });
   *out = s;
   return TGA_OK;
@@ -385,6 +440,9 @@ TGA_API tga_status tga_server_stats_get(tga_server *s, tga_server_stats *st) {
   st->peak_bytes = s->peak;
   st->budget_bytes = s->budget;
   st->launches = s->launches;
+  st->solo_runs = s->solo_runs;
+  st->build_s = s->build_s;
+  st->finish_s = s->finish_s;
   return TGA_OK;
 }
 
@@ -396,6 +454,11 @@ TGA_API tga_status tga_server_destroy(tga_server *s) {
     s->cv_work.notify_all();
   }
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->cv_solo.notify_all();
+  }
+  for (auto &w : s->solo_workers) w.join();
   for (auto &kv : s->done)
     if (kv.second.r) tga_result_free(kv.second.r);
   {

@@ -194,6 +194,8 @@ typedef struct tga_server_stats {
   double elapsed_s, qps;
   uint64_t bytes_in_use, peak_bytes, budget_bytes;
   int64_t launches;
+  int64_t solo_runs;       /* requests run alone on a solo lane */
+  double build_s, finish_s; /* scheduler thread: seconds building and finishing flights */
 } tga_server_stats;
 
 tga_status tga_server_create(tga_context *ctx, uint64_t budget_bytes, int32_t max_batch, tga_server **out);
