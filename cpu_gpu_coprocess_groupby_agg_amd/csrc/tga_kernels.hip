@@ -36,10 +36,32 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Plain 16-byte streaming copy (bandwidth reference for the counters).
-__global__ void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+// 16-byte streaming copy, the bandwidth reference for the roofline and the
+// counters: each lane has four 16-byte loads in flight before it stores, and
+// loads and stores are non-temporal (the data is touched once).  A sweep of
+// this loop over unroll (1, 4, 8), workgroup size (256..1024), workgroups per
+// CU (2..32 wave-equivalents) and cache hints put one 1024-thread workgroup
+// per CU with both hints at the top (about 5.7 TB/s for 2 x 2 GiB, against
+// 4.5-4.9 TB/s for the plain one-load-per-iteration loop).
+__global__ void __launch_bounds__(1024) k_copy16(const uint4 *__restrict__ a0, uint4 *__restrict__ b0, uint64_t n) {
+  typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+  constexpr int U = 4;
+  const v4u *a = (const v4u *)a0;
+  v4u *b = (v4u *)b0;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x * U;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x; base < n; base += step) {
+    v4u v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + (uint64_t)u * blockDim.x;
+      if (i < n) v[u] = __builtin_nontemporal_load(&a[i]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = base + (uint64_t)u * blockDim.x;
+      if (i < n) __builtin_nontemporal_store(v[u], &b[i]);
+    }
+  }
 }
 
 template <typename K>
@@ -1618,7 +1640,10 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) 
 
 hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t st) {
   if (n16 == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy16, dim3(grid_for(n16, 256, 8192)), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n16);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipLaunchKernelGGL(k_copy16, dim3(grid_for(n16, 4096, (uint64_t)cus)), dim3(1024), 0, st, (const uint4 *)src,
+                     (uint4 *)dst, n16);
   return hipGetLastError();
 }
 

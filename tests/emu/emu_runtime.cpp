@@ -477,7 +477,11 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
   p->maxThreadsPerBlock = 1024;
   return hipSuccess;
 }
-hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 163840; return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
+  const char *e = std::getenv("EMU_CUS");
+  *v = a == hipDeviceAttributeMultiprocessorCount ? (e ? std::max(1, std::atoi(e)) : 8) : 163840;
+  return hipSuccess;
+}
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
