@@ -69,7 +69,7 @@ class Options(ctypes.Structure):
                 ("block_threads", ctypes.c_int32), ("chunk_rows", ctypes.c_int64),
                 ("wave_fold", ctypes.c_int32), ("attempts", ctypes.c_int32), ("scatter_layout", ctypes.c_int32),
                 ("write_combine", ctypes.c_int32), ("streamed_tiles", ctypes.c_int32),
-                ("stream_pinned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("stream_pinned", ctypes.c_int32), ("record_words", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Query(ctypes.Structure):

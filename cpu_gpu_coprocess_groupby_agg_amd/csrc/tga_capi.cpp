@@ -299,6 +299,11 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   uint32_t tmax = 64;
   while ((uint64_t)(tmax * 2 + 2) * sb + 256 <= lds_max) tmax *= 2;
 
+  // Records of the partitioned copy: (key, 4-byte value) for one 4-byte
+  // column; otherwise the key and up to kMaxRecValueDwords value words,
+  // written by the write-combining scatter only (one or two passes) and not
+  // with wave folding.
+  const int vd = copy_as_records(q.schema) ? 1 : o.wave_fold == 1 ? 0 : record_value_dwords(q.schema);
   int path = o.path;
   if (path == TGA_PATH_AUTO) path = (est * 2 <= tmax) ? TGA_PATH_DIRECT : TGA_PATH_PARTITIONED;
   p.path = path;
@@ -330,7 +335,7 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
     tfull &= ~63u;
     uint32_t T = o.table_slots > 0 ? (uint32_t)o.table_slots : std::max<uint32_t>(tfull, 256);
     p.table_slots = T;
-    const bool recs = o.scatter_layout != 1 && o.write_combine != 2 && copy_as_records(q.schema);
+    const bool recs = o.scatter_layout != 1 && o.write_combine != 2 && vd > 0;
     int pbits = o.partition_bits;
     if (pbits <= 0) {
       const uint64_t P = next_pow2((est * 2 + (uint64_t)T - 1) / (uint64_t)T);
@@ -343,13 +348,15 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
       // a quarter is not reliably faster than a half.
       if (recs) {
         const int b4 = std::max(1, ilog2(next_pow2((est * 4 + (uint64_t)T - 1) / (uint64_t)T)));
-        if (b4 > pbits && scatter_wc_tile(q.key_bytes, b4, lds_max)) pbits = b4;
+        if (b4 > pbits && scatter_wc_tile(q.key_bytes, vd, b4, lds_max)) pbits = b4;
       }
     }
     // 2^12 partitions at most, 2^13 where two passes of records reach them
     // (2^28 rows over 26.8M groups: 2^12 partitions fill their tables to
-    // 65%, and the aggregation takes 2.7 ms against 1.2 at 2^13)
-    p.max_pbits = recs && scatter_wc_tile(q.key_bytes, 13 - kTwoPassSubBits, lds_max) ? 13 : 12;
+    // 65%, and the aggregation takes 2.7 ms against 1.2 at 2^13), 2^14 for
+    // wider records (larger slots: smaller tables)
+    if (vd == 1) p.max_pbits = recs && scatter_wc_tile(q.key_bytes, 1, 13 - kTwoPassSubBits, lds_max) ? 13 : 12;
+    else p.max_pbits = recs && scatter_wc_tile(q.key_bytes, vd, 14 - kTwoPassMaxSubBits, lds_max) ? 14 : 12;
     pbits = std::min(pbits, p.max_pbits);
     p.pbits = pbits;
     uint32_t tile = o.tile_rows > 0 ? (uint32_t)o.tile_rows : 8192;
@@ -382,13 +389,12 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
   // workgroup keeps every partition's unfinished cache line in LDS, so the
   // scatter writes whole lines only.  Its carry buffer (128 B per partition)
   // takes LDS from the tile, which the tuner shrinks to 6144 or 4096 rows.
-  if (p.path == TGA_PATH_PARTITIONED && p.scatter_layout == 2 && o.write_combine != 2 &&
-      copy_as_records(q.schema)) {
-    const uint32_t wt = scatter_wc_tile(q.key_bytes, p.pbits, lds_max);
+  if (p.path == TGA_PATH_PARTITIONED && p.scatter_layout == 2 && o.write_combine != 2 && vd > 0) {
+    const uint32_t wt = scatter_wc_tile(q.key_bytes, vd, p.pbits, lds_max);
     if (wt && (o.tile_rows <= 0 || (uint32_t)o.tile_rows == wt)) {
       p.tile_rows = wt;
       p.scatter_wc = 1;
-    } else if (!wt && p.pbits - kTwoPassSubBits >= 1) {
+    } else if (!wt && vd == 1 && p.pbits - kTwoPassSubBits >= 1) {
       // A fanout whose carry buffer does not fit: two passes, written
       // whole-line, the first into 2^(pbits - 4) partitions, the second
       // splitting each by the next 4 hash bits.  Past 2^9 partitions the
@@ -396,14 +402,27 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
       // every line it writes is partial (2^28 rows of 8-byte records at 2^12
       // partitions: 7.7 ms in one pass, 1.0 + 1.2 ms in two).
       const int b1 = p.pbits - kTwoPassSubBits;
-      const uint32_t t1 = scatter_wc_tile(q.key_bytes, b1, lds_max);
+      const uint32_t t1 = scatter_wc_tile(q.key_bytes, 1, b1, lds_max);
       if (t1) {
         p.tile_rows = t1;
         p.scatter_wc = 1;
         p.scatter_p1_bits = b1;
       }
+    } else if (!wt && vd > 1) {
+      // wider records: the widest first pass whose tile fits, then at most
+      // 2^kTwoPassMaxSubBits ways
+      for (int b1 = std::min(p.pbits - 1, 9); b1 >= 1 && p.pbits - b1 <= kTwoPassMaxSubBits; --b1) {
+        const uint32_t t1 = scatter_wc_tile(q.key_bytes, vd, b1, lds_max);
+        if (t1 && scatter_wc_tile(q.key_bytes, vd, p.pbits - b1, lds_max)) {
+          p.tile_rows = t1;
+          p.scatter_wc = 1;
+          p.scatter_p1_bits = b1;
+          break;
+        }
+      }
     }
   }
+  if (p.path == TGA_PATH_PARTITIONED) p.rec_vd = vd == 1 ? 1 : p.scatter_wc ? vd : 0;
   p.opt = o;
   p.may_replan = p.path == TGA_PATH_PARTITIONED && o.partition_bits <= 0 && p.pbits < p.max_pbits;
   // global table at load <= 1/2 of the estimate, at most 2 n slots
@@ -426,6 +445,7 @@ void plan_to_options(const Plan &p, tga_options &o) {
   o.est_groups = (int64_t)(p.gslots / 2);
   o.scatter_layout = p.scatter_layout;
   o.write_combine = p.scatter_wc ? 1 : 2;
+  o.record_words = p.rec_vd;
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +603,7 @@ static uint64_t out_elem_bytes(int dt) { return (uint64_t)dtype_bytes(dt); }
 // write-combining scatter is bound by its LDS work, not by its writes, and
 // uses every CU (1.76 ms with 192 workgroups, 1.43 ms with 256).
 static unsigned scatter_grid(const tga_context *ctx, const QuerySpec &q, const Plan &plan, uint64_t m) {
-  const size_t lds = plan.scatter_wc ? scatter_wc_lds_bytes(q.key_bytes, plan.pbits, plan.tile_rows)
+  const size_t lds = plan.scatter_wc ? scatter_wc_lds_bytes(q.key_bytes, plan.rec_vd, plan.pbits, plan.tile_rows)
                                      : scatter_lds_bytes(q.schema, q.key_bytes, plan.pbits, plan.tile_rows);
   const size_t per_cu = std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / std::max<size_t>(lds, 1)));
   const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);
@@ -619,7 +639,7 @@ struct PartBufs {
   DBuf pcols[kMaxCols];
   BucketBufs bb{};
   bool buckets = false;
-  bool rec = false;  // pkeys holds (key, 4-byte value) records, no pcols
+  int rec = 0;  // > 0: pkeys holds records of `rec` value words (Plan::rec_vd), no pcols
   // two-pass partitioning: the first pass's buffers, and its bucket pool
   // (checked against its capacity once the call is done)
   DBuf heads1, meta1, pkeys1, plan1, items1;
@@ -680,7 +700,8 @@ static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const
   hipStream_t st = ctx->stream;
   const int b1 = plan.scatter_p1_bits, b2 = plan.pbits - b1;
   const uint32_t P1 = 1u << b1, P = 1u << plan.pbits;
-  const uint64_t row_bytes = (uint64_t)q.key_bytes + 4;
+  const int vd = plan.rec_vd;
+  const uint64_t row_bytes = (uint64_t)q.key_bytes + 4 * (uint64_t)vd;
   // pass 1
   const uint64_t cap1 = bucket_cap(m, grid, b1, scatter_block(plan.tile_rows, 1));
   BucketBufs b1bufs{};
@@ -713,8 +734,8 @@ static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const
   slots = cap << kBucketShift;
   if (!pb.pkeys.alloc(ctx, slots * row_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   TGA_HIP(ctx, timed(ctx, "part_scatter2", [&] {
-    return launch_part_scatter2(q.key_bytes, pb.pkeys1.p, b1bufs.blist, ap1.pstart, istart, ticket, P1, plan.pbits,
-                                per_item, b2, pb.pkeys.p, pb.bb, grid2, st);
+    return launch_part_scatter2(q.key_bytes, vd, pb.pkeys1.p, b1bufs.blist, ap1.pstart, istart, ticket, P1,
+                                plan.pbits, per_item, b2, pb.pkeys.p, pb.bb, grid2, st);
   }));
   return TGA_OK;
 }
@@ -725,14 +746,14 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
   const uint32_t P = 1u << plan.pbits;
   const unsigned grid = scatter_grid(ctx, q, plan, m);
   pb.buckets = plan.scatter_layout == 2;
-  pb.rec = copy_as_records(s);
+  pb.rec = plan.rec_vd;
   uint64_t slots = m;
   if (pb.buckets && plan.scatter_p1_bits > 0) {
     if (!pb.rec) return fail(ctx, TGA_ERR_INVALID_ARG, "two-pass partitioning needs a record copy");
     if (tga_status e = partition_two_pass(ctx, q, keys, s, m, plan, grid, pb, slots)) return e;
     if (!pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
     pb.sp = s;
-    pb.sp.cols[0] = nullptr;
+    for (int c = 0; c < kMaxCols; ++c) pb.sp.cols[c] = nullptr;  // the values travel in the records
     pb.ap = part_plan(pb.planbuf, P, pb.gs_cap, pb.gs_per_split);
     pb.ap.blist = pb.bb.blist;
     TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb, P, plan.chunk_rows, pb.ap, st); }));
@@ -743,13 +764,14 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     if (tga_status e = bucket_bufs(ctx, cap, P, pb.heads, pb.meta, pb.bb)) return e;
     slots = cap << kBucketShift;
   }
-  const uint64_t row_bytes = (uint64_t)q.key_bytes + (pb.rec ? 4 : 0);
+  const uint64_t row_bytes = (uint64_t)q.key_bytes + 4 * (uint64_t)pb.rec;
   if (!pb.pkeys.alloc(ctx, std::max<uint64_t>(slots, 1) * row_bytes) ||
       !pb.planbuf.alloc(ctx, (uint64_t)(P + 1) * 12 + 64))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   PartOut po{};
   pb.sp = s;
-  if (pb.rec) pb.sp.cols[0] = nullptr;  // the values travel in the records
+  if (pb.rec)
+    for (int c = 0; c < kMaxCols; ++c) pb.sp.cols[c] = nullptr;  // the values travel in the records
   for (int c = 0; c < s.ncols && !pb.rec; ++c) {
     if (!pb.pcols[c].alloc(ctx, std::max<uint64_t>(slots, 1) * (uint64_t)dtype_bytes(s.col_dtype[c])))
       return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");

@@ -641,52 +641,99 @@ __device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schem
 // Raw 64-bit image of value column c at `row` (dtype packed in coldt, 4 bits
 // per column).  VW = 4 or 8: every value column has that width, so the load
 // is branch-free; VW = 0: the width is read from coldt.
-// Value width code of the row loaders for a partitioned copy stored as
-// records (key, 4-byte value): 8 bytes with 32-bit keys, 12 with 64-bit.
-constexpr int kVwRec4 = 100;
+// Value width codes of the row loaders for a partitioned copy stored as
+// records: kVwRecBase + VD, where VD is the record's value dwords (the value
+// columns in order, each at its own width; see RecW).  kVwRec4: one 4-byte
+// value column, 8-byte records with 32-bit keys, 12-byte with 64-bit.
+constexpr int kVwRecBase = 100;
+constexpr int kVwRec4 = kVwRecBase + 1;
+constexpr bool vw_is_rec(int vw) { return vw > kVwRecBase; }
+constexpr int vw_rec_vd(int vw) { return vw > kVwRecBase ? vw - kVwRecBase : 0; }
 
-// A (key, 4-byte value) record.  The 12-byte form is three 4-byte words:
-// the compiler merges their accesses into one dwordx3 load or store.  (A
-// 3-element vector type would not do: clang widens its loads and stores to
-// 16 bytes, which reaches into the next record.)
-struct Rec12 {
-  uint32_t lo, hi, val;
+// A record of n 4-byte words.  The compiler merges the word accesses of one
+// record into wide loads and stores (dwordx3 for 12 bytes).  (A 3-element
+// vector type would not do: clang widens its loads and stores to 16 bytes,
+// which reaches into the next record.)
+template <int N>
+struct Dwords {
+  uint32_t w[N];
 };
+
+// A record of the partitioned copy: the key (KD = 1 or 2 words, low word
+// first) followed by VD value words, the value columns in order at their own
+// widths (an 8-byte column low word first).  With one 4-byte value column
+// these are the 8- and 12-byte (key, value) records.
+template <typename K, int VD>
+struct RecW {
+  static constexpr int KD = (int)sizeof(K) / 4;
+  static constexpr int RD = KD + VD;
+  static constexpr int kBytes = RD * 4;
+  __device__ __forceinline__ static void unpack(const uint32_t *w, K &key, uint32_t (&val)[VD]) {
+    if constexpr (KD == 2) key = (K)(((uint64_t)w[1] << 32) | w[0]);
+    else key = (K)w[0];
+#pragma unroll
+    for (int j = 0; j < VD; ++j) val[j] = w[KD + j];
+  }
+  // record at byte offset `off` (32-bit) of `base`
+  __device__ __forceinline__ static void load_at(const void *base, uint32_t off, K &key, uint32_t (&val)[VD]) {
+    const char *p = (const char *)base + off;
+    if constexpr (RD == 2) {  // 8-byte aligned: one dwordx2
+      const uint2 v = *(const uint2 *)p;
+      const uint32_t w[2] = {v.x, v.y};
+      unpack(w, key, val);
+    } else if constexpr (RD == 4) {  // 16-byte aligned: one dwordx4
+      const uint4 v = *(const uint4 *)p;
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      unpack(w, key, val);
+    } else {
+      const Dwords<RD> v = *(const Dwords<RD> *)p;
+      unpack(v.w, key, val);
+    }
+  }
+  __device__ __forceinline__ static void load(const void *base, uint64_t r, K &key, uint32_t (&val)[VD]) {
+    load_at((const char *)base + r * (uint64_t)kBytes, 0u, key, val);
+  }
+  __device__ __forceinline__ static void store(void *base, uint64_t r, K key, const uint32_t (&val)[VD]) {
+    char *p = (char *)base + r * (uint64_t)kBytes;
+    Dwords<RD> v;
+    v.w[0] = (uint32_t)(uint64_t)key;
+    if constexpr (KD == 2) v.w[1] = (uint32_t)((uint64_t)key >> 32);
+#pragma unroll
+    for (int j = 0; j < VD; ++j) v.w[KD + j] = val[j];
+    if constexpr (RD == 2) *(uint2 *)p = make_uint2(v.w[0], v.w[1]);
+    else *(Dwords<RD> *)p = v;
+  }
+};
+
+// (key, 4-byte value) records
 template <typename K>
 struct Rec4 {
-  static constexpr int kBytes = (int)sizeof(K) + 4;
-  // record at byte offset `off` (32-bit) of `base`
+  static constexpr int kBytes = RecW<K, 1>::kBytes;
   __device__ __forceinline__ static void load_at(const void *base, uint32_t off, K &key, uint32_t &val) {
-    const char *p = (const char *)base + off;
-    if constexpr (sizeof(K) == 8) {
-      const Rec12 v = *(const Rec12 *)p;
-      key = (K)(((uint64_t)v.hi << 32) | v.lo);
-      val = v.val;
-    } else {
-      const uint2 v = *(const uint2 *)p;
-      key = (K)v.x;
-      val = v.y;
-    }
+    uint32_t v[1];
+    RecW<K, 1>::load_at(base, off, key, v);
+    val = v[0];
   }
   __device__ __forceinline__ static void load(const void *base, uint64_t r, K &key, uint32_t &val) {
-    if constexpr (sizeof(K) == 8) {
-      const Rec12 v = ((const Rec12 *)base)[r];
-      key = (K)(((uint64_t)v.hi << 32) | v.lo);
-      val = v.val;
-    } else {
-      const uint2 v = ((const uint2 *)base)[r];
-      key = (K)v.x;
-      val = v.y;
-    }
+    uint32_t v[1];
+    RecW<K, 1>::load(base, r, key, v);
+    val = v[0];
   }
   __device__ __forceinline__ static void store(void *base, uint64_t r, K key, uint32_t val) {
-    if constexpr (sizeof(K) == 8) {
-      ((Rec12 *)base)[r] = Rec12{(uint32_t)(uint64_t)key, (uint32_t)((uint64_t)key >> 32), val};
-    } else {
-      ((uint2 *)base)[r] = make_uint2((uint32_t)key, val);
-    }
+    const uint32_t v[1] = {val};
+    RecW<K, 1>::store(base, r, key, v);
   }
 };
+
+// Word `o` (uniform) of a record's VD value words, by selects (no indexed
+// register access).  Past the end: word 0.
+template <int VD>
+__device__ __forceinline__ uint32_t rec_word(const uint32_t *w, uint32_t o) {
+  uint32_t r = w[0];
+#pragma unroll
+  for (int d = 1; d < VD; ++d) r = o == (uint32_t)d ? w[d] : r;
+  return r;
+}
 
 template <int VW>
 __device__ __forceinline__ uint64_t load_col(const void *const *cols, uint32_t coldt, int c, uint64_t row) {
@@ -765,7 +812,7 @@ __device__ __noinline__ void spill_row(const GTable *g, const Schema *s, K key, 
 // Register type of a prefetched value cell: 4-byte cells stay 32-bit until
 // widen_block, so no 64-bit register pair is tied to a load in flight.
 template <int VW>
-using CellOf = typename std::conditional<(VW == 4 || VW == kVwRec4), uint32_t, uint64_t>::type;
+using CellOf = typename std::conditional<(VW == 4 || vw_is_rec(VW)), uint32_t, uint64_t>::type;
 
 // Rows [base, base + cnt) of a block (base and cnt wave-uniform, cnt >= 1):
 // lane l takes rows u * 64 + l.
@@ -785,6 +832,14 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
       // a scalar block base plus a 32-bit lane offset
       Rec4<K>::load_at((const char *)keys + base * (uint64_t)Rec4<K>::kBytes, j * (uint32_t)Rec4<K>::kBytes, kk[u], v);
       raw[u][0] = v;
+    } else if constexpr (vw_is_rec(VW)) {  // records of VD value words at `keys`
+      constexpr int VD = vw_rec_vd(VW);
+      static_assert(VD <= NC, "one cell per value word");
+      using R = RecW<K, VD>;
+      uint32_t v[VD];
+      R::load_at((const char *)keys + base * (uint64_t)R::kBytes, j * (uint32_t)R::kBytes, kk[u], v);
+#pragma unroll
+      for (int d = 0; d < NC; ++d) raw[u][d] = d < VD ? v[d < VD ? d : 0] : 0u;
     } else if constexpr (VW == 4) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? ((const uint32_t *)cols[c])[r] : 0u;
@@ -803,6 +858,25 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
 // it is issued.
 template <int NC, int VW, int U>
 __device__ __forceinline__ void widen_block(uint32_t coldt, const CellOf<VW> (*cell)[NC], uint64_t (*raw)[NC]) {
+  if constexpr (vw_rec_vd(VW) > 1) {
+    // records of several value words: column c starts at word o_c, the sum
+    // of the widths before it (coldt is uniform, so are the o_c); absent
+    // columns (dtype 0) read as 0
+    constexpr int VD = vw_rec_vd(VW);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint32_t o = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const uint32_t dt = (coldt >> (4 * c)) & 15u;
+        const uint32_t lo = rec_word<VD>(cell[u], o), hi = rec_word<VD>(cell[u], o + 1);
+        raw[u][c] = dt == 0 ? 0 : dt >= DT_U64 ? ((uint64_t)hi << 32 | lo)
+                  : dt == DT_I32 ? (uint64_t)(int64_t)(int32_t)lo : (uint64_t)lo;
+        o += dt >= DT_U64 ? 2u : 1u;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll

@@ -163,8 +163,10 @@ uint32_t lds_slot_bytes(const Schema &s, int key_bytes);
 
 // two-pass partitioning: the second pass splits each first-pass partition
 // into 2^kTwoPassSubBits (2^28 rows, 2^12 partitions: second pass 1.24 ms
-// with 16 sub-partitions, 1.54 ms with 4)
+// with 16 sub-partitions, 1.54 ms with 4).  Records of several value words
+// take the widest first pass their LDS holds and split up to 2^kTwoPassMaxSubBits.
 constexpr int kTwoPassSubBits = 4;
+constexpr int kTwoPassMaxSubBits = 6;
 
 struct Plan {
   int path = TGA_PATH_DIRECT;
@@ -184,6 +186,7 @@ struct Plan {
   int scatter_wc = 0;      // bucket layout of records written as whole cache lines
   int scatter_p1_bits = 0; // > 0: two passes, the first into 2^scatter_p1_bits partitions
   int max_pbits = 12;      // the largest fanout this query's layouts reach
+  int rec_vd = 0;          // partitioned copy as records of rec_vd value words (0: one array per column)
   // re-planning: the options the plan came from, and whether the fanout was
   // the tuner's (so a too-small estimate may raise it)
   tga_options opt{};

@@ -46,9 +46,15 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
                       const GTable &g, int nbits, int max_probe, unsigned grid, unsigned block, hipStream_t st,
                       int emit = 0, const OutDesc *od = nullptr, uint64_t *emit_count = nullptr, int fold = 0,
                       uint32_t *ovf = nullptr, int rec = 0);
-// rec: `keys` holds (key, 4-byte value) records (a partitioned copy of a
-// one-column schema with a 4-byte value column), not keys and s.cols
+// rec: `keys` holds records, not keys and s.cols -- 1: (key, 4-byte value)
+// records (a partitioned copy of a one-column schema with a 4-byte value
+// column); 2..kMaxRecValueDwords: the key followed by that many value words,
+// the columns in order at their own widths (record_value_dwords(s))
 bool copy_as_records(const Schema &s);
+constexpr int kMaxRecValueDwords = 4;
+// value words of a record of this schema's columns, 0 if they take more
+// than kMaxRecValueDwords (or there are none)
+int record_value_dwords(const Schema &s);
 hipError_t launch_plan_direct(const AggPlan &plan, uint64_t n, uint32_t nchunks, hipStream_t st);
 uint64_t scan_tiles(uint64_t n);
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *desc, uint32_t *ticket,
@@ -78,14 +84,17 @@ uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block);
 // tiles; headers record the partition p1 * 2^pbits + sub-partition).
 hipError_t launch_pass2_plan(const uint64_t *pstart, uint32_t P1, uint32_t items_per, uint32_t *istart,
                              uint32_t *ticket, hipStream_t st);
-hipError_t launch_part_scatter2(int key_bytes, const void *rec, const uint64_t *list, const uint64_t *pstart,
+hipError_t launch_part_scatter2(int key_bytes, int vd, const void *rec, const uint64_t *list, const uint64_t *pstart,
                                 const uint32_t *istart, uint32_t *ticket, uint32_t P1, int hash_bits,
                                 uint32_t items_per, int pbits, void *orec, const BucketBufs &bb, unsigned grid,
                                 hipStream_t st);
-// LDS of the write-combining scatter, and its largest tile (4096, 6144 or
-// 8192 rows) whose LDS fits lds_max, 0 if none does
-size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows);
-uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max);
+// tile rows of launch_part_scatter2 for records of vd value words
+uint32_t scatter2_tile(int vd);
+// LDS of the write-combining scatter over records of vd value words, and its
+// largest tile (4096, 6144 or 8192 rows; 4096 only for vd > 1) whose LDS fits
+// lds_max, 0 if none does
+size_t scatter_wc_lds_bytes(int key_bytes, int vd, int pbits, uint32_t tile_rows);
+uint32_t scatter_wc_tile(int key_bytes, int vd, int pbits, size_t lds_max);
 // rows [0, n) into the partitioned copy okeys / o.cols: into buckets with
 // bb, else partition-contiguous at pbase[p] + offs[tile][p] (from
 // launch_part_count + launch_colscan over the same tiles).  wc: buckets of

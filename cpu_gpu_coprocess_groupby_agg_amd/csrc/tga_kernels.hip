@@ -797,8 +797,8 @@ __host__ __device__ inline uint32_t wc_tasks_max(uint32_t P, uint32_t TR, uint32
   return (P * (kLineDw - 1) + TR * RD) / kLineDw + 1 + P;
 }
 
-size_t scatter_wc_lds_bytes(int key_bytes, int pbits, uint32_t tile_rows) {
-  const uint32_t P = 1u << pbits, RD = (uint32_t)(key_bytes + 4) / 4;
+size_t scatter_wc_lds_bytes(int key_bytes, int vd, int pbits, uint32_t tile_rows) {
+  const uint32_t P = 1u << pbits, RD = (uint32_t)key_bytes / 4 + (uint32_t)vd;
   return (size_t)P * (16 + 5 * 4 + kLineDw * 4) + (size_t)tile_rows * RD * 4 +
          (((size_t)wc_tasks_max(P, tile_rows, RD) * 2 + 15) & ~(size_t)15);
 }
@@ -820,11 +820,21 @@ struct Pass2In {
   uint32_t P1, hash_bits, items_per;
 };
 
-template <typename K, int ITEMS, bool TWO>
+// Value words of a record (first pass): word d of row r is vp[d][r << vsh[d]]
+// (a 4-byte column is one word at stride 1, an 8-byte one two at stride 2).
+struct RecSrc {
+  const uint32_t *vp[kMaxCols];
+  uint32_t vsh[kMaxCols];
+};
+
+template <typename K, int VD, int ITEMS, bool TWO>
 __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ keys, Schema s, uint64_t n,
                                                           int pbits, uint32_t ntiles, void *orec, uint32_t *bpart,
-                                                          uint32_t *bcnt, uint32_t *pool, uint32_t cap, Pass2In p2) {
-  constexpr uint32_t RD = (uint32_t)Rec4<K>::kBytes / 4;     // dwords per record
+                                                          uint32_t *bcnt, uint32_t *pool, uint32_t cap, RecSrc rs,
+                                                          Pass2In p2) {
+  using R = RecW<K, VD>;
+  constexpr uint32_t KD = (uint32_t)R::KD;                  // key words per record
+  constexpr uint32_t RD = (uint32_t)R::RD;                  // words per record
   constexpr uint32_t LB = kBucketRows * RD / kLineDw;       // lines per bucket
   static_assert(kBucketRows * RD % kLineDw == 0, "buckets are whole lines");
   extern __shared__ __align__(16) char smem[];
@@ -868,12 +878,11 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
   // TWO: the list entries of tiles t and t + 1, fetched two tiles ahead (at
   // the top of tile t - 2, by the last wave) so no row load waits on one
   __shared__ uint64_t s_ent[2][kBpt];
-  const uint32_t *col0 = (const uint32_t *)s.cols[0];
 
   // Rows of tile t into registers (rows past the end re-read an existing
   // one), with the mask of the valid ones.
   uint64_t ie0 = 0, ie1 = 0;  // TWO: the work item's list entries
-  auto wc_load = [&](uint32_t t, K(&kk)[ITEMS], uint32_t(&cc)[ITEMS][1], uint32_t &vm) {
+  auto wc_load = [&](uint32_t t, K(&kk)[ITEMS], uint32_t(&cc)[ITEMS][VD], uint32_t &vm) {
     vm = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -881,7 +890,8 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
         const uint64_t row = (uint64_t)t * S.TR + (uint64_t)i * blockDim.x + threadIdx.x;
         const uint64_t r = row < n ? row : n - 1;
         kk[i] = keys[r];
-        cc[i][0] = col0[r];
+#pragma unroll
+        for (int d = 0; d < VD; ++d) cc[i][d] = rs.vp[d][r << rs.vsh[d]];
         vm |= (uint32_t)(row < n) << i;
       } else {
         const uint32_t j = (uint32_t)i * blockDim.x + threadIdx.x;
@@ -889,7 +899,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
         const uint32_t rr = j & (kBucketRows - 1);
         const uint64_t ent = s_ent[t & 1][j >> kBucketShift];
         const bool ok = e < ie1 && rr < (uint32_t)(ent >> 32);
-        Rec4<K>::load(p2.rec, ((ent & 0xffffffffULL) << kBucketShift) + (ok ? rr : 0u), kk[i], cc[i][0]);
+        R::load(p2.rec, ((ent & 0xffffffffULL) << kBucketShift) + (ok ? rr : 0u), kk[i], cc[i]);
         vm |= (uint32_t)ok << i;
       }
     }
@@ -943,7 +953,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
     S.hist[p] = 0;
   }
   K ka[ITEMS], kb[ITEMS];
-  uint32_t ca[ITEMS][1], cb[ITEMS][1];
+  uint32_t ca[ITEMS][VD], cb[ITEMS][VD];
   uint32_t va = 0, vb = 0;
   uint32_t pp[ITEMS], rk[ITEMS];
   if constexpr (TWO) {
@@ -955,7 +965,7 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
   lds_barrier();
   wc_rank(ka, va, S.hist, pp, rk);
   __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
-  scatter_pin<K, ITEMS, 1, uint32_t>(ka, ca);
+  scatter_pin<K, ITEMS, VD, uint32_t>(ka, ca);
   wc_load(tile + 1, kb, cb, vb);  // clamped past the end
 
 #define TGA_WC_TILE(X, XC, XV, Y, YC, YV)                                                        \
@@ -993,13 +1003,13 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
         if ((XV >> i) & 1u) {                                                                    \
           uint32_t *r = stage + (S.hist[pp[i]] + rk[i]) * RD;                                    \
           r[0] = (uint32_t)(uint64_t)X[i];                                                       \
-          if constexpr (RD == 3) r[1] = (uint32_t)((uint64_t)X[i] >> 32);                        \
-          r[RD - 1] = XC[i][0];                                                                  \
+          if constexpr (KD == 2) r[1] = (uint32_t)((uint64_t)X[i] >> 32);                        \
+          _Pragma("unroll") for (uint32_t d_ = 0; d_ < (uint32_t)VD; ++d_) r[KD + d_] = XC[i][d_];   \
         }                                                                                        \
       }                                                                                          \
     }                                                                                            \
     __builtin_amdgcn_s_waitcnt(0x0070); /* vmcnt(0) lgkmcnt(0): Y's loads */                     \
-    scatter_pin<K, ITEMS, 1, uint32_t>(Y, YC);                                                   \
+    scatter_pin<K, ITEMS, VD, uint32_t>(Y, YC);                                                  \
     lds_barrier(); /* staged; task list built */                                                 \
     /* tasks: 8 lanes x 16 bytes per line, 8 tasks per wave step.  Each */                     \
     /* lane of task 0 reads the old carry and writes the new one at the */                      \
@@ -1070,8 +1080,10 @@ __global__ void __launch_bounds__(1024) k_bucket_count(const uint32_t *__restric
                                                        const uint32_t *__restrict__ bcnt, const uint32_t *pool,
                                                        uint32_t cap, uint32_t P, uint32_t *pbk, uint64_t *prow) {
   extern __shared__ __align__(16) char smem[];
-  uint64_t *lrow = (uint64_t *)smem;
-  uint32_t *lbk = (uint32_t *)(lrow + P);
+  // one workgroup's rows of a partition stay below 2^31 (a partitioning
+  // pass covers fewer than 2^31 rows)
+  uint32_t *lrow = (uint32_t *)smem;
+  uint32_t *lbk = lrow + P;
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
     lrow[p] = 0;
     lbk[p] = 0;
@@ -1084,7 +1096,7 @@ __global__ void __launch_bounds__(1024) k_bucket_count(const uint32_t *__restric
     const uint32_t p = bpart[id];
     if (p >= P) continue;
     atomicAdd(&lbk[p], 1u);
-    atomicAdd((unsigned long long *)&lrow[p], (unsigned long long)bcnt[id]);
+    atomicAdd(&lrow[p], bcnt[id]);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
@@ -1679,6 +1691,13 @@ bool copy_as_records(const Schema &s) {
   return s.nfx == 0 && s.ncols == 1 && dtype_bytes(s.col_dtype[0]) == 4;
 }
 
+int record_value_dwords(const Schema &s) {
+  if (s.ncols < 1) return 0;
+  int vd = 0;
+  for (int c = 0; c < s.ncols; ++c) vd += dtype_bytes(s.col_dtype[c]) / 4;
+  return vd <= kMaxRecValueDwords ? vd : 0;
+}
+
 // The compile-time accumulator signature a schema matches (see prog_kind),
 // or 0: one 4-byte value column, the accumulators in one of the fixed orders.
 static uint32_t fixed_signature(const Schema &s) {
@@ -1726,11 +1745,28 @@ hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const Ag
   if (sig == kSigCountSumMinMax) return launch_agg_t<K, false, 1, kVwRec4, false, kSigCountSumMinMax>(TGA_AGG_ARGS); \
   if (sig == kSigCountSum) return launch_agg_t<K, false, 1, kVwRec4, false, kSigCountSum>(TGA_AGG_ARGS); \
   return launch_agg_t<K, false, 1, kVwRec4, false>(TGA_AGG_ARGS);
-  if (rec) {  // keys = (key, 4-byte value) records
+#define TGA_AGG_RECW(K, VD)                                                                     \
+  return fx ? launch_agg_t<K, true, kMaxCols, kVwRecBase + VD, false>(TGA_AGG_ARGS)              \
+            : launch_agg_t<K, false, kMaxCols, kVwRecBase + VD, false>(TGA_AGG_ARGS);
+#define TGA_AGG_RECW_VD(K)                                                                      \
+  switch (rec) {                                                                                \
+    case 2: TGA_AGG_RECW(K, 2)                                                                  \
+    case 3: TGA_AGG_RECW(K, 3)                                                                  \
+    case 4: TGA_AGG_RECW(K, 4)                                                                  \
+    default: return hipErrorInvalidValue;                                                       \
+  }
+  if (rec == 1) {  // keys = (key, 4-byte value) records
     if (fx || !one || w8) return hipErrorInvalidValue;
     if (key_bytes == 4) { TGA_AGG_REC(uint32_t) }
     TGA_AGG_REC(uint64_t)
   }
+  if (rec > 1) {  // keys = records of `rec` value words (no wave folding)
+    if (rec != record_value_dwords(s)) return hipErrorInvalidValue;
+    if (key_bytes == 4) { TGA_AGG_RECW_VD(uint32_t) }
+    TGA_AGG_RECW_VD(uint64_t)
+  }
+#undef TGA_AGG_RECW_VD
+#undef TGA_AGG_RECW
 #undef TGA_AGG_REC
   if (key_bytes == 4) { TGA_AGG_CASE(uint32_t) }
   TGA_AGG_CASE(uint64_t)
@@ -1808,24 +1844,48 @@ uint64_t bucket_cap(uint64_t n, unsigned grid, int pbits, unsigned block) {
   return (n + kBucketRows - 1) / kBucketRows + (uint64_t)grid * ((1ULL << pbits) + block) + 1;
 }
 
-uint32_t scatter_wc_tile(int key_bytes, int pbits, size_t lds_max) {
-  if ((1u << pbits) > kWcMaxP) return 0;
+// Records of one value word take 8192-, 6144- or 4096-row tiles; wider ones
+// 4096 rows only (their two register sets of 8 rows a lane would not fit the
+// 128 registers of a 1024-thread workgroup).
+uint32_t scatter_wc_tile(int key_bytes, int vd, int pbits, size_t lds_max) {
+  if ((1u << pbits) > kWcMaxP || vd < 1 || vd > kMaxRecValueDwords) return 0;
   for (uint32_t items : {8u, 6u, 4u}) {
+    if (vd > 1 && items != 4) continue;
     const uint32_t tr = items * 1024;
-    if (scatter_wc_lds_bytes(key_bytes, pbits, tr) + 256 <= lds_max) return tr;
+    if (scatter_wc_lds_bytes(key_bytes, vd, pbits, tr) + 256 <= lds_max) return tr;
   }
   return 0;
 }
 
-template <typename K, int ITEMS, bool TWO = false>
+static RecSrc rec_src(const Schema &s) {
+  RecSrc r{};
+  int d = 0;
+  for (int c = 0; c < s.ncols && d < kMaxCols; ++c) {
+    const uint32_t *p = (const uint32_t *)s.cols[c];
+    if (dtype_bytes(s.col_dtype[c]) == 4) {
+      r.vp[d] = p;
+      r.vsh[d++] = 0;
+    } else {
+      r.vp[d] = p;
+      r.vsh[d++] = 1;
+      if (d < kMaxCols) {
+        r.vp[d] = p + 1;
+        r.vsh[d++] = 1;
+      }
+    }
+  }
+  return r;
+}
+
+template <typename K, int VD, int ITEMS, bool TWO = false>
 static hipError_t launch_scatter_wc_t(const void *keys, const Schema &s, uint64_t n, int pbits, uint32_t tile_rows,
                                       uint32_t ntiles, void *orec, const BucketBufs &bb, unsigned grid,
                                       hipStream_t st, const Pass2In &p2 = Pass2In{}) {
-  const size_t lds = scatter_wc_lds_bytes(sizeof(K), pbits, tile_rows);
-  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter_wc<K, ITEMS, TWO>, lds);
+  const size_t lds = scatter_wc_lds_bytes(sizeof(K), VD, pbits, tile_rows);
+  hipError_t e = set_lds_limit<K>((const void *)&k_part_scatter_wc<K, VD, ITEMS, TWO>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_part_scatter_wc<K, ITEMS, TWO>), dim3(grid), dim3(1024), lds, st, (const K *)keys, s, n,
-                     pbits, ntiles, orec, bb.bpart, bb.bcnt, bb.pool, bb.cap, p2);
+  hipLaunchKernelGGL((k_part_scatter_wc<K, VD, ITEMS, TWO>), dim3(grid), dim3(1024), lds, st, (const K *)keys, s, n,
+                     pbits, ntiles, orec, bb.bpart, bb.bcnt, bb.pool, bb.cap, rec_src(s), p2);
   return hipGetLastError();
 }
 
@@ -1852,17 +1912,27 @@ hipError_t launch_pass2_plan(const uint64_t *pstart, uint32_t P1, uint32_t items
   return hipGetLastError();
 }
 
-hipError_t launch_part_scatter2(int key_bytes, const void *rec, const uint64_t *list, const uint64_t *pstart,
+hipError_t launch_part_scatter2(int key_bytes, int vd, const void *rec, const uint64_t *list, const uint64_t *pstart,
                                 const uint32_t *istart, uint32_t *ticket, uint32_t P1, int hash_bits,
                                 uint32_t items_per, int pbits, void *orec, const BucketBufs &bb, unsigned grid,
                                 hipStream_t st) {
   if (pbits < 1 || hash_bits <= pbits || (1u << pbits) > kWcMaxP || items_per == 0) return hipErrorInvalidValue;
   Pass2In p2{rec, list, pstart, istart, ticket, P1, (uint32_t)hash_bits, items_per};
   const Schema s{};
-  if (key_bytes == 4)
-    return launch_scatter_wc_t<uint32_t, 8, true>(nullptr, s, 0, pbits, 8192, 0, orec, bb, grid, st, p2);
-  return launch_scatter_wc_t<uint64_t, 8, true>(nullptr, s, 0, pbits, 8192, 0, orec, bb, grid, st, p2);
+#define TGA_P2_CASE(K)                                                                                  \
+  switch (vd) {                                                                                         \
+    case 1: return launch_scatter_wc_t<K, 1, 8, true>(nullptr, s, 0, pbits, 8192, 0, orec, bb, grid, st, p2); \
+    case 2: return launch_scatter_wc_t<K, 2, 4, true>(nullptr, s, 0, pbits, 4096, 0, orec, bb, grid, st, p2); \
+    case 3: return launch_scatter_wc_t<K, 3, 4, true>(nullptr, s, 0, pbits, 4096, 0, orec, bb, grid, st, p2); \
+    case 4: return launch_scatter_wc_t<K, 4, 4, true>(nullptr, s, 0, pbits, 4096, 0, orec, bb, grid, st, p2); \
+    default: return hipErrorInvalidValue;                                                               \
+  }
+  if (key_bytes == 4) { TGA_P2_CASE(uint32_t) }
+  TGA_P2_CASE(uint64_t)
+#undef TGA_P2_CASE
 }
+
+uint32_t scatter2_tile(int vd) { return vd > 1 ? 4096u : 8192u; }
 
 hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s, uint64_t n, int pbits,
                                uint32_t tile_rows, void *okeys, const PartOut &o, const BucketBufs *bb,
@@ -1870,12 +1940,17 @@ hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s,
   const uint32_t ntiles = (uint32_t)((n + tile_rows - 1) / tile_rows);
   if (ntiles == 0) return hipSuccess;
   if (wc) {  // records into buckets, complete lines only
-    if (!bb || !copy_as_records(s) || tile_rows % 1024 || (1u << pbits) > kWcMaxP) return hipErrorInvalidValue;
+    const int vd = record_value_dwords(s);
+    if (!bb || !vd || tile_rows % 1024 || (1u << pbits) > kWcMaxP || (vd > 1 && tile_rows != 4096))
+      return hipErrorInvalidValue;
 #define TGA_WC_CASE(K)                                                                                  \
+  if (vd == 2) return launch_scatter_wc_t<K, 2, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+  if (vd == 3) return launch_scatter_wc_t<K, 3, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+  if (vd == 4) return launch_scatter_wc_t<K, 4, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
   switch (tile_rows / 1024) {                                                                           \
-    case 4: return launch_scatter_wc_t<K, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
-    case 6: return launch_scatter_wc_t<K, 6>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
-    case 8: return launch_scatter_wc_t<K, 8>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    case 4: return launch_scatter_wc_t<K, 1, 4>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    case 6: return launch_scatter_wc_t<K, 1, 6>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
+    case 8: return launch_scatter_wc_t<K, 1, 8>(keys, s, n, pbits, tile_rows, ntiles, okeys, *bb, grid, st); \
     default: return hipErrorInvalidValue;                                                               \
   }
     if (key_bytes == 4) { TGA_WC_CASE(uint32_t) }
@@ -1950,7 +2025,7 @@ static hipError_t lds_attr(const void *fn, size_t bytes) {
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
                                hipStream_t st) {
   const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(256, bb.cap / 4096));
-  size_t lds = (size_t)P * 12;
+  size_t lds = (size_t)P * 8;
   hipError_t e = lds_attr((const void *)&k_bucket_count, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_bucket_count, dim3(grid), dim3(1024), lds, st, bb.bpart, bb.bcnt, bb.pool, bb.cap, P, bb.pbk,

@@ -92,7 +92,9 @@ typedef struct tga_options {
                              columns of >= 2^22 rows), 0 = device-resident or staged whole */
   int32_t stream_pinned;  /* tga_last_plan only: 1 = the streamed host columns were pinned, so the copies
                              overlapped the aggregation */
-  int32_t reserved[2];
+  int32_t record_words;   /* tga_last_plan only: the partitioned copy was records of the key and this many
+                             4-byte value words (the value columns in order), 0 = one array per column */
+  int32_t reserved[1];
 } tga_options;
 
 typedef struct tga_context tga_context;

@@ -693,6 +693,40 @@ int main(int argc, char **argv) {
   }
   cases.push_back({"single_key", 64, std::vector<uint64_t>(N, 42), {col_i32(N, -(1LL << 31), (1LL << 31) - 1, 25)},
                    {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2)});
+  // Records of several value words (the key, then the value columns at their
+  // own widths), written by the write-combining scatter in one pass (few
+  // partitions) or two (the first pass as wide as the LDS holds, the second
+  // up to 64 ways), read by the aggregation: 64-bit keys with int32 + fp64
+  // columns (5 words, exact fp64 SUM, fp64 MIN/MAX), 32-bit keys with one
+  // 8-byte column (3 words), 32-bit keys with four int32 columns and 64-bit
+  // keys with two fp64 columns (4 value words)
+  {
+    const int64_t n3 = 3 * N;
+    const std::vector<tga_agg> af = {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_SUM, 1}, {TGA_MIN, 1}, {TGA_MAX, 0}};
+    for (int bits : {5, 9, 11, 14}) {
+      cases.push_back({"recw_u64_i32_f64_p" + std::to_string(bits), 64, keys_u64(n3, 150000, 101 + bits),
+                       {col_i32(n3, -(1LL << 31), (1LL << 31) - 1, 102), col_f64(n3, 103)}, {TGA_I32, TGA_F64}, af,
+                       opts(TGA_PATH_PARTITIONED, 2, bits, 0, 150000)});
+    }
+    cases.push_back({"recw_u32_u64_p12", 32, keys_u32(n3, 200000, 111), {col_u64(n3, 112)}, {TGA_U64},
+                     {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_PARTITIONED, 2, 12, 0, 200000)});
+    cases.push_back({"recw_u32_4xi32_p10", 32, keys_u32(n3, 90000, 113),
+                     {col_i32(n3, -9, 9, 114), col_i32(n3, -(1LL << 31), (1LL << 31) - 1, 115), col_i32(n3, 0, 99, 116),
+                      col_i32(n3, -5, 5, 117)},
+                     {TGA_I32, TGA_I32, TGA_U32, TGA_I32},
+                     {{TGA_SUM, 0}, {TGA_MIN, 1}, {TGA_MAX, 2}, {TGA_SUM, 3}, {TGA_COUNT, 0}, {TGA_MAX, 1}},
+                     opts(TGA_PATH_PARTITIONED, 2, 10, 0, 90000)});
+    cases.push_back({"recw_u64_2xf64_p10", 64, keys_u64(n3, 120000, 118), {col_f64(n3, 119), col_f64(n3, 120)},
+                     {TGA_F64, TGA_F64}, {{TGA_SUM, 0}, {TGA_SUM, 1}, {TGA_MIN, 0}, {TGA_MAX, 1}, {TGA_COUNT, 0}},
+                     opts(TGA_PATH_PARTITIONED, 2, 10, 0, 120000)});
+    // a hot key over half the rows (its first-pass partition cut into work
+    // items, its final partition split into chunks through the global
+    // table), the fanout and group count left to the tuner
+    std::vector<uint64_t> k = keys_u64(n3, 100000, 121);
+    for (int64_t i = 0; i < n3; i += 2) k[i] = 987654321;
+    cases.push_back({"recw_hot_key_auto", 64, k, {col_i32(n3, -(1LL << 31), (1LL << 31) - 1, 122), col_f64(n3, 123)},
+                     {TGA_I32, TGA_F64}, af, opts(TGA_PATH_AUTO, 2)});
+  }
   int bad = 0, ran = 0;
   for (const Case &c : cases) {
     if (!filter.empty() && c.name.find(filter) == std::string::npos) continue;

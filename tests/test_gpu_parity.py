@@ -280,6 +280,47 @@ def test_zipf_hot_keys_int_and_fp64(ctx, path):
     assert s[top] == math.fsum(fh[kh == got["keys"][top]].tolist())
 
 
+@pytest.mark.parametrize("schema,bits", [("i32_f64", 7), ("i32_f64", 9), ("i32_f64", 12), ("i32_f64", 14),
+                                         ("u64", 11), ("f64_f64", 10), ("4xi32", 10), ("i32_f64", 0)])
+def test_record_copy_several_value_words(ctx, schema, bits):
+    """The partitioned copy as records of the key and several value words
+    (columns in order at their own widths), written by the write-combining
+    scatter in one pass or two (up to 2^14 partitions), read back by the
+    aggregation: integer aggregates bit-exact, fp64 SUM within 1e-12 relative
+    of the compensated oracle.  bits 0: the tuner's plan for Zipf(1.1) keys."""
+    n = 1 << 22
+    D = 1 << 20
+    if bits == 0:
+        k = ctx.generate(tga.GEN_ZIPF_U64, n, a=D, x=1.1, seed=81, salt=4)
+    else:
+        k = ctx.generate(tga.GEN_KEYS_U64, n, a=D, seed=81, salt=4)
+    i32 = lambda s: ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=s)
+    f64 = lambda s: ctx.generate(tga.GEN_NORMAL_F64, n, x=0.0, y=1e3, seed=s)
+    if schema == "i32_f64":
+        cols, aggs = [i32(82), f64(83)], ["count", ("sum", 0), ("sum", 1), ("min", 1), ("max", 0)]
+    elif schema == "u64":
+        cols, aggs = [ctx.generate(tga.GEN_KEYS_U64, n, a=1 << 40, seed=84, salt=1)], AGGS4
+    elif schema == "f64_f64":
+        cols, aggs = [f64(85), f64(86)], [("sum", 0), ("sum", 1), ("max", 0), ("min", 1), "count"]
+    else:
+        cols = [i32(87 + c) for c in range(4)]
+        aggs = [("sum", 0), ("min", 1), ("max", 2), ("sum", 3), "count"]
+    opts = {} if bits == 0 else {"path": tga.PATH_PARTITIONED, "partition_bits": bits, "est_groups": D}
+    got = ctx.groupby(k, cols, aggs, **opts)
+    plan = ctx.last_plan()
+    assert plan["record_words"] > 1 and plan["write_combine"] == 1, plan
+    if bits:
+        assert plan["partition_bits"] == bits
+    ref = O.groupby(k.to_numpy(), [c.to_numpy() for c in cols], aggs)
+    assert got["n_groups"] == ref["n_groups"]
+    np.testing.assert_array_equal(got["keys"], ref["keys"])
+    for a, g, r in zip(aggs, got["aggs"], ref["aggs"]):
+        if g.dtype == np.float64 and a != "count" and a[0] == "sum":
+            assert np.all(np.abs(g - r) <= 1e-12 * np.maximum(np.abs(r), 1e-300))
+        else:
+            np.testing.assert_array_equal(g, r)
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("path", [tga.PATH_AUTO, tga.PATH_DIRECT, tga.PATH_PARTITIONED])
 def test_closed_form_columns_every_group(ctx, bits, path):
