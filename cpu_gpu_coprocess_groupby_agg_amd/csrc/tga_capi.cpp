@@ -504,8 +504,9 @@ uint64_t estimate_from_stats(uint64_t d, uint64_t f1, uint64_t top, uint64_t m, 
 }
 
 tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est,
-                           double *top_share) {
+                           double *top_share, bool *deferred, uint64_t *sample_d_out) {
   if (top_share) *top_share = 0.0;
+  if (deferred) *deferred = false;
   if (n == 0) { *est = 1; return TGA_OK; }
   // the sample is counted on the GPU; only its counts come back
   const uint64_t m = std::min<uint64_t>(n, 16384);
@@ -531,6 +532,11 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
     for (uint64_t i = 0; i < m; ++i) sample_d += (i == 0 || v[i] != v[i - 1]);  // v is sorted now
   }
   if (reliable || n < kSliceMinRows) return TGA_OK;
+  if (deferred) {
+    *deferred = true;
+    if (sample_d_out) *sample_d_out = sample_d;
+    return TGA_OK;
+  }
   // Otherwise count the distinct keys of one 2^-10 slice of the partition
   // hash exactly, over all rows (one read of the keys): whatever the
   // frequencies, 2^10 times that count is within ~(D/2^10)^-1/2 of D.
@@ -540,6 +546,35 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   bool overflow = false;
   if (tga_status e = slice_count(ctx, d_keys, key_bytes, n, &count, &overflow)) return e;
   *est = combine_slice_estimate(*est, sample_d, count, overflow, n);
+  return TGA_OK;
+}
+
+// The widest first-pass fanout of an adaptive plan: the write-combining
+// scatter's largest (at most 2^9) for this record width, where the second
+// pass may still reach the plan's maximum fanout.
+static int adapt_first_bits(const tga_context *ctx, const QuerySpec &q, const Plan &p) {
+  const size_t lds_max = ctx->lds_max - 1024;
+  for (int b = 9; b >= 1; --b)
+    if (scatter_wc_tile(q.key_bytes, p.rec_vd, b, lds_max) && b + kTwoPassMaxSubBits >= p.max_pbits) return b;
+  return 0;
+}
+
+tga_status resolve_deferred_estimate(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n,
+                                     uint64_t est, uint64_t sample_d, const tga_options *opt, Plan &plan) {
+  const bool forced = opt && (opt->partition_bits > 0 || opt->tile_rows > 0 || opt->path != TGA_PATH_AUTO);
+  if (!forced && plan.path == TGA_PATH_PARTITIONED && plan.scatter_layout == 2 && plan.scatter_wc &&
+      plan.rec_vd > 0 && n < (1ULL << 31)) {
+    const int b1 = adapt_first_bits(ctx, q, plan);
+    if (b1 > 0 && b1 <= kSliceBits) {
+      plan.adapt_bits = b1;
+      plan.sample_d = sample_d;
+      return TGA_OK;
+    }
+  }
+  uint64_t c = 0;
+  bool ovf = false;
+  if (tga_status e = slice_count(ctx, d_keys, q.key_bytes, n, &c, &ovf)) return e;
+  plan = autotune(ctx, q, n, combine_slice_estimate(est, sample_d, c, ovf, n), opt);
   return TGA_OK;
 }
 
@@ -645,6 +680,12 @@ struct PartBufs {
   DBuf heads1, meta1, pkeys1, plan1, items1;
   uint32_t *pool1 = nullptr;
   uint32_t cap1 = 0;
+  BucketBufs bb1{};
+  AggPlan ap1{};
+  // adaptive plans: the first pass is done (pass1_final: it is the final
+  // copy, its lists and plan built for the aggregation with gs_cap1)
+  bool pass1_done = false, pass1_final = false;
+  uint64_t gs_cap1 = 0;
   AggPlan ap{};
   Schema sp{};  // the schema over the partitioned copy
   // > 0: the plan kernel sizes the global table, up to gs_cap slots
@@ -695,6 +736,33 @@ static tga_status bucket_bufs(tga_context *ctx, uint64_t cap, uint32_t P, DBuf &
 // partitions spread over more items).
 static uint32_t pass2_entries(uint32_t P1) { return P1 >= 256 ? 512 : 1024; }
 
+// First pass of a bucket-layout partitioning of records into 2^b1
+// partitions (write-combining scatter with plan.tile_rows-row tiles), and its
+// bucket lists and work plan (items of chmax rows; gs_cap > 0: the plan
+// kernel also sizes the global table) into the *1 buffers of pb.
+static tga_status first_pass(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s, uint64_t m,
+                             const Plan &plan, int b1, unsigned grid, uint64_t chmax, uint64_t gs_cap,
+                             uint32_t gs_per_split, PartBufs &pb) {
+  hipStream_t st = ctx->stream;
+  const uint32_t P1 = 1u << b1;
+  const uint64_t row_bytes = (uint64_t)q.key_bytes + 4 * (uint64_t)plan.rec_vd;
+  const uint64_t cap1 = bucket_cap(m, grid, b1, scatter_block(plan.tile_rows, 1));
+  if (tga_status e = bucket_bufs(ctx, cap1, P1, pb.heads1, pb.meta1, pb.bb1)) return e;
+  if (!pb.pkeys1.alloc(ctx, (cap1 << kBucketShift) * row_bytes) || !pb.plan1.alloc(ctx, (uint64_t)(P1 + 1) * 12 + 64))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
+  pb.pool1 = pb.bb1.pool;
+  pb.cap1 = (uint32_t)cap1;
+  PartOut po{};
+  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
+    return launch_part_scatter(q.key_bytes, keys, s, m, b1, plan.tile_rows, pb.pkeys1.p, po, &pb.bb1, nullptr, nullptr,
+                               grid, st, 1);
+  }));
+  pb.ap1 = part_plan(pb.plan1, P1, gs_cap, gs_per_split);
+  pb.ap1.blist = pb.bb1.blist;
+  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb1, P1, chmax, pb.ap1, st); }));
+  return TGA_OK;
+}
+
 static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const void *keys, const Schema &s,
                                      uint64_t m, const Plan &plan, unsigned grid, PartBufs &pb, uint64_t &slots) {
   hipStream_t st = ctx->stream;
@@ -702,29 +770,15 @@ static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const
   const uint32_t P1 = 1u << b1, P = 1u << plan.pbits;
   const int vd = plan.rec_vd;
   const uint64_t row_bytes = (uint64_t)q.key_bytes + 4 * (uint64_t)vd;
-  // pass 1
-  const uint64_t cap1 = bucket_cap(m, grid, b1, scatter_block(plan.tile_rows, 1));
-  BucketBufs b1bufs{};
-  if (tga_status e = bucket_bufs(ctx, cap1, P1, pb.heads1, pb.meta1, b1bufs)) return e;
-  if (!pb.pkeys1.alloc(ctx, (cap1 << kBucketShift) * row_bytes) || !pb.plan1.alloc(ctx, (uint64_t)(P1 + 1) * 12 + 64) ||
-      !pb.items1.alloc(ctx, (uint64_t)(P1 + 2) * 4))
-    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
-  pb.pool1 = b1bufs.pool;
-  pb.cap1 = (uint32_t)cap1;
-  PartOut po{};
-  TGA_HIP(ctx, timed(ctx, "part_scatter", [&] {
-    return launch_part_scatter(q.key_bytes, keys, s, m, b1, plan.tile_rows, pb.pkeys1.p, po, &b1bufs, nullptr, nullptr,
-                               grid, st, 1);
-  }));
-  AggPlan ap1 = part_plan(pb.plan1, P1);
-  ap1.blist = b1bufs.blist;
+  // pass 1 (already done by an adaptive plan)
+  if (!pb.pass1_done)
+    if (tga_status e = first_pass(ctx, q, keys, s, m, plan, b1, grid, ~0ULL >> 1, 0, 0, pb)) return e;
+  const uint64_t cap1 = pb.cap1;
+  const AggPlan &ap1 = pb.ap1;
+  if (!pb.items1.alloc(ctx, (uint64_t)(P1 + 2) * 4)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   const uint32_t per_item = pass2_entries(P1);
   uint32_t *istart = pb.items1.as<uint32_t>(), *ticket = istart + P1 + 1;
-  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] {
-    hipError_t e = launch_bucket_lists(b1bufs, P1, ~0ULL >> 1, ap1, st);
-    if (e != hipSuccess) return e;
-    return launch_pass2_plan(ap1.pstart, P1, per_item, istart, ticket, st);
-  }));
+  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_pass2_plan(ap1.pstart, P1, per_item, istart, ticket, st); }));
   // pass 2: at most P1 + cap1 / per_item work items, each leaving up to
   // 2^b2 buckets open, and a reserve per wave
   const unsigned grid2 = (unsigned)(ctx->cus + 7) / 8 * 8;  // every CU, a multiple of the XCD count
@@ -734,7 +788,7 @@ static tga_status partition_two_pass(tga_context *ctx, const QuerySpec &q, const
   slots = cap << kBucketShift;
   if (!pb.pkeys.alloc(ctx, slots * row_bytes)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   TGA_HIP(ctx, timed(ctx, "part_scatter2", [&] {
-    return launch_part_scatter2(q.key_bytes, vd, pb.pkeys1.p, b1bufs.blist, ap1.pstart, istart, ticket, P1,
+    return launch_part_scatter2(q.key_bytes, vd, pb.pkeys1.p, pb.bb1.blist, ap1.pstart, istart, ticket, P1,
                                 plan.pbits, per_item, b2, pb.pkeys.p, pb.bb, grid2, st);
   }));
   return TGA_OK;
@@ -748,6 +802,20 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
   pb.buckets = plan.scatter_layout == 2;
   pb.rec = plan.rec_vd;
   uint64_t slots = m;
+  if (pb.pass1_final) {
+    // an adaptive plan's first pass is the partitioned copy
+    pb.pkeys = std::move(pb.pkeys1);
+    pb.heads = std::move(pb.heads1);
+    pb.meta = std::move(pb.meta1);
+    pb.planbuf = std::move(pb.plan1);
+    pb.bb = pb.bb1;
+    pb.ap = pb.ap1;
+    pb.pool1 = nullptr;
+    pb.cap1 = 0;
+    pb.sp = s;
+    for (int c = 0; c < kMaxCols; ++c) pb.sp.cols[c] = nullptr;  // the values travel in the records
+    return TGA_OK;
+  }
   if (pb.buckets && plan.scatter_p1_bits > 0) {
     if (!pb.rec) return fail(ctx, TGA_ERR_INVALID_ARG, "two-pass partitioning needs a record copy");
     if (tga_status e = partition_two_pass(ctx, q, keys, s, m, plan, grid, pb, slots)) return e;
@@ -889,6 +957,69 @@ tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, c
 // plan.gslots slots, so a full-size one may do).  *replan_est > 0: partitions processed
 // whole overflowed their LDS table and were abandoned; the value estimates
 // the group count for a finer partitioning (retry with it).
+// Adaptive plan (plan.adapt_bits > 0): the first pass into 2^b1 partitions
+// with its bucket lists and work plan as a one-pass plan at 2^b1 would have
+// them, then the exact distinct count of the keys in the 2^-kSliceBits hash
+// slice, which lies inside its partition 0 (only that partition's records
+// are read: 1/2^b1 of the rows, where the count over all keys reads every
+// key), then the plan for the resulting estimate: the first pass is the
+// partitioned copy if that plan's fanout is at most 2^b1, else a second pass
+// splits it up to 2^kTwoPassMaxSubBits ways.  plan.adapt_bits is 0 after.
+static tga_status adapt_first_pass(tga_context *ctx, const QuerySpec &q, const void *keys, uint64_t n, Plan &plan,
+                                   bool full_table, PartBufs &pb) {
+  hipStream_t st = ctx->stream;
+  const int b1 = plan.adapt_bits, vd = plan.rec_vd;
+  const uint32_t P1 = 1u << b1;
+  const size_t lds_max = ctx->lds_max - 1024;
+  Plan p1 = plan;  // the first pass as a one-pass plan at 2^b1
+  p1.pbits = b1;
+  p1.scatter_p1_bits = 0;
+  p1.scatter_wc = 1;
+  p1.tile_rows = scatter_wc_tile(q.key_bytes, vd, b1, lds_max);
+  p1.chunk_rows = plan.opt.chunk_rows > 0 ? (uint64_t)plan.opt.chunk_rows
+                                          : std::max<uint64_t>(65536, 2 * ((n + P1 - 1) / P1));
+  if (!p1.tile_rows) return fail(ctx, TGA_ERR_INVALID_ARG, "adaptive plan without a first-pass tile");
+  const unsigned grid = scatter_grid(ctx, q, p1, n);
+  const uint64_t T = plan.table_slots;
+  pb.gs_cap1 = full_table ? 0 : next_pow2(std::max<uint64_t>(4096, T * P1));
+  if (tga_status e = first_pass(ctx, q, keys, q.schema, n, p1, b1, grid, p1.chunk_rows, pb.gs_cap1,
+                                (uint32_t)(2 * T), pb))
+    return e;
+  // the hash-slice count over partition 0
+  const uint32_t slots = slice_set_slots(n);
+  DBuf set, cbuf;
+  if (!set.alloc(ctx, (uint64_t)slots * q.key_bytes) || !cbuf.alloc(ctx, 64))
+    return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "slice set");
+  TGA_HIP(ctx, timed(ctx, "slice_records", [&] {
+    return launch_slice_records(q.key_bytes, pb.pkeys1.p, (uint32_t)(q.key_bytes / 4 + vd), pb.ap1.blist,
+                                pb.ap1.pstart, set.p, slots, cbuf.as<uint64_t>(), (unsigned)ctx->cus, st);
+  }));
+  uint64_t c[2];
+  TGA_HIP(ctx, hipMemcpyAsync(c, cbuf.p, sizeof c, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipStreamSynchronize(st));
+  const uint64_t est = combine_slice_estimate(plan.est, plan.sample_d, c[0], c[1] != 0, n);
+  Plan np = autotune(ctx, q, n, est, &plan.opt);
+  int bits = np.path == TGA_PATH_PARTITIONED ? np.pbits : b1;
+  bits = std::max(b1, std::min(bits, b1 + kTwoPassMaxSubBits));
+  if (np.path != TGA_PATH_PARTITIONED || np.rec_vd != vd || np.table_slots != plan.table_slots) np = plan;
+  np.est = est;
+  np.gslots = std::min<uint64_t>(next_pow2(std::max<uint64_t>(1024, est * 2)),
+                                 next_pow2(std::max<uint64_t>(1024, 2 * n)));
+  np.pbits = bits;
+  np.max_pbits = std::max(np.max_pbits, bits);
+  np.scatter_wc = 1;
+  np.tile_rows = p1.tile_rows;
+  np.scatter_p1_bits = bits > b1 ? b1 : 0;
+  np.chunk_rows = plan.opt.chunk_rows > 0 ? (uint64_t)plan.opt.chunk_rows
+                                          : std::max<uint64_t>(65536, 2 * ((n + (1ULL << bits) - 1) >> bits));
+  np.may_replan = bits < np.max_pbits;
+  np.adapt_bits = 0;
+  plan = np;
+  pb.pass1_done = true;
+  pb.pass1_final = bits == b1;
+  return TGA_OK;
+}
+
 static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n, Plan &plan,
                                   bool full_table, PlainOut *out, void *ext_keys, void *const *ext_cols,
                                   uint64_t ext_cap, uint64_t *groups, bool *overflow, uint64_t *replan_est,
@@ -896,6 +1027,9 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   *replan_est = 0;
   hipStream_t st = ctx->stream;
   const Schema &s = q.schema;
+  PartBufs pb;
+  if (plan.adapt_bits > 0)
+    if (tga_status e = adapt_first_pass(ctx, q, d_keys, n, plan, full_table, pb)) return e;
   const uint32_t P = 1u << plan.pbits;
   DBuf counters, ovf;
   // at most one abandoned record per work item: partitions plus split chunks
@@ -916,11 +1050,13 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   // too small (spills from partitions holding more groups than their LDS
   // table) raises the overflow flag, and the call is retried with the
   // full-size one.
-  PartBufs pb;
   const uint64_t T = plan.table_slots;
   uint64_t S = plan.gslots;
   if (!full_table) {
     S = std::min(S, next_pow2(std::max<uint64_t>(4096, T * P)));
+    // a first pass kept as the copy had its plan kernel size the table up
+    // to gs_cap1
+    if (pb.pass1_final) S = std::max(S, pb.gs_cap1);
     pb.gs_cap = S;
     pb.gs_per_split = (uint32_t)(2 * T);
   }
@@ -1230,11 +1366,14 @@ tga_status prepare(tga_context *ctx, const tga_column *key, const tga_column *va
   if (st) return st;
   for (int c = 0; c < n_values; ++c) q.schema.cols[c] = sg.vals[c].data;
   uint64_t est = opt && opt->est_groups > 0 ? (uint64_t)opt->est_groups : 0;
+  bool deferred = false;
+  uint64_t sample_d = 0;
   if (!est) {
-    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est);
+    st = estimate_groups(ctx, sg.key.data, q.key_bytes, (uint64_t)n_rows, &est, nullptr, &deferred, &sample_d);
     if (st) return st;
   }
   plan = autotune(ctx, q, (uint64_t)n_rows, est, opt);
+  if (deferred) return resolve_deferred_estimate(ctx, q, sg.key.data, (uint64_t)n_rows, est, sample_d, opt, plan);
   return TGA_OK;
 }
 }  // namespace

@@ -187,6 +187,12 @@ struct Plan {
   int scatter_p1_bits = 0; // > 0: two passes, the first into 2^scatter_p1_bits partitions
   int max_pbits = 12;      // the largest fanout this query's layouts reach
   int rec_vd = 0;          // partitioned copy as records of rec_vd value words (0: one array per column)
+  // Adaptive fanout (adapt_bits > 0): the group count is still open (the
+  // sample left it so), so the first pass partitions into 2^adapt_bits, the
+  // hash-slice count is taken over that pass's partition 0, and the final
+  // fanout (adapt_bits up to adapt_bits + kTwoPassMaxSubBits) follows from it.
+  int adapt_bits = 0;
+  uint64_t sample_d = 0;   // the sample's distinct keys (a lower bound)
   // re-planning: the options the plan came from, and whether the fanout was
   // the tuner's (so a too-small estimate may raise it)
   tga_options opt{};
@@ -205,8 +211,13 @@ void plan_to_options(const Plan &p, tga_options &o);
 // Charikar et al. 2000, or the equal-frequency model), replaced by an exact
 // count over a 2^-10 hash slice of all keys when the sample leaves the
 // count open.  Needs the context lock.
+// With `deferred` non-null, an unreliable sample over at least
+// kSliceMinRows rows returns the sample's estimate with *deferred = true and
+// *sample_d = the sample's distinct keys, and the caller counts the slice
+// itself (resolve_deferred_estimate, or over the first pass of an adaptive
+// partitioned plan).
 tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, uint64_t n, uint64_t *est,
-                           double *top_share = nullptr);
+                           double *top_share = nullptr, bool *deferred = nullptr, uint64_t *sample_d = nullptr);
 // Distinct-count estimate of n rows from a sample of m keys (sorted in place).
 // *top_share (optional): the largest share of the sample one key takes.
 // *reliable (optional): false when the sample, mostly singletons, leaves
@@ -230,6 +241,12 @@ uint64_t combine_slice_estimate(uint64_t sample_est, uint64_t sample_d, uint64_t
 // An unreliable sample estimate over at least this many rows is replaced by
 // an exact count over a hash slice of the keys.
 constexpr uint64_t kSliceMinRows = 1ULL << 22;
+// After a deferred estimate: the plan made from the sample's estimate becomes
+// adaptive when its partitioned copy is records in bucket layout (the slice
+// is then counted over the first pass); otherwise the slice is counted over
+// all keys now and the plan made again.
+tga_status resolve_deferred_estimate(tga_context *ctx, const QuerySpec &q, const void *d_keys, uint64_t n,
+                                     uint64_t est, uint64_t sample_d, const tga_options *opt, Plan &plan);
 
 inline uint64_t next_pow2_u64(uint64_t x) {
   uint64_t p = 1;

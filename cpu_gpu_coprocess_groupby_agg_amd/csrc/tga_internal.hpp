@@ -130,6 +130,12 @@ constexpr int kSliceBits = 10;
 uint32_t slice_set_slots(uint64_t n);
 hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, void *set, uint32_t slots, uint64_t *out,
                                  unsigned grid, hipStream_t st);
+// The same count over the records (rd words, key first) of partition 0 of a
+// bucket-layout copy of fanout <= 2^kSliceBits (list entries [pstart[0],
+// pstart[1]) of `list`): the slice lies inside that partition.
+hipError_t launch_slice_records(int key_bytes, const void *rec, uint32_t rd, const uint64_t *list,
+                                const uint64_t *pstart, void *set, uint32_t slots, uint64_t *out, unsigned grid,
+                                hipStream_t st);
 hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const OutDesc &o, uint64_t *desc,
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 // out[0..3] = {distinct keys, keys seen once, largest count of one key,

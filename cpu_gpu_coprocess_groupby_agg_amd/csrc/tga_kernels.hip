@@ -1486,6 +1486,74 @@ __device__ __forceinline__ uint32_t slice_set_insert(K *set, uint32_t slots, K k
   return 0;
 }
 
+// A slice key seen by a workgroup: deduplicated in its LDS set while that is
+// under half full, else inserted into the global set (new keys counted into
+// *added).  Keys outside the slice and the empty marker are skipped.
+template <typename K>
+__device__ __forceinline__ void slice_visit(K k, K *lset, uint32_t *s_fill, K *set, uint32_t slots,
+                                            unsigned long long *out, uint32_t &added) {
+  if (key_partition((uint64_t)k, kSliceBits) != 0 || k == KeyTraits<K>::kEmpty) return;
+  if (*(volatile uint32_t *)s_fill < kSliceLds / 2) {
+    uint32_t s = key_slot_hash((uint64_t)k) & (kSliceLds - 1);
+    for (int i = 0; i < 64; ++i) {
+      const K cur = lset[s];
+      if (cur == k) return;
+      if (cur == KeyTraits<K>::kEmpty) {
+        const K prev = atomicCAS(&lset[s], KeyTraits<K>::kEmpty, k);
+        if (prev == KeyTraits<K>::kEmpty) atomicAdd(s_fill, 1u);
+        if (prev == KeyTraits<K>::kEmpty || prev == k) return;
+      }
+      s = (s + 1) & (kSliceLds - 1);
+    }
+  }
+  added += slice_set_insert<K>(set, slots, k, out);
+}
+
+// The workgroup's LDS set into the global set, and its count of new keys
+// into out[0] (one atomic per wave).  After a barrier; every thread calls it.
+template <typename K>
+__device__ __forceinline__ void slice_finish(const K *lset, K *set, uint32_t slots, unsigned long long *out,
+                                             uint32_t added) {
+  for (uint32_t i = threadIdx.x; i < kSliceLds; i += blockDim.x) {
+    const K k = lset[i];
+    if (k != KeyTraits<K>::kEmpty) added += slice_set_insert<K>(set, slots, k, out);
+  }
+  const int lane = (int)__lane_id();
+  for (int o = kWave / 2; o > 0; o >>= 1) added += (uint32_t)__shfl((int)added, lane ^ o, kWave);
+  if (lane == 0 && added) atomicAdd(&out[0], (unsigned long long)added);
+}
+
+// The same count over a partitioned copy of records (rd words each, the key
+// first) in bucket layout, whose fanout is at most 2^kSliceBits: the slice
+// lies inside partition 0, so only partition 0's buckets (list entries
+// [pstart[0], pstart[1]), rows << 32 | bucket id) are read.  Each workgroup
+// takes four buckets at a time, one row a thread.
+template <typename K>
+__global__ void __launch_bounds__(1024) k_slice_records(const uint32_t *__restrict__ rec, uint32_t rd,
+                                                        const uint64_t *__restrict__ list,
+                                                        const uint64_t *__restrict__ pstart, K *set, uint32_t slots,
+                                                        unsigned long long *out) {
+  __shared__ K lset[kSliceLds];
+  __shared__ uint32_t s_fill;
+  for (uint32_t i = threadIdx.x; i < kSliceLds; i += blockDim.x) lset[i] = KeyTraits<K>::kEmpty;
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  uint32_t added = 0;
+  const uint64_t e0 = pstart[0], ne = pstart[1] - e0;
+  const uint32_t sub = threadIdx.x >> kBucketShift, r = threadIdx.x & (kBucketRows - 1);
+  const uint32_t per = blockDim.x >> kBucketShift;
+  for (uint64_t b = (uint64_t)blockIdx.x * per + sub; b < ne; b += (uint64_t)gridDim.x * per) {
+    const uint64_t ent = list[e0 + b];
+    if (r >= (uint32_t)(ent >> 32)) continue;
+    const uint32_t *w = rec + (((ent & 0xffffffffULL) << kBucketShift) + r) * rd;
+    K k = (K)w[0];
+    if constexpr (sizeof(K) == 8) k = (K)((uint64_t)w[1] << 32 | w[0]);
+    slice_visit<K>(k, lset, &s_fill, set, slots, out, added);
+  }
+  __syncthreads();
+  slice_finish<K>(lset, set, slots, out, added);
+}
+
 // VEC: keys 16-byte aligned, read 16 bytes a lane, 4 loads in flight.
 template <typename K, bool VEC>
 __global__ void __launch_bounds__(1024) k_slice_distinct(const K *__restrict__ keys, uint64_t n, K *set,
@@ -1496,23 +1564,7 @@ __global__ void __launch_bounds__(1024) k_slice_distinct(const K *__restrict__ k
   if (threadIdx.x == 0) s_fill = 0;
   __syncthreads();
   uint32_t added = 0;
-  auto visit = [&](K k) {
-    if (key_partition((uint64_t)k, kSliceBits) != 0 || k == KeyTraits<K>::kEmpty) return;
-    if (*(volatile uint32_t *)&s_fill < kSliceLds / 2) {
-      uint32_t s = key_slot_hash((uint64_t)k) & (kSliceLds - 1);
-      for (int i = 0; i < 64; ++i) {
-        const K cur = lset[s];
-        if (cur == k) return;
-        if (cur == KeyTraits<K>::kEmpty) {
-          const K prev = atomicCAS(&lset[s], KeyTraits<K>::kEmpty, k);
-          if (prev == KeyTraits<K>::kEmpty) atomicAdd(&s_fill, 1u);
-          if (prev == KeyTraits<K>::kEmpty || prev == k) return;
-        }
-        s = (s + 1) & (kSliceLds - 1);
-      }
-    }
-    added += slice_set_insert<K>(set, slots, k, out);
-  };
+  auto visit = [&](K k) { slice_visit<K>(k, lset, &s_fill, set, slots, out, added); };
   constexpr uint32_t E = VEC ? 16 / sizeof(K) : 1;  // keys per load
   constexpr int U = 4;
   const uint64_t nv = n / E;
@@ -1541,13 +1593,7 @@ __global__ void __launch_bounds__(1024) k_slice_distinct(const K *__restrict__ k
   if (VEC && blockIdx.x == 0)  // rows past the last whole vector
     for (uint64_t r = nv * E + threadIdx.x; r < n; r += blockDim.x) visit(keys[r]);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kSliceLds; i += blockDim.x) {
-    const K k = lset[i];
-    if (k != KeyTraits<K>::kEmpty) added += slice_set_insert<K>(set, slots, k, out);
-  }
-  const int lane = (int)__lane_id();
-  for (int o = kWave / 2; o > 0; o >>= 1) added += (uint32_t)__shfl((int)added, lane ^ o, kWave);
-  if (lane == 0 && added) atomicAdd(&out[0], (unsigned long long)added);
+  slice_finish<K>(lset, set, slots, out, added);
 }
 
 
@@ -2117,6 +2163,24 @@ hipError_t launch_slice_distinct(int key_bytes, const void *keys, uint64_t n, vo
     auto *fn = vec ? &k_slice_distinct<uint64_t, true> : &k_slice_distinct<uint64_t, false>;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(1024), 0, st, (const uint64_t *)keys, n, (uint64_t *)set, slots,
                        (unsigned long long *)out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_slice_records(int key_bytes, const void *rec, uint32_t rd, const uint64_t *list,
+                                const uint64_t *pstart, void *set, uint32_t slots, uint64_t *out, unsigned grid,
+                                hipStream_t st) {
+  if (slots == 0 || (slots & (slots - 1)) || grid == 0 || rd < (uint32_t)key_bytes / 4) return hipErrorInvalidValue;
+  hipError_t e = launch_fill_u64(out, 2, 0, st);
+  if (e != hipSuccess) return e;
+  if (key_bytes == 4) {
+    if ((e = launch_fill_u32((uint32_t *)set, slots, 0xFFFFFFFFu, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_slice_records<uint32_t>, dim3(grid), dim3(1024), 0, st, (const uint32_t *)rec, rd, list,
+                       pstart, (uint32_t *)set, slots, (unsigned long long *)out);
+  } else {
+    if ((e = launch_fill_u64((uint64_t *)set, slots, ~0ULL, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_slice_records<uint64_t>, dim3(grid), dim3(1024), 0, st, (const uint32_t *)rec, rd, list,
+                       pstart, (uint64_t *)set, slots, (unsigned long long *)out);
   }
   return hipGetLastError();
 }

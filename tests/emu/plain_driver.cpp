@@ -51,6 +51,7 @@ struct Case {
   std::vector<int> col_dt;
   std::vector<tga_agg> aggs;
   tga_options opt;
+  bool device = false;  // columns copied to device memory first (host columns of >= 2^22 rows are streamed)
 };
 
 std::vector<unsigned char> col_i32(int64_t n, int64_t lo, int64_t hi, uint64_t seed) {
@@ -114,12 +115,30 @@ int run_case(tga_context *ctx, const Case &c) {
     if (kb == 4) { const uint32_t v = (uint32_t)c.keys[i]; std::memcpy(&kbuf[i * 4], &v, 4); }
     else std::memcpy(&kbuf[i * 8], &c.keys[i], 8);
   }
-  const tga_column key{kbuf.data(), kb == 4 ? TGA_U32 : TGA_U64, TGA_HOST};
+  tga_column key{kbuf.data(), kb == 4 ? TGA_U32 : TGA_U64, TGA_HOST};
   std::vector<tga_column> vals;
   for (size_t j = 0; j < c.cols.size(); ++j) vals.push_back(tga_column{(void *)c.cols[j].data(), c.col_dt[j], TGA_HOST});
+  std::vector<void *> dev;
+  if (c.device) {
+    auto up = [&](tga_column &col, uint64_t bytes) {
+      void *d = nullptr;
+      if (tga_device_alloc(ctx, std::max<uint64_t>(bytes, 8), &d) || tga_memcpy_h2d(ctx, d, col.data, bytes)) return false;
+      dev.push_back(d);
+      col.data = d;
+      col.location = TGA_DEVICE;
+      return true;
+    };
+    bool ok = up(key, (uint64_t)n * kb);
+    for (size_t j = 0; j < vals.size() && ok; ++j) ok = up(vals[j], (uint64_t)n * dt_bytes(c.col_dt[j]));
+    if (!ok) {
+      std::printf("%s: device columns failed\n", c.name.c_str());
+      return 1;
+    }
+  }
   tga_result *r = nullptr;
   tga_status st = tga_groupby(ctx, &key, vals.data(), (int32_t)vals.size(), n, c.aggs.data(), (int32_t)c.aggs.size(),
                               &c.opt, &r);
+  for (void *d : dev) tga_device_free(ctx, d);
   if (st) {
     std::printf("%s: tga_groupby -> %d (%s)\n", c.name.c_str(), (int)st, tga_last_error(ctx));
     return 1;
@@ -184,9 +203,10 @@ int run_case(tga_context *ctx, const Case &c) {
   tga_options plan;
   tga_last_plan(ctx, &plan);
   std::printf("%s: %lld rows, %lld groups (path %d, 2^%d partitions, %d slots, tile %d, layout %d, wc %d, "
-              "attempts %d) %s\n",
+              "record words %d, attempts %d) %s\n",
               c.name.c_str(), (long long)n, (long long)g, plan.path, plan.partition_bits, plan.table_slots,
-              plan.tile_rows, plan.scatter_layout, plan.write_combine, plan.attempts, bad ? "FAIL" : "ok");
+              plan.tile_rows, plan.scatter_layout, plan.write_combine, plan.record_words, plan.attempts,
+              bad ? "FAIL" : "ok");
   return bad ? 1 : 0;
 }
 
@@ -726,6 +746,26 @@ int main(int argc, char **argv) {
     for (int64_t i = 0; i < n3; i += 2) k[i] = 987654321;
     cases.push_back({"recw_hot_key_auto", 64, k, {col_i32(n3, -(1LL << 31), (1LL << 31) - 1, 122), col_f64(n3, 123)},
                      {TGA_I32, TGA_F64}, af, opts(TGA_PATH_AUTO, 2)});
+  }
+  // No hint over 2^22 rows whose sample is mostly singletons: the plan is
+  // adaptive -- the first pass at the widest write-combining fanout, the
+  // hash-slice count over its partition 0, then either that pass as the copy
+  // (200K uniform keys; 80% of the rows on 20K keys and the rest nearly all
+  // distinct) or a second pass (3M keys: 2^11 partitions; 1M keys with an
+  // int32 and an fp64 column: 5-word records, first pass at 2^8)
+  {
+    const int64_t n4 = 1LL << 22;
+    cases.push_back({"adaptive_u64_200k", 64, keys_u64(n4, 200000, 131), {col_i32(n4, -(1LL << 31), (1LL << 31) - 1, 132)},
+                     {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_AUTO, 2), true});
+    std::vector<uint64_t> k = keys_u64(n4, 20000, 133);
+    for (int64_t i = 0; i < n4; i += 5) k[i] = tga_o_splitmix(134, (uint64_t)i) | 1;
+    cases.push_back({"adaptive_selfsim_u64", 64, k, {col_i32(n4, -99, 99, 135)}, {TGA_I32},
+                     {a4[0], a4[1]}, opts(TGA_PATH_AUTO, 2), true});
+    cases.push_back({"adaptive_u32_3m_two_pass", 32, keys_u32(n4, 3000000, 136), {col_i32(n4, -99, 99, 137)},
+                     {TGA_I32}, {a4[0], a4[1], a4[2], a4[3]}, opts(TGA_PATH_AUTO, 2), true});
+    cases.push_back({"adaptive_recw_u64_1m", 64, keys_u64(n4, 1000000, 138),
+                     {col_i32(n4, -(1LL << 31), (1LL << 31) - 1, 139), col_f64(n4, 140)}, {TGA_I32, TGA_F64},
+                     {{TGA_COUNT, 0}, {TGA_SUM, 0}, {TGA_SUM, 1}}, opts(TGA_PATH_AUTO, 2), true});
   }
   int bad = 0, ran = 0;
   for (const Case &c : cases) {

@@ -51,14 +51,20 @@ def test_slice_estimate_plans_in_one_attempt(ctx, dist):
     # 2^23 rows at 10% cardinality and no hint: the key sample is mostly
     # singletons, which leaves the group count open (under Zipf or
     # self-similar draws it reads ~3x low), so the estimate comes from an
-    # exact count over one 2^-10 slice of the key hash, and the first plan
-    # holds every partition
+    # exact count over one 2^-10 slice of the key hash -- taken over partition
+    # 0 of the first partitioning pass (the adaptive plan), not over all keys
+    # -- and the plan holds every partition in one attempt
     n = 1 << 23
     card = n // 10
     k = ctx.generate(tga.GEN_REF_U32, n, a=card, b=dist, x=0.9, seed=17, salt=3)
     v = ctx.generate(tga.GEN_VALS_I32, n, a=0, b=1 << 31, seed=18)
     aggs = ["count", ("sum", 0)]
+    ctx.profile(True)
+    ctx.profile_reset()
     got = ctx.groupby(k, [v], aggs)
+    prof = ctx.profile_entries()
+    ctx.profile(False)
+    assert "slice_records" in prof and "slice_distinct" not in prof, prof
     assert ctx.last_plan()["attempts"] == 1
     ref = O.groupby(k.to_numpy(), [v.to_numpy()], aggs)
     assert got["n_groups"] == ref["n_groups"] == card
