@@ -140,8 +140,9 @@ hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const
                           uint32_t *ticket, uint64_t *count_out, hipStream_t st, const uint64_t *base = nullptr);
 // out[0..3] = {distinct keys, keys seen once, largest count of one key,
 // overflow} of the m <= 16384 keys sampled at random rows (all rows when m >= n; gathered into
-// smp[m], then counted in an LDS hash table by one workgroup; overflow 1:
-// the counts are incomplete, smp still holds the sample)
+// smp[m], then counted in LDS hash tables by 16 workgroups, each taking a
+// sixteenth of the key hash; overflow 1: the counts are incomplete, smp
+// still holds the sample)
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st);
 // Copies n[i] 4-byte words from src[i] (i < count <= 8) one after another to
@@ -153,8 +154,10 @@ struct WordGather {
   uint32_t count;
 };
 hipError_t launch_gather_words(const WordGather &wg, uint32_t *dst, hipStream_t st);
+// m keys at random rows (all rows when m >= n) into out[m], widened to 64
+// bits; stats (optional): four counters zeroed for launch_sample_stats
 hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
-                              hipStream_t st);
+                              hipStream_t st, uint64_t *stats = nullptr);
 hipError_t launch_gen(int kind, void *out, uint64_t n, uint64_t a, uint64_t b, double x, double y, uint64_t seed,
                       uint64_t salt, hipStream_t st);
 

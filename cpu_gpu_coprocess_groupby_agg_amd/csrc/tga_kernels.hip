@@ -1439,24 +1439,28 @@ __global__ void k_gen_ref_u32(uint32_t *out, RefGenConst c) {
 }
 
 // key sample for the cardinality estimate
+// stats (optional): the four counters k_sample_stats accumulates into,
+// zeroed here.
 template <typename K>
-__global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *out) {
+__global__ void k_sample_keys(const K *keys, uint64_t n, uint64_t m, uint64_t *out, uint64_t *stats) {
+  if (stats && blockIdx.x == 0 && threadIdx.x < 4) stats[threadIdx.x] = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t row = sample_row(i, m, n);
     out[i] = (uint64_t)keys[row];
   }
 }
 
-// Statistics of a key sample for the cardinality estimate, in one
-// workgroup: of the m <= kSampleMax keys in `smp` (gathered by k_sample_keys
-// with the whole grid: one workgroup alone has too few loads in flight),
-// out = {distinct keys, keys seen once, largest count of one key, overflow}.
-// The keys are counted in an LDS hash table, one quarter of the key hash at
-// a time (so a table of kSampleSlots holds any quarter of a sample of
-// distinct keys with room to spare); the key equal to the empty marker is
-// counted on its own.  overflow = 1: some quarter did not fit (only keys
-// chosen against the hash can do that), the counts are incomplete.
-constexpr uint32_t kSampleMax = 16384, kSamplePer = kSampleMax / 1024, kSampleSlots = 8192;
+// Statistics of a key sample for the cardinality estimate: of the m <=
+// kSampleMax keys in `smp` (gathered by k_sample_keys), out = {distinct keys,
+// keys seen once, largest count of one key, overflow}, accumulated with
+// device atomics into counters k_sample_keys zeroed.  kSampleGroups
+// workgroups each count the keys whose hash has its top 4 bits equal to the
+// workgroup's index, in an LDS hash table (the sample is read whole by each:
+// 128 KB from the L2); the key equal to the empty marker is counted by
+// workgroup 0 on its own.  overflow = 1: some workgroup's keys did not fit
+// its table (only keys chosen against the hash can do that), the counts are
+// incomplete.
+constexpr uint32_t kSampleMax = 16384, kSampleGroups = 16, kSampleSlots = 4096;
 
 // Cardinality from a hash slice: the distinct keys whose partition hash
 // (key_partition's top kSliceBits bits) is 0, counted exactly over all n
@@ -1602,67 +1606,55 @@ __global__ void __launch_bounds__(1024) k_sample_stats(const uint64_t *__restric
   uint64_t *tk = (uint64_t *)smem;               // [kSampleSlots]
   uint32_t *tc = (uint32_t *)(tk + kSampleSlots);  // [kSampleSlots]
   __shared__ uint32_t s_d, s_f1, s_top, s_empty, s_over;
-  const uint32_t tid = threadIdx.x, nt = blockDim.x;  // nt = 1024
-  uint64_t v[kSamplePer];
-#pragma unroll
-  for (uint32_t r = 0; r < kSamplePer; ++r) {
-    const uint32_t i = tid + r * nt;
-    v[r] = i < m ? smp[i] : 0;
-  }
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, grp = blockIdx.x;
   for (uint32_t sl = tid; sl < kSampleSlots; sl += nt) {
     tk[sl] = ~0ULL;
     tc[sl] = 0;
   }
-  if (tid == 0) {
-    s_d = s_f1 = s_top = s_empty = s_over = 0;
-  }
+  if (tid == 0) s_d = s_f1 = s_top = s_empty = s_over = 0;
   __syncthreads();
   uint32_t empties = 0;
-#pragma unroll
-  for (uint32_t r = 0; r < kSamplePer; ++r) empties += tid + r * nt < m && v[r] == ~0ULL;
-  if (empties) atomicAdd(&s_empty, empties);
-  uint32_t d = 0, f1 = 0, top = 0;
-  for (uint32_t q = 0; q < 4; ++q) {
-#pragma unroll
-    for (uint32_t r = 0; r < kSamplePer; ++r) {
-      const uint64_t key = v[r];
-      const uint64_t h = key_hash(key);
-      if (tid + r * nt >= m || key == ~0ULL || (h >> 62) != q) continue;
-      uint32_t sl = (uint32_t)h & (kSampleSlots - 1);
-      bool done = false;
-      for (uint32_t p = 0; p < kSampleSlots && !done; ++p) {
-        const uint64_t prev = atomicCAS((unsigned long long *)&tk[sl], ~0ULL, (unsigned long long)key);
-        if (prev == ~0ULL || prev == key) {
-          atomicAdd(&tc[sl], 1u);
-          done = true;
-        } else {
-          sl = (sl + 1) & (kSampleSlots - 1);
-        }
+  for (uint32_t i = tid; i < m; i += nt) {
+    const uint64_t key = smp[i];
+    if (key == ~0ULL) {
+      empties += grp == 0;
+      continue;
+    }
+    const uint64_t h = key_hash(key);
+    if ((uint32_t)(h >> 60) != grp) continue;
+    uint32_t sl = (uint32_t)h & (kSampleSlots - 1);
+    bool done = false;
+    for (uint32_t p = 0; p < kSampleSlots && !done; ++p) {
+      const uint64_t prev = atomicCAS((unsigned long long *)&tk[sl], ~0ULL, (unsigned long long)key);
+      if (prev == ~0ULL || prev == key) {
+        atomicAdd(&tc[sl], 1u);
+        done = true;
+      } else {
+        sl = (sl + 1) & (kSampleSlots - 1);
       }
-      if (!done) s_over = 1;
     }
-    __syncthreads();
-    // this quarter's counts; each thread clears the slots it read
-    for (uint32_t sl = tid; sl < kSampleSlots; sl += nt) {
-      const uint32_t c = tc[sl];
-      d += c != 0;
-      f1 += c == 1;
-      top = max(top, c);
-      tk[sl] = ~0ULL;
-      tc[sl] = 0;
-    }
-    __syncthreads();
+    if (!done) s_over = 1;
   }
-  atomicAdd(&s_d, d);
-  atomicAdd(&s_f1, f1);
-  atomicMax(&s_top, top);
+  if (empties) atomicAdd(&s_empty, empties);
+  __syncthreads();
+  uint32_t d = 0, f1 = 0, top = 0;
+  for (uint32_t sl = tid; sl < kSampleSlots; sl += nt) {
+    const uint32_t c = tc[sl];
+    d += c != 0;
+    f1 += c == 1;
+    top = max(top, c);
+  }
+  if (d) atomicAdd(&s_d, d);
+  if (f1) atomicAdd(&s_f1, f1);
+  if (top) atomicMax(&s_top, top);
   __syncthreads();
   if (tid == 0) {
-    const uint32_t c = s_empty;
-    out[0] = s_d + (c != 0);
-    out[1] = s_f1 + (c == 1);
-    out[2] = max(s_top, c);
-    out[3] = s_over;
+    const uint32_t c = s_empty;  // nonzero in workgroup 0 only
+    const uint64_t dd = s_d + (c != 0), ff = s_f1 + (c == 1), tt = max(s_top, c);
+    if (dd) atomicAdd((unsigned long long *)&out[0], (unsigned long long)dd);
+    if (ff) atomicAdd((unsigned long long *)&out[1], (unsigned long long)ff);
+    if (tt) atomicMax((unsigned long long *)&out[2], (unsigned long long)tt);
+    if (s_over) atomicOr((unsigned long long *)&out[3], 1ULL);
   }
 }
 
@@ -2113,11 +2105,13 @@ hipError_t launch_compact(int key_bytes, const GTable &g, const Schema &s, const
 }
 
 hipError_t launch_sample_keys(int key_bytes, const void *keys, uint64_t n, uint64_t m, uint64_t *out,
-                              hipStream_t st) {
+                              hipStream_t st, uint64_t *stats) {
   if (key_bytes == 4)
-    hipLaunchKernelGGL(k_sample_keys<uint32_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint32_t *)keys, n, m, out);
+    hipLaunchKernelGGL(k_sample_keys<uint32_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint32_t *)keys, n, m,
+                       out, stats);
   else
-    hipLaunchKernelGGL(k_sample_keys<uint64_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint64_t *)keys, n, m, out);
+    hipLaunchKernelGGL(k_sample_keys<uint64_t>, dim3(grid_for(m, 256)), dim3(256), 0, st, (const uint64_t *)keys, n, m,
+                       out, stats);
   return hipGetLastError();
 }
 
@@ -2132,12 +2126,12 @@ hipError_t launch_gather_words(const WordGather &wg, uint32_t *dst, hipStream_t 
 hipError_t launch_sample_stats(int key_bytes, const void *keys, uint64_t n, uint32_t m, uint64_t *smp, uint64_t *out,
                                hipStream_t st) {
   if (m == 0 || m > kSampleMax) return hipErrorInvalidValue;
-  hipError_t e = launch_sample_keys(key_bytes, keys, n, m, smp, st);
+  hipError_t e = launch_sample_keys(key_bytes, keys, n, m, smp, st, out);
   if (e != hipSuccess) return e;
   const size_t lds = (size_t)kSampleSlots * 12;
   e = set_lds_limit<uint64_t>((const void *)&k_sample_stats, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sample_stats, dim3(1), dim3(1024), lds, st, (const uint64_t *)smp, m, out);
+  hipLaunchKernelGGL(k_sample_stats, dim3(kSampleGroups), dim3(1024), lds, st, (const uint64_t *)smp, m, out);
   return hipGetLastError();
 }
 
