@@ -753,6 +753,8 @@ __device__ __forceinline__ uint64_t load_col(const void *const *cols, uint32_t c
 template <int NC>
 __device__ __forceinline__ uint64_t pick_col(const uint64_t *raw, uint32_t c) {
   if constexpr (NC == 1) return raw[0];
+  else if constexpr (NC == 2) return c == 0 ? raw[0] : raw[1];
+  else if constexpr (NC == 3) return c == 0 ? raw[0] : c == 1 ? raw[1] : raw[2];
   else return c == 0 ? raw[0] : c == 1 ? raw[1] : c == 2 ? raw[2] : raw[3];
 }
 
