@@ -819,11 +819,29 @@ __global__ void __launch_bounds__(1024) k_part_scatter_wc(const K *__restrict__ 
       }
     }
   };
+  // Rank of each row inside its partition's run of the tile.  TWO: a work
+  // item of a hot first-pass partition sends most rows of a wave to one
+  // sub-partition, whose counter would take one atomic per lane (a 64-way
+  // same-address chain); the lanes sharing the first valid lane's
+  // sub-partition take their ranks from one atomic instead.
   auto wc_rank = [&](const K(&kk)[ITEMS], uint32_t vm, uint32_t *h, uint32_t(&pp)[ITEMS], uint32_t(&rk)[ITEMS]) {
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       pp[i] = key_partition((uint64_t)kk[i], (int)hbits) & (S.P - 1);
-      rk[i] = (vm >> i) & 1u ? atomicAdd(&h[pp[i]], 1u) : 0u;
+      const bool v = (vm >> i) & 1u;
+      if constexpr (TWO) {
+        const uint64_t act = __ballot(v);
+        const uint32_t lp = (uint32_t)__shfl((int)pp[i], act ? (int)__builtin_ctzll(act) : 0, kWave);
+        const bool same = v && pp[i] == lp;
+        const uint64_t sm = __ballot(same);
+        const uint64_t below = sm & lanemask_lt();
+        uint32_t base = 0;
+        if (same && below == 0) base = atomicAdd(&h[lp], (uint32_t)__builtin_popcountll(sm));
+        base = (uint32_t)__shfl((int)base, sm ? (int)__builtin_ctzll(sm) : 0, kWave);
+        rk[i] = same ? base + (uint32_t)__builtin_popcountll(below) : v ? atomicAdd(&h[pp[i]], 1u) : 0u;
+      } else {
+        rk[i] = v ? atomicAdd(&h[pp[i]], 1u) : 0u;
+      }
     }
   };
 
