@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-rate", type=float, default=20_000.0)
     ap.add_argument("--serve-sweep", type=lambda x: [float(v) for v in x.split(",")],
-                    default=[5_000.0, 10_000.0, 15_000.0, 20_000.0])
+                    default=[5_000.0, 10_000.0, 15_000.0, 17_500.0, 20_000.0])
     ap.add_argument("--only", choices=["all", "headline", "copy", "workloads", "config1", "config3", "batch", "serve",
                                        "stream", "coprocess"],
                     default="all",

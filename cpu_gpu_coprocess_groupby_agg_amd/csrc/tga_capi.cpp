@@ -514,9 +514,12 @@ tga_status estimate_groups(tga_context *ctx, const void *d_keys, int key_bytes, 
   if (!buf.alloc(ctx, m * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "sample buffer");
   uint64_t *smp = buf.as<uint64_t>() + 8;
   TGA_HIP(ctx, launch_sample_stats(key_bytes, d_keys, n, (uint32_t)m, smp, buf.as<uint64_t>(), ctx->stream));
+  // read back through the context's pinned status block (a pageable
+  // destination costs a staged copy on this wait)
   uint64_t h[4];
-  TGA_HIP(ctx, hipMemcpyAsync(h, buf.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, buf.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(h, ctx->status_host, sizeof h);
   bool reliable = true;
   uint64_t sample_d = h[0];  // distinct keys seen: a lower bound on the group count
   if (!h[3]) {
@@ -589,8 +592,9 @@ tga_status slice_count(tga_context *ctx, const void *d_keys, int key_bytes, uint
                                  ctx->stream);
   }));
   uint64_t c[2];
-  TGA_HIP(ctx, hipMemcpyAsync(c, buf.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+  TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, buf.p, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
   TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(c, ctx->status_host, sizeof c);
   *count = c[0];
   *overflow = c[1] != 0;
   return TGA_OK;
@@ -935,17 +939,27 @@ tga_status compact_into(tga_context *ctx, const QuerySpec &q, const GTable &g, c
                         uint64_t *hcount) {
   hipStream_t st = ctx->stream;
   const uint64_t ctiles = compact_tiles(g.S + 1);
-  DBuf cdesc;
-  if (!cdesc.alloc(ctx, (ctiles + 2) * 8 + 64)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "compaction");
+  DBuf cdesc;  // look-back descriptors | ticket | count | gathered status words
+  if (!cdesc.alloc(ctx, (ctiles + 2) * 8 + kStatusWords * 4)) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "compaction");
   uint32_t *cticket = (uint32_t *)(cdesc.as<uint64_t>() + ctiles);
   uint64_t *count = cdesc.as<uint64_t>() + ctiles + 1;
   TGA_HIP(ctx, launch_fill_u64(cdesc.as<uint64_t>(), ctiles + 2, 0, st));
   TGA_HIP(ctx, timed(ctx, "compact", [&] {
     return launch_compact(q.key_bytes, g, q.schema, od, cdesc.as<uint64_t>(), cticket, count, st);
   }));
-  TGA_HIP(ctx, hipMemcpyAsync(hflags, g.flags, sizeof(uint32_t) * kFlagWords, hipMemcpyDeviceToHost, st));
-  TGA_HIP(ctx, hipMemcpyAsync(hcount, count, 8, hipMemcpyDeviceToHost, st));
+  // flags and group count gathered into one pinned copy
+  WordGather wg{};
+  wg.src[0] = g.flags;
+  wg.n[0] = kFlagWords;
+  wg.src[1] = (const uint32_t *)count;
+  wg.n[1] = 2;
+  wg.count = 2;
+  uint32_t *dst = (uint32_t *)(cdesc.as<uint64_t>() + ctiles + 2);
+  TGA_HIP(ctx, launch_gather_words(wg, dst, st));
+  TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, dst, (kFlagWords + 2) * 4, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
+  std::memcpy(hflags, ctx->status_host, sizeof(uint32_t) * kFlagWords);
+  std::memcpy(hcount, ctx->status_host + kFlagWords, 8);
   return TGA_OK;
 }
 
@@ -995,8 +1009,9 @@ static tga_status adapt_first_pass(tga_context *ctx, const QuerySpec &q, const v
                                 pb.ap1.pstart, set.p, slots, cbuf.as<uint64_t>(), (unsigned)ctx->cus, st);
   }));
   uint64_t c[2];
-  TGA_HIP(ctx, hipMemcpyAsync(c, cbuf.p, sizeof c, hipMemcpyDeviceToHost, st));
+  TGA_HIP(ctx, hipMemcpyAsync(ctx->status_host, cbuf.p, sizeof c, hipMemcpyDeviceToHost, st));
   TGA_HIP(ctx, hipStreamSynchronize(st));
+  std::memcpy(c, ctx->status_host, sizeof c);
   const uint64_t est = combine_slice_estimate(plan.est, plan.sample_d, c[0], c[1] != 0, n);
   Plan np = autotune(ctx, q, n, est, &plan.opt);
   int bits = np.path == TGA_PATH_PARTITIONED ? np.pbits : b1;
