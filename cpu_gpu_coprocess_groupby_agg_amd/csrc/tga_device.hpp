@@ -1122,14 +1122,31 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
 #pragma unroll
             for (int u = 0; u < U; ++u) atomicAdd((unsigned long long *)arr + sl[u], (unsigned long long)pick_col<NC>(raw[u], col));
             break;
-          case ACC_MIN_I32:
+          // MIN / MAX: a slot's value only moves one way, so a row that does
+          // not improve the value read does not need its atomic (after k rows
+          // of a group, the next improves it with probability ~1/k)
+          case ACC_MIN_I32: {
+            int cur[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) atomicMin((int *)arr + sl[u], (int)(int32_t)pick_col<NC>(raw[u], col));
-            break;
-          case ACC_MAX_I32:
+            for (int u = 0; u < U; ++u) cur[u] = ((const int *)arr)[sl[u]];
 #pragma unroll
-            for (int u = 0; u < U; ++u) atomicMax((int *)arr + sl[u], (int)(int32_t)pick_col<NC>(raw[u], col));
+            for (int u = 0; u < U; ++u) {
+              const int v = (int)(int32_t)pick_col<NC>(raw[u], col);
+              if (v < cur[u]) atomicMin((int *)arr + sl[u], v);
+            }
             break;
+          }
+          case ACC_MAX_I32: {
+            int cur[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) cur[u] = ((const int *)arr)[sl[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const int v = (int)(int32_t)pick_col<NC>(raw[u], col);
+              if (v > cur[u]) atomicMax((int *)arr + sl[u], v);
+            }
+            break;
+          }
           case ACC_MIN_I64:
 #pragma unroll
             for (int u = 0; u < U; ++u)
