@@ -1147,16 +1147,28 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
             }
             break;
           }
-          case ACC_MIN_I64:
+          case ACC_MIN_I64: {
+            long long cur[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-              atomicMin((long long *)arr + sl[u], (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col)));
-            break;
-          case ACC_MAX_I64:
+            for (int u = 0; u < U; ++u) cur[u] = ((const long long *)arr)[sl[u]];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-              atomicMax((long long *)arr + sl[u], (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col)));
+            for (int u = 0; u < U; ++u) {
+              const long long v = (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col));
+              if (v < cur[u]) atomicMin((long long *)arr + sl[u], v);
+            }
             break;
+          }
+          case ACC_MAX_I64: {
+            long long cur[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) cur[u] = ((const long long *)arr)[sl[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              const long long v = (long long)prog_operand(kind, map, pick_col<NC>(raw[u], col));
+              if (v > cur[u]) atomicMax((long long *)arr + sl[u], v);
+            }
+            break;
+          }
           case ACC_SUM_F:
             if constexpr (FXA) {
               // limb l of slot s at l * (slots + 1) + s; rows without a slot
