@@ -56,7 +56,7 @@ struct PhaseClock {
     __syncthreads();
     if (threadIdx.x == 0) {
       const uint64_t n = batch_clock();
-      atomicAdd((unsigned long long *)&p[k], (unsigned long long)(n - t));
+      g_add((unsigned long long *)p + k, (unsigned long long)(n - t));
       t = n;
     }
   }
@@ -69,11 +69,11 @@ struct PhaseClock {
 // producer no cache-wide release (which writes back the whole L2 of its XCD).
 template <typename T>
 __device__ __forceinline__ void st_agent(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T ld_agent(const T *p) {
-  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gp(const_cast<T *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void wait_done(const BQuery &Q, uint32_t type) {
@@ -82,32 +82,45 @@ __device__ __forceinline__ void wait_done(const BQuery &Q, uint32_t type) {
   while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < Q.need[type]) __builtin_amdgcn_s_sleep(2);
 }
 
+// Item functions read their query record through a constant-memory pointer
+// (the record is not written while the kernel runs), made wave-uniform: its
+// fields are loaded once, by scalar loads, never through generic
+// instructions.
+__device__ __forceinline__ const BQuery &query_of(const TGA_CAS BQuery *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return *(const BQuery *)(const TGA_CAS BQuery *)(((uint64_t)hi << 32) | lo);
+}
+
 template <typename K>
-__device__ __noinline__ void item_init(const BQuery &Q, uint64_t a, uint64_t b) {
+__device__ __noinline__ void item_init(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b) {
+  const BQuery &Q = query_of(Qp);
   const GTable &g = Q.g;
   const uint64_t n = g.S + 1;
   K *keys = (K *)g.keys;
-  for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) keys[i] = KeyTraits<K>::kEmpty;
+  for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) gp(keys)[i] = KeyTraits<K>::kEmpty;
   #pragma unroll
   for (int c = 0; c < kMaxAccs; ++c) {
     if (c >= Q.s.nacc) break;
     const int kind = Q.s.acc[c].kind;
     if (kind == ACC_SUM_F) {
       for (int l = 0; l < kLimbs; ++l)
-        for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint64_t *)g.acc[c])[(uint64_t)l * n + i] = 0;
+        for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) gp((uint64_t *)g.acc[c])[(uint64_t)l * n + i] = 0;
     } else if (acc_global_bytes(kind) == 4) {
       const uint32_t id = (uint32_t)acc_identity(kind);
-      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint32_t *)g.acc[c])[i] = id;
+      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) gp((uint32_t *)g.acc[c])[i] = id;
     } else {
       const uint64_t id = acc_identity(kind);
-      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) ((uint64_t *)g.acc[c])[i] = id;
+      for (uint64_t i = a + threadIdx.x; i < b; i += blockDim.x) gp((uint64_t *)g.acc[c])[i] = id;
     }
   }
 }
 
 // Partition histogram of one tile; each thread has 16 key loads in flight.
 template <typename K>
-__device__ __noinline__ void item_count(const BQuery &Q, uint32_t tile, uint32_t *hist) {
+__device__ __noinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t tile, uint32_t *hist) {
+  const BQuery &Q = query_of(Qp);
   constexpr int R = 16;
   const uint32_t P = 1u << Q.pbits;
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
@@ -119,7 +132,7 @@ __device__ __noinline__ void item_count(const BQuery &Q, uint32_t tile, uint32_t
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       const uint64_t r = r0 + (uint64_t)i * blockDim.x + threadIdx.x;
-      kk[i] = r < e ? keys[r] : KeyTraits<K>::kEmpty;
+      kk[i] = r < e ? gp(keys)[r] : KeyTraits<K>::kEmpty;
     }
 #pragma unroll
     for (int i = 0; i < R; ++i)
@@ -127,29 +140,30 @@ __device__ __noinline__ void item_count(const BQuery &Q, uint32_t tile, uint32_t
         atomicAdd(&hist[hash_partition(key_hash((uint64_t)kk[i]), Q.pbits)], 1u);
   }
   __syncthreads();
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) st_agent(&Q.cnt[(uint64_t)p * Q.ntiles + tile], hist[p]);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) st_agent(Q.cnt + (uint64_t)p * Q.ntiles + tile, hist[p]);
 }
 
 // One workgroup scans the whole [P][ntiles] histogram of a query.
-__device__ __noinline__ void item_scan(const BQuery &Q, uint32_t *buf, uint32_t *scratch) {
+__device__ __noinline__ void item_scan(const TGA_CAS BQuery *Qp, uint32_t *buf, uint32_t *scratch) {
+  const BQuery &Q = query_of(Qp);
   const uint32_t P = 1u << Q.pbits;
   const uint64_t cells = (uint64_t)P * Q.ntiles;
   const uint32_t chunk = blockDim.x * 8;
   uint32_t run = 0;
   for (uint64_t base = 0; base < cells; base += chunk) {
     const uint32_t m = (uint32_t)min<uint64_t>(chunk, cells - base);
-    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = ld_agent(&Q.cnt[base + i]);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = ld_agent(Q.cnt + base + i);
     __syncthreads();
     const uint32_t tot = block_exclusive_scan_lds(buf, m, scratch);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
       const uint64_t idx = base + i;
-      st_agent(&Q.offs[idx], run + buf[i]);
-      if (idx % Q.ntiles == 0) st_agent(&Q.pstart[idx / Q.ntiles], (uint64_t)(run + buf[i]));  // first tile of p
+      st_agent(Q.offs + idx, run + buf[i]);
+      if (idx % Q.ntiles == 0) st_agent(Q.pstart + idx / Q.ntiles, (uint64_t)(run + buf[i]));  // first tile of p
     }
     run += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) st_agent(&Q.pstart[P], Q.n);
+  if (threadIdx.x == 0) st_agent(Q.pstart + P, Q.n);
 }
 
 // Rows [b, b + kSubRows) of a tile into registers (past e: empty / 0).
@@ -160,8 +174,8 @@ __device__ __forceinline__ void load_sub(const K *keys, const uint32_t *vals, ui
   for (int i = 0; i < kScatterItems; ++i) {
     const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
     const bool act = row < e;
-    kk[i] = act ? keys[row] : KeyTraits<K>::kEmpty;
-    vv[i] = act ? vals[row] : 0u;
+    kk[i] = act ? gp(keys)[row] : KeyTraits<K>::kEmpty;
+    vv[i] = act ? gp(vals)[row] : 0u;
   }
 }
 
@@ -171,8 +185,9 @@ __device__ __forceinline__ void load_sub(const K *keys, const uint32_t *vals, ui
 // the current one is sorted.  A destination outside [0, n) (offsets that do
 // not add up) is dropped and reported as an overflow, never written.
 template <typename K>
-__device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *smem, uint32_t *scratch,
+__device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t tile, char *smem, uint32_t *scratch,
                                           uint64_t *ph) {
+  const BQuery &Q = query_of(Qp);
   PhaseClock pc(ph);
   constexpr int IT = kScatterItems;
   const uint32_t P = 1u << Q.pbits;
@@ -186,7 +201,7 @@ __device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *
   K *pk = (K *)Q.pkeys;
   uint32_t *pv = (uint32_t *)Q.pcols[0];
   const uint64_t tb = (uint64_t)tile * Q.tile_rows, te = min(Q.n, tb + Q.tile_rows);
-  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = ld_agent(&Q.offs[(uint64_t)p * Q.ntiles + tile]);
+  for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = ld_agent(Q.offs + (uint64_t)p * Q.ntiles + tile);
   K kk[IT];
   uint32_t vv[IT];
   load_sub<K>(keys, vals, tb, te, kk, vv);
@@ -225,8 +240,8 @@ __device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *
       const uint32_t p = spid[j];
       const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
       if (dst < Q.n) {
-        st_agent(&pk[dst], skey[j]);
-        st_agent(&pv[dst], sval[j]);
+        st_agent(pk + dst, skey[j]);
+        st_agent(pv + dst, sval[j]);
       } else {
         bad = true;
       }
@@ -236,7 +251,7 @@ __device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *
     __syncthreads();
     pc.mark(2);
   }
-  if (bad) atomicOr(&Q.g.flags[FLAG_OVERFLOW], 1u);
+  if (bad) g_or(Q.g.flags + FLAG_OVERFLOW, 1u);
 }
 
 // DIRECT items (a row range) flush their table into the query's global
@@ -245,10 +260,15 @@ __device__ __noinline__ void item_scatter(const BQuery &Q, uint32_t tile, char *
 // small enough for one item -- emit their table straight into the query's
 // output, unless a row had to spill to the global table; then the table is
 // flushed there as well.
+// part: the rows are the query's partitioned copy (a PAGG item), else its
+// input columns.
 template <typename K, bool FX, int NC>
-__device__ __noinline__ void item_agg(const BQuery &Q, const void *keys, const Schema &s, uint64_t b, uint64_t e,
-                                      char *smem, uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch,
+__device__ __noinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, uint64_t b, uint64_t e, char *smem,
+                                      uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch,
                                       uint64_t *s_base, uint64_t *ph) {
+  const BQuery &Q = query_of(Qp);
+  const Schema &s = Q.s;
+  const void *keys = part ? Q.pkeys : Q.keys;
   PhaseClock pc(ph);
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
@@ -258,13 +278,13 @@ __device__ __noinline__ void item_agg(const BQuery &Q, const void *keys, const S
   if constexpr (FX) {
     const void *cols[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
+    for (int c = 0; c < NC; ++c) cols[c] = part ? Q.pcols[c] : s.cols[c];
     agg_rows_int<K, NC, 0, false, 0, true>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
                                            nullptr, spilled);
   } else {
     const void *cols[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) cols[c] = s.cols[c];
+    for (int c = 0; c < NC; ++c) cols[c] = part ? Q.pcols[c] : s.cols[c];
     if constexpr (NC == 1) {
       if (dtype_bytes((int)(Q.ap.coldt & 15u)) == 8)
         agg_rows_int<K, 1, 8, false>((const K *)keys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
@@ -287,8 +307,9 @@ __device__ __noinline__ void item_agg(const BQuery &Q, const void *keys, const S
 // Compacts table slots [a, b) of one query; output rows are reserved with an
 // atomic on the query's counter (order inside a query is unspecified).
 template <typename K>
-__device__ __noinline__ void item_compact(const BQuery &Q, uint64_t a, uint64_t b, uint32_t *scratch,
+__device__ __noinline__ void item_compact(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b, uint32_t *scratch,
                                           uint64_t *s_base) {
+  const BQuery &Q = query_of(Qp);
   const GTable &g = Q.g;
   const uint64_t n = g.S + 1;
   const K *keys = (const K *)g.keys;
@@ -296,8 +317,8 @@ __device__ __noinline__ void item_compact(const BQuery &Q, uint64_t a, uint64_t 
     const uint64_t slot = base + threadIdx.x;
     bool occ = false;
     if (slot < b) {
-      if (slot < g.S) occ = keys[slot] != KeyTraits<K>::kEmpty;
-      else occ = g.flags[FLAG_SENTINEL] != 0;
+      if (slot < g.S) occ = gp(keys)[slot] != KeyTraits<K>::kEmpty;
+      else occ = gp(g.flags)[FLAG_SENTINEL] != 0;
     }
     const uint64_t bal = __ballot(occ);
     const uint32_t pre = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
@@ -307,31 +328,31 @@ __device__ __noinline__ void item_compact(const BQuery &Q, uint64_t a, uint64_t 
     if (threadIdx.x == 0) {
       uint32_t run = 0;
       for (uint32_t w = 0; w < blockDim.x / kWave; ++w) { const uint32_t c = scratch[w]; scratch[w] = run; run += c; }
-      *s_base = run ? atomicAdd((unsigned long long *)Q.count_out, (unsigned long long)run) : 0;
+      *s_base = run ? g_add((unsigned long long *)Q.count_out, (unsigned long long)run) : 0;
     }
     __syncthreads();
     if (occ) {
       const uint64_t pos = *s_base + scratch[wave] + pre;
-      if (pos >= Q.od.capacity) atomicOr(&Q.g.flags[FLAG_OVERFLOW], 1u);
+      if (pos >= Q.od.capacity) g_or(Q.g.flags + FLAG_OVERFLOW, 1u);
       else {
-        ((K *)Q.od.keys)[pos] = slot < g.S ? keys[slot] : KeyTraits<K>::kEmpty;
+        gp((K *)Q.od.keys)[pos] = slot < g.S ? gp(keys)[slot] : KeyTraits<K>::kEmpty;
         #pragma unroll
   for (int c = 0; c < kMaxAccs; ++c) {
     if (c >= Q.s.nacc) break;
           const int kind = Q.s.acc[c].kind;
-          if (kind == ACC_COUNT || kind == ACC_SUM_I) ((uint64_t *)Q.od.cols[c])[pos] = ((const uint64_t *)g.acc[c])[slot];
-          else if (kind == ACC_MIN_I32 || kind == ACC_MAX_I32) ((uint32_t *)Q.od.cols[c])[pos] = ((const uint32_t *)g.acc[c])[slot];
+          if (kind == ACC_COUNT || kind == ACC_SUM_I) gp((uint64_t *)Q.od.cols[c])[pos] = gp((const uint64_t *)g.acc[c])[slot];
+          else if (kind == ACC_MIN_I32 || kind == ACC_MAX_I32) gp((uint32_t *)Q.od.cols[c])[pos] = gp((const uint32_t *)g.acc[c])[slot];
           else if (kind == ACC_SUM_F) {
             int64_t Lm[kLimbs];
-            for (int l = 0; l < kLimbs; ++l) Lm[l] = ((const int64_t *)g.acc[c])[(uint64_t)l * n + slot];
-            ((double *)Q.od.cols[c])[pos] = fx_to_double(Lm);
+            for (int l = 0; l < kLimbs; ++l) Lm[l] = gp((const int64_t *)g.acc[c])[(uint64_t)l * n + slot];
+            gp((double *)Q.od.cols[c])[pos] = fx_to_double(Lm);
           } else {
-            const int64_t v = ((const int64_t *)g.acc[c])[slot];
+            const int64_t v = gp((const int64_t *)g.acc[c])[slot];
             const int dt = Q.s.acc[c].col_dtype;
-            if (dt == DT_U64) ((uint64_t *)Q.od.cols[c])[pos] = (uint64_t)v ^ (1ULL << 63);
-            else if (dt == DT_F64) ((uint64_t *)Q.od.cols[c])[pos] = ordered_to_f64_bits(v);
-            else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)Q.od.cols[c])[pos] = (uint32_t)v;
-            else ((int64_t *)Q.od.cols[c])[pos] = v;
+            if (dt == DT_U64) gp((uint64_t *)Q.od.cols[c])[pos] = (uint64_t)v ^ (1ULL << 63);
+            else if (dt == DT_F64) gp((uint64_t *)Q.od.cols[c])[pos] = ordered_to_f64_bits(v);
+            else if (dt == DT_U32 || dt == DT_I32) gp((uint32_t *)Q.od.cols[c])[pos] = (uint32_t)v;
+            else gp((int64_t *)Q.od.cols[c])[pos] = v;
           }
         }
       }
@@ -341,29 +362,26 @@ __device__ __noinline__ void item_compact(const BQuery &Q, uint64_t a, uint64_t 
 }
 
 template <typename K, bool FX, int NC>
-__device__ void run_item(const BQuery &Q, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
+__device__ void run_item(const TGA_CAS BQuery *Qp, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
                          uint64_t *s_base, uint32_t *spilled, uint64_t *ph) {
+  const BQuery &Q = query_of(Qp);
   switch (it.type) {
-    case IT_INIT: item_init<K>(Q, it.a, it.b); break;
-    case IT_COUNT: item_count<K>(Q, (uint32_t)it.a, (uint32_t *)smem); break;
+    case IT_INIT: item_init<K>(Qp, it.a, it.b); break;
+    case IT_COUNT: item_count<K>(Qp, (uint32_t)it.a, (uint32_t *)smem); break;
     case IT_DIRECT:
-      item_agg<K, FX, NC>(Q, Q.keys, Q.s, it.a, it.b, smem, sentinel, it.a == 0 && it.b == Q.n, spilled, scratch,
-                          s_base, ph);
+      item_agg<K, FX, NC>(Qp, false, it.a, it.b, smem, sentinel, it.a == 0 && it.b == Q.n, spilled, scratch, s_base,
+                          ph);
       break;
-    case IT_SCAN: item_scan(Q, (uint32_t *)smem, scratch); break;
-    case IT_SCATTER: item_scatter<K>(Q, (uint32_t)it.a, smem, scratch, ph); break;
+    case IT_SCAN: item_scan(Qp, (uint32_t *)smem, scratch); break;
+    case IT_SCATTER: item_scatter<K>(Qp, (uint32_t)it.a, smem, scratch, ph); break;
     case IT_PAGG: {
-      Schema sp = Q.s;
-#pragma unroll
-      for (int c = 0; c < kMaxCols; ++c)
-        if (c < sp.ncols) sp.cols[c] = Q.pcols[c];
       // a range outside [0, n) (offsets that do not add up) is cut to it
-      const uint64_t e = min(ld_agent(&Q.pstart[it.a + 1]), Q.n);
-      const uint64_t b = min(ld_agent(&Q.pstart[it.a]), e);
-      item_agg<K, FX, NC>(Q, Q.pkeys, sp, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
+      const uint64_t e = min(ld_agent(Q.pstart + it.a + 1), Q.n);
+      const uint64_t b = min(ld_agent(Q.pstart + it.a), e);
+      item_agg<K, FX, NC>(Qp, true, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
       break;
     }
-    case IT_COMPACT: item_compact<K>(Q, it.a, it.b, scratch, s_base); break;
+    case IT_COMPACT: item_compact<K>(Qp, it.a, it.b, scratch, s_base); break;
     default: break;
   }
 }
@@ -417,8 +435,9 @@ __global__ void __launch_bounds__(kBatchThreads, 2 * kBatchThreads / kWave / 4) 
     if (it.type == IT_PAGG || it.type == IT_COMPACT) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const uint64_t c1f = prof && threadIdx.x == 0 ? batch_clock() : 0;
     uint64_t *ph = prof ? prof + 4 * kItemTypes : nullptr;
-    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
-    else run_item<uint64_t, FX, NC>(Q, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
+    const TGA_CAS BQuery *Qp = (const TGA_CAS BQuery *)(qs + it.q);
+    if (Q.key_bytes == 4) run_item<uint32_t, FX, NC>(Qp, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
+    else run_item<uint64_t, FX, NC>(Qp, it, smem, scratch, &s_sentinel, &s_base, &s_spilled, ph);
     // profiling: every wave's work ends before the release is timed
     if (prof) __syncthreads();
     const uint64_t c2w = prof && threadIdx.x == 0 ? batch_clock() : 0;

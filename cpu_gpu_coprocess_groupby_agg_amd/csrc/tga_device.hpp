@@ -223,6 +223,64 @@ __device__ __forceinline__ void lds_apply_limbs(const LdsTable<K> &t, int a, int
 }
 
 // ---------------------------------------------------------------------------
+// Global-memory accessors.  A pointer the compiler cannot place -- one read
+// from memory (a query record, a table descriptor) or passed into a function
+// it does not inline -- is accessed with generic ("flat") instructions, which
+// count against the LDS counter as well as the vector-memory one: a wait for
+// an LDS result then also waits for every global load in flight, and a
+// prefetch stops overlapping anything.  Through gp() and the g_* atomics
+// the accesses are global instructions (same relaxed, device-scope atomics
+// as atomicAdd and its kin).
+// ---------------------------------------------------------------------------
+#if defined(__AMDGCN__)
+#define TGA_GAS __attribute__((address_space(1)))
+// constant memory: data no kernel writes while it runs (query records)
+#define TGA_CAS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ TGA_GAS T *gp(T *p) {
+  return (TGA_GAS T *)p;
+}
+template <typename T, typename U>
+__device__ __forceinline__ T g_add(T *p, U v) {
+  return __hip_atomic_fetch_add(gp(p), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T, typename U>
+__device__ __forceinline__ T g_min(T *p, U v) {
+  return __hip_atomic_fetch_min(gp(p), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T, typename U>
+__device__ __forceinline__ T g_max(T *p, U v) {
+  return __hip_atomic_fetch_max(gp(p), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T, typename U>
+__device__ __forceinline__ T g_or(T *p, U v) {
+  return __hip_atomic_fetch_or(gp(p), (T)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// compare-and-swap; returns the word found
+template <typename T>
+__device__ __forceinline__ T g_cas(T *p, T cmp, T val) {
+  __hip_atomic_compare_exchange_strong(gp(p), &cmp, val, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return cmp;
+}
+#else  // host pass and host model of the kernels: plain pointers, the HIP atomics
+#define TGA_GAS
+#define TGA_CAS
+template <typename T>
+__device__ __forceinline__ T *gp(T *p) { return p; }
+template <typename T, typename U>
+__device__ __forceinline__ T g_add(T *p, U v) { return atomicAdd(p, (T)v); }
+template <typename T, typename U>
+__device__ __forceinline__ T g_min(T *p, U v) { return atomicMin(p, (T)v); }
+template <typename T, typename U>
+__device__ __forceinline__ T g_max(T *p, U v) { return atomicMax(p, (T)v); }
+template <typename T, typename U>
+__device__ __forceinline__ T g_or(T *p, U v) { return atomicOr(p, (T)v); }
+template <typename T>
+__device__ __forceinline__ T g_cas(T *p, T cmp, T val) { return atomicCAS(p, cmp, val); }
+#endif
+
+// ---------------------------------------------------------------------------
 // global table in HBM: the same SoA shape with `S` regular slots
 // ---------------------------------------------------------------------------
 
@@ -238,7 +296,7 @@ __device__ __forceinline__ void lds_apply_limbs(const LdsTable<K> &t, int a, int
 template <typename K>
 __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint64_t h) {
   if (key == KeyTraits<K>::kEmpty) {
-    atomicOr(&g.flags[FLAG_SENTINEL], 1u);
+    g_or(g.flags + FLAG_SENTINEL, 1u);
     return (int64_t)g.S;
   }
   K *keys = (K *)g.keys;
@@ -246,26 +304,26 @@ __device__ __forceinline__ int64_t g_find_or_insert(const GTable &g, K key, uint
   uint64_t s = hash_global_slot(h, S);
   const uint64_t limit = S < kMaxGlobalProbe ? S : kMaxGlobalProbe;
   for (uint64_t i = 0; i < limit; ++i) {
-    if ((i & 31) == 31 && *(volatile uint32_t *)&g.flags[FLAG_OVERFLOW]) return -1;
-    const K cur = keys[s];
+    if ((i & 31) == 31 && *(volatile TGA_GAS uint32_t *)gp(g.flags + FLAG_OVERFLOW)) return -1;
+    const K cur = gp(keys)[s];
     if (cur == key) return (int64_t)s;
     if (cur == KeyTraits<K>::kEmpty) {
-      const K prev = atomicCAS(&keys[s], KeyTraits<K>::kEmpty, key);
+      const K prev = g_cas(keys + s, KeyTraits<K>::kEmpty, key);
       if (prev == KeyTraits<K>::kEmpty || prev == key) return (int64_t)s;
     }
     s = (s + 1 == S) ? 0 : s + 1;
   }
-  atomicOr(&g.flags[FLAG_OVERFLOW], 1u);
+  g_or(g.flags + FLAG_OVERFLOW, 1u);
   return -1;
 }
 
 __device__ __forceinline__ void g_apply(const GTable &g, int a, int kind, int64_t s, uint64_t v) {
   switch (kind) {
-    case ACC_COUNT: case ACC_SUM_I: atomicAdd((unsigned long long *)g.acc[a] + s, (unsigned long long)v); break;
-    case ACC_MIN_I32: atomicMin((int *)g.acc[a] + s, (int)(int32_t)v); break;
-    case ACC_MAX_I32: atomicMax((int *)g.acc[a] + s, (int)(int32_t)v); break;
-    case ACC_MIN_I64: atomicMin((long long *)g.acc[a] + s, (long long)v); break;
-    case ACC_MAX_I64: atomicMax((long long *)g.acc[a] + s, (long long)v); break;
+    case ACC_COUNT: case ACC_SUM_I: g_add((unsigned long long *)g.acc[a] + s, (unsigned long long)v); break;
+    case ACC_MIN_I32: g_min((int *)g.acc[a] + s, (int)(int32_t)v); break;
+    case ACC_MAX_I32: g_max((int *)g.acc[a] + s, (int)(int32_t)v); break;
+    case ACC_MIN_I64: g_min((long long *)g.acc[a] + s, (long long)v); break;
+    case ACC_MAX_I64: g_max((long long *)g.acc[a] + s, (long long)v); break;
     default: break;
   }
 }
@@ -275,7 +333,7 @@ __device__ __forceinline__ void g_apply_limbs(const GTable &g, int a, int64_t s,
   const uint64_t n = g.S + 1;
 #pragma unroll
   for (int l = 0; l < kLimbs; ++l)
-    if (lv[l]) atomicAdd(p + (uint64_t)l * n + s, (unsigned long long)lv[l]);
+    if (lv[l]) g_add(p + (uint64_t)l * n + s, (unsigned long long)lv[l]);
 }
 
 // Expand a double into a limb vector (zeros except at most three limbs).
@@ -285,7 +343,7 @@ __device__ __forceinline__ void fx_expand(uint64_t bits, int64_t *lv, uint32_t *
   int64_t c[3];
   int range = 0;
   const int i0 = fx_split(bits, c, &range);
-  if (range) atomicOr(&flags[FLAG_RANGE], 1u);
+  if (range) g_or(flags + FLAG_RANGE, 1u);
   if (i0 < 0) return;
 #pragma unroll
   for (int l = 0; l < kLimbs; ++l) {
@@ -739,14 +797,14 @@ template <int VW>
 __device__ __forceinline__ uint64_t load_col(const void *const *cols, uint32_t coldt, int c, uint64_t row) {
   const uint32_t dt = (coldt >> (4 * c)) & 15u;
   if constexpr (VW == 4) {
-    const uint32_t v = ((const uint32_t *)cols[c])[row];
+    const uint32_t v = gp((const uint32_t *)cols[c])[row];
     return dt == DT_I32 ? (uint64_t)(int64_t)(int32_t)v : (uint64_t)v;
   } else if constexpr (VW == 8) {
-    return ((const uint64_t *)cols[c])[row];
+    return gp((const uint64_t *)cols[c])[row];
   } else {
-    if (dt == DT_U32) return (uint64_t)((const uint32_t *)cols[c])[row];
-    if (dt == DT_I32) return (uint64_t)(int64_t)((const int32_t *)cols[c])[row];
-    return ((const uint64_t *)cols[c])[row];
+    if (dt == DT_U32) return (uint64_t)gp((const uint32_t *)cols[c])[row];
+    if (dt == DT_I32) return (uint64_t)(int64_t)gp((const int32_t *)cols[c])[row];
+    return gp((const uint64_t *)cols[c])[row];
   }
 }
 
@@ -844,10 +902,10 @@ __device__ __forceinline__ void load_block(const K *__restrict__ keys, const voi
       for (int d = 0; d < NC; ++d) raw[u][d] = d < VD ? v[d < VD ? d : 0] : 0u;
     } else if constexpr (VW == 4) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? ((const uint32_t *)cols[c])[r] : 0u;
-      kk[u] = keys[r];
+      for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? gp((const uint32_t *)cols[c])[r] : 0u;
+      kk[u] = gp(keys)[r];
     } else {
-      kk[u] = keys[r];
+      kk[u] = gp(keys)[r];
 #pragma unroll
       for (int c = 0; c < NC; ++c) raw[u][c] = (uint32_t)c < ncols ? load_col<VW>(cols, coldt, c, r) : 0;
     }
@@ -1180,7 +1238,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
                 int64_t c[3];
                 int range = 0;
                 const int i0 = fx_split(pick_col<NC>(raw[u], col), c, &range);
-                if (range) atomicOr(&gsp->flags[FLAG_RANGE], 1u);
+                if (range) g_or(gsp->flags + FLAG_RANGE, 1u);
                 if (i0 < 0) continue;
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
@@ -1254,11 +1312,11 @@ __device__ __forceinline__ void lds_flush(const LdsTable<K> &t, const Schema &s,
       key[r] = i < S ? keys[i] : kE;
       occ[r] = key[r] != kE;
       gs0[r] = occ[r] ? hash_global_slot(key_hash((uint64_t)key[r]), g.S) : 0;
-      cur[r] = occ[r] ? gk[gs0[r]] : kE;
+      cur[r] = occ[r] ? gp(gk)[gs0[r]] : kE;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (occ[r] && cur[r] == kE) cur[r] = atomicCAS(&gk[gs0[r]], kE, key[r]);
+      if (occ[r] && cur[r] == kE) cur[r] = g_cas(gk + gs0[r], kE, key[r]);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (!occ[r]) continue;
@@ -1282,30 +1340,30 @@ template <typename K>
 __device__ __forceinline__ void emit_row(const LdsTable<K> &t, const Schema &s, const OutDesc &od, uint32_t i, K key,
                                          uint64_t pos) {
   const uint32_t n = t.L.slots + 1;
-  ((K *)od.keys)[pos] = key;
+  gp((K *)od.keys)[pos] = key;
   #pragma unroll
   for (int a = 0; a < kMaxAccs; ++a) {
     if (a >= s.nacc) break;
     const int kind = s.acc[a].kind;
     switch (kind) {
-      case ACC_COUNT: ((uint64_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
-      case ACC_SUM_I: ((uint64_t *)od.cols[a])[pos] = t.template arr<uint64_t>(a)[i]; break;
-      case ACC_MIN_I32: case ACC_MAX_I32: ((uint32_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
+      case ACC_COUNT: gp((uint64_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
+      case ACC_SUM_I: gp((uint64_t *)od.cols[a])[pos] = t.template arr<uint64_t>(a)[i]; break;
+      case ACC_MIN_I32: case ACC_MAX_I32: gp((uint32_t *)od.cols[a])[pos] = t.template arr<uint32_t>(a)[i]; break;
       case ACC_SUM_F: {
         int64_t L[kLimbs];
         const int64_t *p = t.template arr<int64_t>(a);
 #pragma unroll
         for (int l = 0; l < kLimbs; ++l) L[l] = p[(size_t)l * n + i];
-        ((double *)od.cols[a])[pos] = fx_to_double(L);
+        gp((double *)od.cols[a])[pos] = fx_to_double(L);
         break;
       }
       default: {
         const int64_t v = t.template arr<int64_t>(a)[i];
         const int dt = s.acc[a].col_dtype;
-        if (dt == DT_U64) ((uint64_t *)od.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
-        else if (dt == DT_F64) ((uint64_t *)od.cols[a])[pos] = ordered_to_f64_bits(v);
-        else if (dt == DT_U32 || dt == DT_I32) ((uint32_t *)od.cols[a])[pos] = (uint32_t)v;
-        else ((int64_t *)od.cols[a])[pos] = v;
+        if (dt == DT_U64) gp((uint64_t *)od.cols[a])[pos] = (uint64_t)v ^ (1ULL << 63);
+        else if (dt == DT_F64) gp((uint64_t *)od.cols[a])[pos] = ordered_to_f64_bits(v);
+        else if (dt == DT_U32 || dt == DT_I32) gp((uint32_t *)od.cols[a])[pos] = (uint32_t)v;
+        else gp((int64_t *)od.cols[a])[pos] = v;
         break;
       }
     }
@@ -1336,8 +1394,8 @@ __device__ __forceinline__ void lds_emit(const LdsTable<K> &t, const Schema &s, 
   if (threadIdx.x == 0) {
     uint32_t run = 0;
     for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) { const uint32_t v = scratch[w]; scratch[w] = run; run += v; }
-    *s_base = run ? atomicAdd((unsigned long long *)count, (unsigned long long)run) : 0;
-    if (*s_base + run > od.capacity) atomicOr(&flags[FLAG_OVERFLOW], 1u);
+    *s_base = run ? g_add((unsigned long long *)count, (unsigned long long)run) : 0;
+    if (*s_base + run > od.capacity) g_or(flags + FLAG_OVERFLOW, 1u);
   }
   __syncthreads();
   uint64_t pos = *s_base + scratch[wave] + (x - local);
