@@ -53,7 +53,7 @@ struct PhaseClock {
   __device__ explicit PhaseClock(uint64_t *p_) : p(p_), t(p_ && threadIdx.x == 0 ? batch_clock() : 0) {}
   __device__ void mark(int k) {
     if (!p) return;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
       const uint64_t n = batch_clock();
       g_add((unsigned long long *)p + k, (unsigned long long)(n - t));
@@ -94,7 +94,7 @@ __device__ __forceinline__ const BQuery &query_of(const TGA_CAS BQuery *p) {
 }
 
 template <typename K>
-__device__ __noinline__ void item_init(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b) {
+__device__ __forceinline__ void item_init(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b) {
   const BQuery &Q = query_of(Qp);
   const GTable &g = Q.g;
   const uint64_t n = g.S + 1;
@@ -119,12 +119,12 @@ __device__ __noinline__ void item_init(const TGA_CAS BQuery *Qp, uint64_t a, uin
 
 // Partition histogram of one tile; each thread has 16 key loads in flight.
 template <typename K>
-__device__ __noinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t tile, uint32_t *hist) {
+__device__ __forceinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t tile, uint32_t *hist) {
   const BQuery &Q = query_of(Qp);
   constexpr int R = 16;
   const uint32_t P = 1u << Q.pbits;
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-  __syncthreads();
+  lds_barrier();
   const K *keys = (const K *)Q.keys;
   const uint64_t b = (uint64_t)tile * Q.tile_rows, e = min(Q.n, b + Q.tile_rows);
   for (uint64_t r0 = b; r0 < e; r0 += (uint64_t)R * blockDim.x) {
@@ -139,12 +139,14 @@ __device__ __noinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t tile,
       if (r0 + (uint64_t)i * blockDim.x + threadIdx.x < e)
         atomicAdd(&hist[hash_partition(key_hash((uint64_t)kk[i]), Q.pbits)], 1u);
   }
-  __syncthreads();
+  lds_barrier();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) st_agent(Q.cnt + (uint64_t)p * Q.ntiles + tile, hist[p]);
 }
 
-// One workgroup scans the whole [P][ntiles] histogram of a query.
-__device__ __noinline__ void item_scan(const TGA_CAS BQuery *Qp, uint32_t *buf, uint32_t *scratch) {
+// One workgroup scans the whole [P][ntiles] histogram of a query.  (Barriers
+// inside items order LDS accesses only: global loads and the agent-coherent
+// stores stay in flight across them, and the item's end waits for all.)
+__device__ __forceinline__ void item_scan(const TGA_CAS BQuery *Qp, uint32_t *buf, uint32_t *scratch) {
   const BQuery &Q = query_of(Qp);
   const uint32_t P = 1u << Q.pbits;
   const uint64_t cells = (uint64_t)P * Q.ntiles;
@@ -153,15 +155,15 @@ __device__ __noinline__ void item_scan(const TGA_CAS BQuery *Qp, uint32_t *buf, 
   for (uint64_t base = 0; base < cells; base += chunk) {
     const uint32_t m = (uint32_t)min<uint64_t>(chunk, cells - base);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) buf[i] = ld_agent(Q.cnt + base + i);
-    __syncthreads();
-    const uint32_t tot = block_exclusive_scan_lds(buf, m, scratch);
+    lds_barrier();
+    const uint32_t tot = block_exclusive_scan_lds<LdsBarrier>(buf, m, scratch);
     for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
       const uint64_t idx = base + i;
       st_agent(Q.offs + idx, run + buf[i]);
       if (idx % Q.ntiles == 0) st_agent(Q.pstart + idx / Q.ntiles, (uint64_t)(run + buf[i]));  // first tile of p
     }
     run += tot;
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x == 0) st_agent(Q.pstart + P, Q.n);
 }
@@ -185,7 +187,7 @@ __device__ __forceinline__ void load_sub(const K *keys, const uint32_t *vals, ui
 // the current one is sorted.  A destination outside [0, n) (offsets that do
 // not add up) is dropped and reported as an overflow, never written.
 template <typename K>
-__device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t tile, char *smem, uint32_t *scratch,
+__device__ __forceinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t tile, char *smem, uint32_t *scratch,
                                           uint64_t *ph) {
   const BQuery &Q = query_of(Qp);
   PhaseClock pc(ph);
@@ -208,7 +210,7 @@ __device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t til
   bool bad = false;
   for (uint64_t sb = tb; sb < te; sb += kSubRows) {
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) hist[p] = 0;
-    __syncthreads();
+    lds_barrier();
     K ck[IT];
     uint32_t cv[IT], pp[IT], rk[IT];
 #pragma unroll
@@ -220,9 +222,9 @@ __device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t til
       rk[i] = row < te ? atomicAdd(&hist[pp[i]], 1u) : 0u;
     }
     if (sb + kSubRows < te) load_sub<K>(keys, vals, sb + kSubRows, te, kk, vv);
-    __syncthreads();
+    lds_barrier();
     pc.mark(0);
-    block_exclusive_scan_lds(hist, P, scratch);
+    block_exclusive_scan_lds<LdsBarrier>(hist, P, scratch);
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
       const uint64_t row = sb + (uint64_t)i * blockDim.x + threadIdx.x;
@@ -233,7 +235,7 @@ __device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t til
         spid[lp] = (uint16_t)pp[i];
       }
     }
-    __syncthreads();
+    lds_barrier();
     pc.mark(1);
     const uint32_t rows = (uint32_t)min<uint64_t>(kSubRows, te - sb);
     for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
@@ -246,9 +248,9 @@ __device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t til
         bad = true;
       }
     }
-    __syncthreads();
+    lds_barrier();
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] += (p + 1 < P ? hist[p + 1] : rows) - hist[p];
-    __syncthreads();
+    lds_barrier();
     pc.mark(2);
   }
   if (bad) g_or(Q.g.flags + FLAG_OVERFLOW, 1u);
@@ -263,7 +265,7 @@ __device__ __noinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t til
 // part: the rows are the query's partitioned copy (a PAGG item), else its
 // input columns.
 template <typename K, bool FX, int NC>
-__device__ __noinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, uint64_t b, uint64_t e, char *smem,
+__device__ __forceinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, uint64_t b, uint64_t e, char *smem,
                                       uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch,
                                       uint64_t *s_base, uint64_t *ph) {
   const BQuery &Q = query_of(Qp);
@@ -273,7 +275,7 @@ __device__ __noinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, uint6
   LdsTable<K> t{smem, Q.L, sentinel};
   lds_init(t, s);
   if (threadIdx.x == 0) *spilled = FX ? 1u : 0u;  // fp64 limbs: always through the global table
-  __syncthreads();
+  lds_barrier();
   pc.mark(whole ? 6 : 3);
   if constexpr (FX) {
     const void *cols[NC];
@@ -307,7 +309,7 @@ __device__ __noinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, uint6
 // Compacts table slots [a, b) of one query; output rows are reserved with an
 // atomic on the query's counter (order inside a query is unspecified).
 template <typename K>
-__device__ __noinline__ void item_compact(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b, uint32_t *scratch,
+__device__ __forceinline__ void item_compact(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b, uint32_t *scratch,
                                           uint64_t *s_base) {
   const BQuery &Q = query_of(Qp);
   const GTable &g = Q.g;
@@ -324,13 +326,13 @@ __device__ __noinline__ void item_compact(const TGA_CAS BQuery *Qp, uint64_t a, 
     const uint32_t pre = (uint32_t)__builtin_popcountll(bal & lanemask_lt());
     const int wave = threadIdx.x >> 6;
     if (__lane_id() == 0) scratch[wave] = (uint32_t)__builtin_popcountll(bal);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
       uint32_t run = 0;
       for (uint32_t w = 0; w < blockDim.x / kWave; ++w) { const uint32_t c = scratch[w]; scratch[w] = run; run += c; }
       *s_base = run ? g_add((unsigned long long *)Q.count_out, (unsigned long long)run) : 0;
     }
-    __syncthreads();
+    lds_barrier();
     if (occ) {
       const uint64_t pos = *s_base + scratch[wave] + pre;
       if (pos >= Q.od.capacity) g_or(Q.g.flags + FLAG_OVERFLOW, 1u);
@@ -357,31 +359,83 @@ __device__ __noinline__ void item_compact(const TGA_CAS BQuery *Qp, uint64_t a, 
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
+// The same items as functions of their own, each with its own register
+// allocation: the kernel variants for fp64 SUM or several value columns call
+// these, since all items inlined into one kernel would spill in their row
+// loops; the one-column integer variant inlines every item (no call saves
+// and restores of registers per item).
+template <typename K>
+__device__ __noinline__ void item_init_fn(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b) {
+  item_init<K>(Qp, a, b);
+}
+template <typename K>
+__device__ __noinline__ void item_count_fn(const TGA_CAS BQuery *Qp, uint32_t tile, uint32_t *hist) {
+  item_count<K>(Qp, tile, hist);
+}
+__device__ __noinline__ void item_scan_fn(const TGA_CAS BQuery *Qp, uint32_t *buf, uint32_t *scratch) {
+  item_scan(Qp, buf, scratch);
+}
+template <typename K>
+__device__ __noinline__ void item_scatter_fn(const TGA_CAS BQuery *Qp, uint32_t tile, char *smem, uint32_t *scratch,
+                                             uint64_t *ph) {
+  item_scatter<K>(Qp, tile, smem, scratch, ph);
+}
 template <typename K, bool FX, int NC>
-__device__ void run_item(const TGA_CAS BQuery *Qp, const BItem &it, char *smem, uint32_t *scratch, uint32_t *sentinel,
-                         uint64_t *s_base, uint32_t *spilled, uint64_t *ph) {
+__device__ __noinline__ void item_agg_fn(const TGA_CAS BQuery *Qp, bool part, uint64_t b, uint64_t e, char *smem,
+                                         uint32_t *sentinel, bool whole, uint32_t *spilled, uint32_t *scratch,
+                                         uint64_t *s_base, uint64_t *ph) {
+  item_agg<K, FX, NC>(Qp, part, b, e, smem, sentinel, whole, spilled, scratch, s_base, ph);
+}
+template <typename K>
+__device__ __noinline__ void item_compact_fn(const TGA_CAS BQuery *Qp, uint64_t a, uint64_t b, uint32_t *scratch,
+                                             uint64_t *s_base) {
+  item_compact<K>(Qp, a, b, scratch, s_base);
+}
+
+template <typename K, bool FX, int NC>
+__device__ __forceinline__ void run_item(const TGA_CAS BQuery *Qp, const BItem &it, char *smem, uint32_t *scratch,
+                                         uint32_t *sentinel, uint64_t *s_base, uint32_t *spilled, uint64_t *ph) {
+  constexpr bool kInl = !FX && NC == 1;
   const BQuery &Q = query_of(Qp);
   switch (it.type) {
-    case IT_INIT: item_init<K>(Qp, it.a, it.b); break;
-    case IT_COUNT: item_count<K>(Qp, (uint32_t)it.a, (uint32_t *)smem); break;
-    case IT_DIRECT:
-      item_agg<K, FX, NC>(Qp, false, it.a, it.b, smem, sentinel, it.a == 0 && it.b == Q.n, spilled, scratch, s_base,
-                          ph);
+    case IT_INIT:
+      if constexpr (kInl) item_init<K>(Qp, it.a, it.b);
+      else item_init_fn<K>(Qp, it.a, it.b);
       break;
-    case IT_SCAN: item_scan(Qp, (uint32_t *)smem, scratch); break;
-    case IT_SCATTER: item_scatter<K>(Qp, (uint32_t)it.a, smem, scratch, ph); break;
+    case IT_COUNT:
+      if constexpr (kInl) item_count<K>(Qp, (uint32_t)it.a, (uint32_t *)smem);
+      else item_count_fn<K>(Qp, (uint32_t)it.a, (uint32_t *)smem);
+      break;
+    case IT_DIRECT: {
+      const bool whole = it.a == 0 && it.b == Q.n;
+      if constexpr (kInl) item_agg<K, FX, NC>(Qp, false, it.a, it.b, smem, sentinel, whole, spilled, scratch, s_base, ph);
+      else item_agg_fn<K, FX, NC>(Qp, false, it.a, it.b, smem, sentinel, whole, spilled, scratch, s_base, ph);
+      break;
+    }
+    case IT_SCAN:
+      if constexpr (kInl) item_scan(Qp, (uint32_t *)smem, scratch);
+      else item_scan_fn(Qp, (uint32_t *)smem, scratch);
+      break;
+    case IT_SCATTER:
+      if constexpr (kInl) item_scatter<K>(Qp, (uint32_t)it.a, smem, scratch, ph);
+      else item_scatter_fn<K>(Qp, (uint32_t)it.a, smem, scratch, ph);
+      break;
     case IT_PAGG: {
       // a range outside [0, n) (offsets that do not add up) is cut to it
       const uint64_t e = min(ld_agent(Q.pstart + it.a + 1), Q.n);
       const uint64_t b = min(ld_agent(Q.pstart + it.a), e);
-      item_agg<K, FX, NC>(Qp, true, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
+      if constexpr (kInl) item_agg<K, FX, NC>(Qp, true, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
+      else item_agg_fn<K, FX, NC>(Qp, true, b, e, smem, sentinel, true, spilled, scratch, s_base, ph);
       break;
     }
-    case IT_COMPACT: item_compact<K>(Qp, it.a, it.b, scratch, s_base); break;
+    case IT_COMPACT:
+      if constexpr (kInl) item_compact<K>(Qp, it.a, it.b, scratch, s_base);
+      else item_compact_fn<K>(Qp, it.a, it.b, scratch, s_base);
+      break;
     default: break;
   }
 }
