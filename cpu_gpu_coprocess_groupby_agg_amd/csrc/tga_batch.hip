@@ -117,6 +117,13 @@ __device__ __forceinline__ void item_init(const TGA_CAS BQuery *Qp, uint64_t a, 
   }
 }
 
+// Partitions of a query: the top bits of the key's multiplicative hash, as in
+// the plain operator's partitioned path.
+template <typename K>
+__device__ __forceinline__ uint32_t batch_partition(K k, int bits) {
+  return key_partition((uint64_t)k, bits);
+}
+
 // Partition histogram of one tile; each thread has 16 key loads in flight.
 template <typename K>
 __device__ __forceinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t tile, uint32_t *hist) {
@@ -136,8 +143,7 @@ __device__ __forceinline__ void item_count(const TGA_CAS BQuery *Qp, uint32_t ti
     }
 #pragma unroll
     for (int i = 0; i < R; ++i)
-      if (r0 + (uint64_t)i * blockDim.x + threadIdx.x < e)
-        atomicAdd(&hist[hash_partition(key_hash((uint64_t)kk[i]), Q.pbits)], 1u);
+      if (r0 + (uint64_t)i * blockDim.x + threadIdx.x < e) atomicAdd(&hist[batch_partition(kk[i], Q.pbits)], 1u);
   }
   lds_barrier();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) st_agent(Q.cnt + (uint64_t)p * Q.ntiles + tile, hist[p]);
@@ -215,11 +221,11 @@ __device__ __forceinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t 
     uint32_t cv[IT], pp[IT], rk[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-      const uint64_t row = sb + (uint64_t)i * blockDim.x + threadIdx.x;
+      const bool v = sb + (uint64_t)i * blockDim.x + threadIdx.x < te;
       ck[i] = kk[i];
       cv[i] = vv[i];
-      pp[i] = hash_partition(key_hash((uint64_t)ck[i]), Q.pbits);
-      rk[i] = row < te ? atomicAdd(&hist[pp[i]], 1u) : 0u;
+      pp[i] = batch_partition(ck[i], Q.pbits);
+      rk[i] = v ? atomicAdd(&hist[pp[i]], 1u) : 0u;
     }
     if (sb + kSubRows < te) load_sub<K>(keys, vals, sb + kSubRows, te, kk, vv);
     lds_barrier();
