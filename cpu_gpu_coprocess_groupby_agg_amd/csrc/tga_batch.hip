@@ -38,7 +38,7 @@ constexpr int kPhases = 8;
 // staged through LDS on its own
 constexpr int kScatterItems = 8;
 constexpr uint32_t kSubRows = kBatchThreads * kScatterItems;
-constexpr uint32_t kSubTiles = 4;
+constexpr uint32_t kSubTiles = 16;
 constexpr uint32_t kTileRows = kSubRows * kSubTiles;
 
 // Device clock for the optional profile (100 MHz).
@@ -174,16 +174,17 @@ __device__ __forceinline__ void item_scan(const TGA_CAS BQuery *Qp, uint32_t *bu
   if (threadIdx.x == 0) st_agent(Q.pstart + P, Q.n);
 }
 
-// Rows [b, b + kSubRows) of a tile into registers (past e: empty / 0).
+// Rows [b, b + kSubRows) of a tile into registers (b < e).  Rows past e
+// re-read row e - 1: every lane issues the same loads, no branches, so the
+// compiler knows how many memory operations follow them (see item_scatter).
 template <typename K>
 __device__ __forceinline__ void load_sub(const K *keys, const uint32_t *vals, uint64_t b, uint64_t e, K *kk,
                                          uint32_t *vv) {
 #pragma unroll
   for (int i = 0; i < kScatterItems; ++i) {
-    const uint64_t row = b + (uint64_t)i * blockDim.x + threadIdx.x;
-    const bool act = row < e;
-    kk[i] = act ? gp(keys)[row] : KeyTraits<K>::kEmpty;
-    vv[i] = act ? gp(vals)[row] : 0u;
+    const uint64_t row = min(b + (uint64_t)i * kBatchThreads + threadIdx.x, e - 1);
+    kk[i] = gp(keys)[row];
+    vv[i] = gp(vals)[row];
   }
 }
 
@@ -243,16 +244,24 @@ __device__ __forceinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t 
     }
     lds_barrier();
     pc.mark(1);
+    // A fixed number of stores per lane, none skipped: loads and stores
+    // share one counter, so the wait for the next sub-tile's keys (issued
+    // before these stores) can leave exactly these stores in flight; with a
+    // data-dependent count the compiler would wait for every store to reach
+    // memory first.  Lanes past the sub-tile's rows store its last row again
+    // (same value, same place); a destination outside [0, n) (offsets that
+    // do not add up) is reported and goes to row 0 of this query's copy,
+    // whose result is then discarded and the query rerun alone.
     const uint32_t rows = (uint32_t)min<uint64_t>(kSubRows, te - sb);
-    for (uint32_t j = threadIdx.x; j < rows; j += blockDim.x) {
+#pragma unroll
+    for (uint32_t r = 0; r < kScatterItems; ++r) {
+      const uint32_t j = min(r * kBatchThreads + threadIdx.x, rows - 1);
       const uint32_t p = spid[j];
-      const uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
-      if (dst < Q.n) {
-        st_agent(pk + dst, skey[j]);
-        st_agent(pv + dst, sval[j]);
-      } else {
-        bad = true;
-      }
+      uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
+      bad |= dst >= Q.n;
+      dst = dst < Q.n ? dst : 0;
+      st_agent(pk + dst, skey[j]);
+      st_agent(pv + dst, sval[j]);
     }
     lds_barrier();
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] += (p + 1 < P ? hist[p + 1] : rows) - hist[p];
