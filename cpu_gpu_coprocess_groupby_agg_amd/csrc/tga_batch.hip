@@ -209,6 +209,7 @@ __device__ __forceinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t 
   const uint32_t *vals = (const uint32_t *)Q.s.cols[0];
   K *pk = (K *)Q.pkeys;
   uint32_t *pv = (uint32_t *)Q.pcols[0];
+  uint64_t *prec = (uint64_t *)Q.pkeys;  // 4-byte keys: (key, value) records
   const uint64_t tb = (uint64_t)tile * Q.tile_rows, te = min(Q.n, tb + Q.tile_rows);
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] = ld_agent(Q.offs + (uint64_t)p * Q.ntiles + tile);
   K kk[IT];
@@ -260,8 +261,12 @@ __device__ __forceinline__ void item_scatter(const TGA_CAS BQuery *Qp, uint32_t 
       uint64_t dst = (uint64_t)goff[p] + (j - hist[p]);
       bad |= dst >= Q.n;
       dst = dst < Q.n ? dst : 0;
-      st_agent(pk + dst, skey[j]);
-      st_agent(pv + dst, sval[j]);
+      if constexpr (sizeof(K) == 4) {
+        st_agent(prec + dst, (uint64_t)sval[j] << 32 | (uint64_t)(uint32_t)skey[j]);
+      } else {
+        st_agent(pk + dst, skey[j]);
+        st_agent(pv + dst, sval[j]);
+      }
     }
     lds_barrier();
     for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) goff[p] += (p + 1 < P ? hist[p + 1] : rows) - hist[p];
@@ -292,7 +297,12 @@ __device__ __forceinline__ void item_agg(const TGA_CAS BQuery *Qp, bool part, ui
   if (threadIdx.x == 0) *spilled = FX ? 1u : 0u;  // fp64 limbs: always through the global table
   lds_barrier();
   pc.mark(whole ? 6 : 3);
-  if constexpr (FX) {
+  if (sizeof(K) == 4 && part) {
+    // the partitioned copy of a 4-byte-key query: 8-byte (key, value) records
+    const void *cols[1] = {nullptr};
+    agg_rows_int<K, 1, kVwRec4, false>((const K *)Q.pkeys, cols, b, e, t, Q.ap, Q.nbits, Q.max_probe, &Q.g, &Q.s,
+                                       nullptr, spilled);
+  } else if constexpr (FX) {
     const void *cols[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) cols[c] = part ? Q.pcols[c] : s.cols[c];
@@ -745,20 +755,23 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
       Q.tile_rows = z.tile_rows;
       Q.ntiles = z.ntiles;
       const uint64_t cells = (uint64_t)PP * Q.ntiles;
+      // the partitioned copy: 4-byte keys as 8-byte (key, value) records in
+      // one array, 8-byte keys as a key array and a value array
+      const bool rec = q.key_bytes == 4;
       DBuf c1, c2, c3, c4, c5;
       if (!c1.alloc(ctx, cells * 4) || !c2.alloc(ctx, cells * 4) || !c3.alloc(ctx, (PP + 1) * 8) ||
-          !c4.alloc(ctx, Q.n * q.key_bytes) || !c5.alloc(ctx, Q.n * 4))
+          !c4.alloc(ctx, Q.n * (rec ? 8 : q.key_bytes)) || (!rec && !c5.alloc(ctx, Q.n * 4)))
         return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch partition buffers");
       Q.cnt = c1.as<uint32_t>();
       Q.offs = c2.as<uint32_t>();
       Q.pstart = c3.as<uint64_t>();
       Q.pkeys = c4.p;
-      Q.pcols[0] = c5.p;
+      Q.pcols[0] = rec ? nullptr : c5.p;
       b.bufs.push_back(std::move(c1));
       b.bufs.push_back(std::move(c2));
       b.bufs.push_back(std::move(c3));
       b.bufs.push_back(std::move(c4));
-      b.bufs.push_back(std::move(c5));
+      if (!rec) b.bufs.push_back(std::move(c5));
       lds_need = std::max(lds_need, scatter_lds(z.pbits, q.key_bytes));
       lds_need = std::max<size_t>(lds_need, (size_t)4 * std::min<uint64_t>(cells, kBatchThreads * 8));
       for (uint32_t t = 0; t < Q.ntiles; ++t) lists[IT_COUNT].push_back({(uint32_t)i, IT_COUNT, t, 0});

@@ -735,9 +735,9 @@ struct RecW {
   // record at byte offset `off` (32-bit) of `base`
   __device__ __forceinline__ static void load_at(const void *base, uint32_t off, K &key, uint32_t (&val)[VD]) {
     const char *p = (const char *)base + off;
-    if constexpr (RD == 2) {  // 8-byte aligned: one dwordx2
-      const uint2 v = *(const uint2 *)p;
-      const uint32_t w[2] = {v.x, v.y};
+    if constexpr (RD == 2) {  // 8-byte aligned: one dwordx2, a global access
+      const uint64_t v = *gp((const uint64_t *)p);
+      const uint32_t w[2] = {(uint32_t)v, (uint32_t)(v >> 32)};
       unpack(w, key, val);
     } else if constexpr (RD == 4) {  // 16-byte aligned: one dwordx4
       const uint4 v = *(const uint4 *)p;
