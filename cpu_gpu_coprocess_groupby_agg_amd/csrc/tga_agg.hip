@@ -121,8 +121,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
 template <typename K>
 static hipError_t set_lds_limit(const void *fn, size_t bytes) {
   (void)sizeof(K);
-  if (bytes <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return raise_lds_limit(fn, bytes);
 }
 
 template <typename K, bool FX, int NC, int VW, bool FOLD, uint32_t SIG = 0>

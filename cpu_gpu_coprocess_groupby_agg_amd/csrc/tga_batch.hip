@@ -806,7 +806,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
   b.kernel = fx ? (const void *)&k_batch<true, kMaxCols> : multi ? (const void *)&k_batch<false, kMaxCols>
                                                                  : (const void *)&k_batch<false, 1>;
   if (b.lds > 65536)
-    TGA_HIP(ctx, hipFuncSetAttribute(b.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b.lds));
+    TGA_HIP(ctx, raise_lds_limit(b.kernel, b.lds));
   const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(4, ctx->lds_max / b.lds));
   b.grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)ctx->cus * per_cu, b.nitems));
   return TGA_OK;

@@ -1052,9 +1052,17 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   if (!counters.alloc(ctx, 64) || !ovf.alloc(ctx, (uint64_t)(ovf_cap + 1) * 8))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   uint64_t *emit_count = counters.as<uint64_t>();
-  TGA_HIP(ctx, launch_fill_u64(emit_count, 2, 0, st));
   const bool may_abandon = plan.may_replan;
-  if (may_abandon) TGA_HIP(ctx, launch_fill_u64(ovf.as<uint64_t>(), 1, (uint64_t)ovf_cap << 32, st));
+  // both are first used by the aggregation: the global table's init launch
+  // clears them
+  WordInit wi{};
+  wi.p[0] = emit_count;
+  wi.n[0] = 2;
+  if (may_abandon) {
+    wi.p[1] = ovf.as<uint64_t>();
+    wi.v[1] = (uint64_t)ovf_cap << 32;
+    wi.n[1] = 1;
+  }
   // Global table: split partitions and LDS spills only.  A partition is
   // split past chunk_rows (twice its expected rows, so only a key holding
   // more than half a partition's share of the rows splits one), and it holds
@@ -1086,7 +1094,7 @@ static tga_status run_partitioned(tga_context *ctx, const QuerySpec &q, const vo
   if (gtable_alloc(ctx, q, S, g, gbufs, flags) != hipSuccess)
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "global table of %llu slots", (unsigned long long)S);
   g.S_dev = ap.gslots;
-  TGA_HIP(ctx, timed(ctx, "gtable_init", [&] { return launch_gtable_init(q.key_bytes, g, s, st); }));
+  TGA_HIP(ctx, timed(ctx, "gtable_init", [&] { return launch_gtable_init(q.key_bytes, g, s, st, wi); }));
   // outputs
   const uint64_t cap = ext_keys ? ext_cap : plan.gslots + 1;
   PlainOut tmp;

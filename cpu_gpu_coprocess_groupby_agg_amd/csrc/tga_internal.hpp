@@ -39,7 +39,19 @@ enum GenKind : int { GEN_KEYS_U32 = 1, GEN_KEYS_U64 = 2, GEN_VALS_I32 = 3, GEN_Z
 hipError_t launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t st);
-hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st);
+// Raises a kernel's dynamic-LDS limit to `bytes` (past 64 KB) on the current
+// device; the attribute is set once per kernel and size.
+hipError_t raise_lds_limit(const void *fn, size_t bytes);
+
+// Up to two counter arrays the table's init launch also sets: p[k][0, n[k])
+// = v[k] (p[k] null: none).
+struct WordInit {
+  uint64_t *p[2];
+  uint64_t v[2];
+  uint32_t n[2];
+};
+hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st,
+                              const WordInit &wi = WordInit{});
 // emit != 0: work items that hold a whole partition write their groups
 // straight to `od` (rows reserved on *emit_count); the others flush into g.
 hipError_t launch_agg(int key_bytes, const void *keys, const Schema &s, const AggPlan &plan, const Layout &L,

@@ -19,6 +19,9 @@
 //                                                        -> k_part_bucket + (c)
 //   K8/K9 intra-partition agg   T/src/kernel.cu:294-378  -> k_agg over partitions
 //   S5 + K10 compaction         T/src/kernel.cu:380-406  -> k_compact
+#include <map>
+#include <mutex>
+
 #include "tga_device.hpp"
 #include "tga_internal.hpp"
 
@@ -64,7 +67,7 @@ __global__ void __launch_bounds__(1024) k_copy16(const uint4 *__restrict__ a0, u
 }
 
 template <typename K>
-__global__ void k_gtable_init(GTable g, Schema s) {
+__global__ void k_gtable_init(GTable g, Schema s, WordInit wi) {
   if (g.S_dev) g.S = *g.S_dev;
   const uint64_t n = g.S + 1;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -89,6 +92,9 @@ __global__ void k_gtable_init(GTable g, Schema s) {
     }
   }
   if (t0 < kFlagWords) g.flags[t0] = 0;
+  // the caller's counters, set in the same launch
+  for (int k = 0; k < 2; ++k)
+    if (wi.p[k] && t0 < wi.n[k]) wi.p[k][t0] = wi.v[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,19 +1627,35 @@ hipError_t launch_copy16(const void *src, void *dst, uint64_t n16, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st) {
+hipError_t launch_gtable_init(int key_bytes, const GTable &g, const Schema &s, hipStream_t st, const WordInit &wi) {
   // sized for the allocation; with S_dev set most of it may go unused
   const unsigned grid = grid_for((g.S + 1) * kLimbs, 256, g.S_dev ? 2048 : 8192);
-  if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid), dim3(256), 0, st, g, s);
-  else hipLaunchKernelGGL(k_gtable_init<uint64_t>, dim3(grid), dim3(256), 0, st, g, s);
+  if (key_bytes == 4) hipLaunchKernelGGL(k_gtable_init<uint32_t>, dim3(grid), dim3(256), 0, st, g, s, wi);
+  else hipLaunchKernelGGL(k_gtable_init<uint64_t>, dim3(grid), dim3(256), 0, st, g, s, wi);
   return hipGetLastError();
+}
+
+// A kernel's dynamic-LDS limit, raised past 64 KB once per (device, kernel)
+// and size: the attribute call costs host time on every launch otherwise.
+hipError_t raise_lds_limit(const void *fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> done;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  size_t &have = done[{dev, fn}];
+  if (have >= bytes) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) have = bytes;
+  return e;
 }
 
 template <typename K>
 static hipError_t set_lds_limit(const void *fn, size_t bytes) {
   (void)sizeof(K);
-  if (bytes <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  return raise_lds_limit(fn, bytes);
 }
 
 bool copy_as_records(const Schema &s) {
@@ -1851,10 +1873,7 @@ hipError_t launch_part_scatter(int key_bytes, const void *keys, const Schema &s,
 hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
                               hipStream_t st) {
   const size_t lds = (size_t)(P + 1) * 4;
-  if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)&k_plan_chunks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
+  if (hipError_t e = raise_lds_limit((const void *)&k_plan_chunks, lds)) return e;
   hipLaunchKernelGGL(k_plan_chunks, dim3(1), dim3(1024), lds, st, pbase, P, n, chmax, plan);
   return hipGetLastError();
 }
@@ -1882,18 +1901,13 @@ hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = (size_t)P * 4;
-  if (lds > 65536) {
-    e = hipFuncSetAttribute((const void *)&k_colbase, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-  }
+  e = raise_lds_limit((const void *)&k_colbase, lds);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_colbase, dim3(1), dim3(1024), lds, st, total, P, pbase);
   return hipGetLastError();
 }
 
-static hipError_t lds_attr(const void *fn, size_t bytes) {
-  if (bytes <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
+static hipError_t lds_attr(const void *fn, size_t bytes) { return raise_lds_limit(fn, bytes); }
 
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
                                hipStream_t st) {
