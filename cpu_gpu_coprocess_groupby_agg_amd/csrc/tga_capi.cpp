@@ -349,6 +349,14 @@ Plan autotune(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t e
       if (recs) {
         const int b4 = std::max(1, ilog2(next_pow2((est * 4 + (uint64_t)T - 1) / (uint64_t)T)));
         if (b4 > pbits && scatter_wc_tile(q.key_bytes, vd, b4, lds_max)) pbits = b4;
+        // At least a partition per CU: the aggregation gives each partition
+        // one workgroup, so the fanout the groups need alone can leave most
+        // CUs idle (2^24 rows over 2^16 groups: 2^6 partitions, the
+        // aggregation on 64 of 256 CUs).  Only while every partition keeps
+        // 16K rows or more and one write-combining pass reaches the fanout.
+        int bw = std::min(ilog2(next_pow2((uint64_t)ctx->cus)), ilog2(std::max<uint64_t>(1, n >> 14)));
+        while (bw > pbits && !scatter_wc_tile(q.key_bytes, vd, bw, lds_max)) --bw;
+        pbits = std::max(pbits, bw);
       }
     }
     // 2^12 partitions at most, 2^13 where two passes of records reach them

@@ -113,11 +113,11 @@ def test_two_pass_partitioning(ctx, key_bits, hot, bits):
 
 
 def test_split_partitions_without_key_sample(ctx):
-    # 40% of the rows on each of two keys (partitions 1 and 8 of the 16 the
-    # tuner picks for tables a quarter full) and a group-count hint, so no
-    # key sample is taken: both partitions are split into chunks, and the
-    # work plan sizes the global table for their groups on the GPU, so one
-    # attempt does
+    # 40% of the rows on each of two keys (partitions 5 and 35 of the 64 the
+    # tuner picks: tables a quarter full would take 16, and 2^20 rows keep
+    # 16K rows in each of 64) and a group-count hint, so no key sample is
+    # taken: both partitions are split into chunks, and the work plan sizes
+    # the global table for their groups on the GPU, so one attempt does
     n = 1 << 20
     rng = np.random.default_rng(81)
     k = rng.integers(0, 1 << 62, size=20000, dtype=np.uint64)[rng.integers(0, 20000, size=n)]
@@ -127,7 +127,7 @@ def test_split_partitions_without_key_sample(ctx):
     v = rng.integers(-99, 100, size=n).astype(np.int32)
     got = ctx.groupby(k, [v], AGGS4, path=tga.PATH_PARTITIONED, est_groups=20002)
     plan = ctx.last_plan()
-    assert plan["partition_bits"] == 4 and plan["attempts"] == 1
+    assert plan["partition_bits"] == 6 and plan["attempts"] == 1
     check_equal(got, O.groupby(k, [v], AGGS4))
 
 
