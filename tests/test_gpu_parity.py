@@ -131,6 +131,23 @@ def test_split_partitions_without_key_sample(ctx):
     check_equal(got, O.groupby(k, [v], AGGS4))
 
 
+def test_mid_size_query_takes_a_partition_per_cu(ctx):
+    # 2^22 rows over 2^14 groups: tables a quarter full would take 2^4
+    # partitions, and the aggregation (a workgroup per partition) would run
+    # on 16 CUs; the tuner raises the fanout to a partition per CU while
+    # each partition keeps 16K rows (2^22 / 2^14 = 2^8 partitions at most)
+    n, d = 1 << 22, 1 << 14
+    k = ctx.generate(tga.GEN_KEYS_U64, n, a=d, seed=91, salt=6)
+    v = ctx.generate(tga.GEN_VALS_I32, n, a=(-(1 << 30)) & (2**64 - 1), b=1 << 31, seed=92)
+    got = ctx.groupby(k, [v], AGGS4, est_groups=d)
+    plan = ctx.last_plan()
+    cus = ctx.device_info()["cus"]
+    want = min(max(1, (cus - 1).bit_length()), 8)
+    assert plan["path"] == tga.PATH_PARTITIONED and plan["partition_bits"] == want and plan["attempts"] == 1, plan
+    check_equal(got, O.groupby(k.to_numpy(), [v.to_numpy()], AGGS4))
+    assert got["n_groups"] == d
+
+
 def test_spills_overflow_planned_global_table(ctx):
     # 20000 groups over 4 partitions with 256-slot LDS tables: nearly every
     # group spills to the global table.  The plan sees its partitions hold far
