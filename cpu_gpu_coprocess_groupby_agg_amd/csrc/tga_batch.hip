@@ -598,7 +598,7 @@ QShape shape_of(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t
   z.cap = std::min<uint64_t>(z.S + 1, n + 1);
   // partitioned: whole partitions emit into the output; the global table
   // only takes the rows of partitions that spill (a miss is rerun alone)
-  if (z.part) z.S = std::max<uint64_t>(4096, z.S / 8);
+  if (z.part) z.S = std::max<uint64_t>(4096, z.S / 32);
   if (z.part) {
     // partitions past the scatter's LDS are not made: larger partitions
     // spill part of their rows to the global table instead
