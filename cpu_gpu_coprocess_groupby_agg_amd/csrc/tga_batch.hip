@@ -746,7 +746,7 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
     if (Q.n > 0 && !z.part) {
       // enough rows per item to amortise the table flush (a flush of T slots
       // costs about as much as 8T rows)
-      const uint64_t per = std::max<uint64_t>(65536, std::min<uint64_t>(1 << 20, 256 * (uint64_t)z.T));
+      const uint64_t per = std::max<uint64_t>(262144, std::min<uint64_t>(1 << 20, 256 * (uint64_t)z.T));
       for (uint64_t a = 0; a < Q.n; a += per)
         lists[IT_DIRECT].push_back({(uint32_t)i, IT_DIRECT, a, std::min<uint64_t>(Q.n, a + per)});
     } else if (Q.n > 0) {
