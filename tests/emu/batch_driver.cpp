@@ -38,6 +38,7 @@ int main(int argc, char **argv) {
       {1 << 17, 30, 64, false},      {0, 1, 32, true},           {1, 1, 32, true},
       {200000, 50, 32, false},       {100000, 90000, 32, false}, {70000, 64, 32, true},
       {1 << 18, 100000, 32, true, 10}, {20000, 6000, 64, true, 10},
+      {1 << 20, 200, 32, true},  // direct over several row-range items: flushes, compaction
   };
   tga_context *ctx = nullptr;
   if (tga_context_create(0, 0, &ctx)) { std::fprintf(stderr, "context\n"); return 1; }

@@ -51,6 +51,7 @@ SPECS = [
     (200_000, 50, 32, False),
     (100_000, 90_000, 32, False),
     (70_000, 64, 32, True),
+    (1 << 20, 200, 32, True),        # direct, several row-range items: flushes and compaction
 ]
 
 
