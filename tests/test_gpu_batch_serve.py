@@ -93,6 +93,46 @@ def test_batch_matches_oracle(ctx, mode):
         lib().tga_batch_destroy(b)
 
 
+def test_batch_profile_per_item_type(ctx):
+    # tga_batch_profile: per item type (INIT, COUNT, DIRECT, SCAN, SCATTER,
+    # PAGG, COMPACT) the items run and their workgroup-ms, for a batch built
+    # with kernel timing on; TGA_ERR_UNSUPPORTED for one built without
+    qs, keep, host = _queries(ctx, SPECS)
+    n = len(SPECS)
+    arrs = lambda: ((ctypes.c_double * 7)(), (ctypes.c_double * 7)(), (ctypes.c_int64 * 7)(),
+                    (ctypes.c_double * 7)(), (ctypes.c_double * 8)())
+    b = ctypes.c_void_p()
+    ctx._check(lib().tga_batch_create(ctx.h, qs, n, 0, ctypes.byref(b)))
+    try:
+        ctx._check(lib().tga_batch_run(b))
+        assert lib().tga_batch_profile(b, *arrs()) == tga.ERR_UNSUPPORTED
+    finally:
+        lib().tga_batch_destroy(b)
+    ctx.profile(True)
+    b = ctypes.c_void_p()
+    try:
+        ctx._check(lib().tga_batch_create(ctx.h, qs, n, 0, ctypes.byref(b)))
+        ctx._check(lib().tga_batch_run(b))
+        work, wait, items, fence, phase = arrs()
+        ctx._check(lib().tga_batch_profile(b, work, wait, items, fence, phase))
+        names = ["INIT", "COUNT", "DIRECT", "SCAN", "SCATTER", "PAGG", "COMPACT"]
+        got = {t: items[i] for i, t in enumerate(names)}
+        # every query clears its table and compacts it; partitioned queries
+        # (three hinted ones in the mix, and a sampled one may be) take a
+        # histogram scan each, tiles and partitions
+        assert got["INIT"] >= n and got["COMPACT"] >= n and 3 <= got["SCAN"] <= 4, got
+        assert got["DIRECT"] > 0 and got["COUNT"] == got["SCATTER"] > 0 and got["PAGG"] > 0, got
+        for i, t in enumerate(names):
+            assert work[i] >= 0 and wait[i] >= 0 and fence[i] >= 0
+            if got[t]:
+                assert work[i] > 0, t
+        assert all(x >= 0 for x in phase)
+    finally:
+        if b.value:
+            lib().tga_batch_destroy(b)
+        ctx.profile(False)
+
+
 def test_batch_underestimate_reruns(ctx):
     # hints far below the truth: table overflow is detected and the query
     # rerun alone, at every replay.  A second batch built between replays and
