@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(1024) k_agg(const K *__restrict__ keys, Schema
       s_abort[1] = 0;
     }
     __syncthreads();
-    const BucketList bl{plan.blist};
+    const BucketList bl{plan.blist, plan.blist2};
     {
       const void *cols[NC];
 #pragma unroll

@@ -24,6 +24,7 @@
 // complete.  PAGG and COMPACT, which read with plain loads, acquire after
 // observing the counter.  Two workgroups share each CU.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 
 #include "tga_batch_internal.hpp"
@@ -616,6 +617,7 @@ BatchExec::~BatchExec() {
   if (ctx) {
     ctx->pinned->put(h_counts, (uint64_t)std::max(1, nq) * 8);
     ctx->pinned->put(h_flags, (uint64_t)std::max(1, nq) * kFlagWords * 4);
+    ctx->pinned->put(h_stage, stage_bytes);
   }
 }
 
@@ -678,7 +680,7 @@ tga_status batch_prepare(tga_context *ctx, const tga_query *queries, int32_t n, 
   return TGA_OK;
 }
 
-tga_status batch_build(BatchExec &b, const tga_query *qs) {
+tga_status batch_build(BatchExec &b, const tga_query *qs, hipStream_t upload) {
   tga_context *ctx = b.ctx;
   const int32_t n = b.nq;
   b.hq.assign(n, BQuery{});
@@ -795,10 +797,25 @@ tga_status batch_build(BatchExec &b, const tga_query *qs) {
   b.nitems = (uint32_t)items.size();
   if (!b.dq.alloc(ctx, std::max(1, n) * sizeof(BQuery)) || !b.ditems.alloc(ctx, std::max<size_t>(1, items.size()) * sizeof(BItem)))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "batch queue");
-  if (n) TGA_HIP(ctx, hipMemcpyAsync(b.dq.p, b.hq.data(), (uint64_t)n * sizeof(BQuery), hipMemcpyHostToDevice, ctx->stream));
-  if (!items.empty())
-    TGA_HIP(ctx, hipMemcpyAsync(b.ditems.p, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, ctx->stream));
-  TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const uint64_t qbytes = (uint64_t)n * sizeof(BQuery), ibytes = items.size() * sizeof(BItem);
+  if (upload) {
+    // A server thread building the next flight must not wait here: while
+    // other streams' kernels hold the CUs, an upload from pageable memory and
+    // its wait cost it hundreds of microseconds.  The pinned staging stays
+    // with the batch until it is destroyed, after its stream has finished.
+    b.stage_bytes = std::max<uint64_t>(1, qbytes + ibytes);
+    b.h_stage = ctx->pinned->get(b.stage_bytes);
+    if (!b.h_stage) return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "pinned batch queue");
+    char *hs = (char *)b.h_stage;
+    if (qbytes) std::memcpy(hs, b.hq.data(), qbytes);
+    if (ibytes) std::memcpy(hs + qbytes, items.data(), ibytes);
+    if (qbytes) TGA_HIP(ctx, hipMemcpyAsync(b.dq.p, hs, qbytes, hipMemcpyHostToDevice, upload));
+    if (ibytes) TGA_HIP(ctx, hipMemcpyAsync(b.ditems.p, hs + qbytes, ibytes, hipMemcpyHostToDevice, upload));
+  } else {
+    if (qbytes) TGA_HIP(ctx, hipMemcpyAsync(b.dq.p, b.hq.data(), qbytes, hipMemcpyHostToDevice, ctx->stream));
+    if (ibytes) TGA_HIP(ctx, hipMemcpyAsync(b.ditems.p, items.data(), ibytes, hipMemcpyHostToDevice, ctx->stream));
+    TGA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
   b.lds = std::max<size_t>(lds_need, 1024);
   // kernel variant: fp64 SUM anywhere in the batch / more than one value column
   bool fx = false, multi = false;

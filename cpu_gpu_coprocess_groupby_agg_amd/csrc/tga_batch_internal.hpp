@@ -67,13 +67,17 @@ struct BatchExec {
   const void *kernel = nullptr;  // k_batch variant for this batch
   uint64_t *h_counts = nullptr;  // pinned: group count per query
   uint32_t *h_flags = nullptr;   // pinned: status words per query
+  void *h_stage = nullptr;       // pinned: query records + work items, when uploaded without a wait
+  uint64_t stage_bytes = 0;
   ~BatchExec();
 };
 
 // Validates the queries and computes the cardinality estimates.
 tga_status batch_prepare(tga_context *ctx, const tga_query *qs, int32_t n, BatchExec &b);
-// Allocates tables/outputs and uploads the work queue (uses ctx->stream).
-tga_status batch_build(BatchExec &b, const tga_query *qs);
+// Allocates tables/outputs and uploads the work queue: on ctx->stream, waited
+// for, or (upload != null) from pinned staging on `upload`, not waited for --
+// the caller then enqueues the batch on that same stream.
+tga_status batch_build(BatchExec &b, const tga_query *qs, hipStream_t upload = nullptr);
 // Device bytes batch_build takes from the arena for one query (admission).
 uint64_t batch_query_bytes(const tga_context *ctx, const QuerySpec &q, uint64_t n, uint64_t est);
 // Queues reset + persistent kernel + status copies on `st`.

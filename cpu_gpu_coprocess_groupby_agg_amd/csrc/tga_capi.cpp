@@ -721,18 +721,21 @@ struct PartBufs {
 static tga_status bucket_bufs(tga_context *ctx, uint64_t cap, uint32_t P, DBuf &heads, DBuf &meta_buf,
                               BucketBufs &bb) {
   if (cap >= (1ULL << 32)) return fail(ctx, TGA_ERR_UNSUPPORTED, "too many buckets (%llu)", (unsigned long long)cap);
-  // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | prow [P] u64
-  const uint64_t meta_words = 1 + ((uint64_t)P * 16 + 8) / 8;
-  if (!heads.alloc(ctx, cap * 16) || !meta_buf.alloc(ctx, meta_words * 8))
+  // meta: pool u32 (+ pad) | pbk [P] u32 | bcur [P] u32 | bend [P] u32 (+ pad) | prow [P] u64
+  const uint64_t head_words = 1 + ((uint64_t)P * 12 + 7) / 8;
+  const uint64_t meta_words = head_words + P;
+  if (!heads.alloc(ctx, cap * 24) || !meta_buf.alloc(ctx, meta_words * 8))
     return fail(ctx, TGA_ERR_OUT_OF_MEMORY, "partition buffers");
   bb.blist = heads.as<uint64_t>();
-  bb.bpart = (uint32_t *)(bb.blist + cap);
+  bb.blist2 = bb.blist + cap;
+  bb.bpart = (uint32_t *)(bb.blist2 + cap);
   bb.bcnt = bb.bpart + cap;
   uint64_t *meta = meta_buf.as<uint64_t>();
   bb.pool = (uint32_t *)meta;
   bb.pbk = (uint32_t *)(meta + 1);
   bb.bcur = bb.pbk + P;
-  bb.prow = meta + 1 + ((uint64_t)P * 8 + 7) / 8;
+  bb.bend = bb.bcur + P;
+  bb.prow = meta + head_words;
   bb.cap = (uint32_t)cap;
   TGA_HIP(ctx, launch_fill_u64(meta, meta_words, 0, ctx->stream));
   return TGA_OK;
@@ -771,7 +774,7 @@ static tga_status first_pass(tga_context *ctx, const QuerySpec &q, const void *k
   }));
   pb.ap1 = part_plan(pb.plan1, P1, gs_cap, gs_per_split);
   pb.ap1.blist = pb.bb1.blist;
-  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb1, P1, chmax, pb.ap1, st); }));
+  TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb1, P1, chmax, pb.ap1, st, false); }));
   return TGA_OK;
 }
 
@@ -822,6 +825,8 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     pb.planbuf = std::move(pb.plan1);
     pb.bb = pb.bb1;
     pb.ap = pb.ap1;
+    pb.ap.blist2 = pb.bb.blist2;
+    TGA_HIP(ctx, timed(ctx, "bucket_pairs", [&] { return launch_bucket_pairs(pb.bb, P, pb.ap, st); }));
     pb.pool1 = nullptr;
     pb.cap1 = 0;
     pb.sp = s;
@@ -836,7 +841,8 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
     for (int c = 0; c < kMaxCols; ++c) pb.sp.cols[c] = nullptr;  // the values travel in the records
     pb.ap = part_plan(pb.planbuf, P, pb.gs_cap, pb.gs_per_split);
     pb.ap.blist = pb.bb.blist;
-    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb, P, plan.chunk_rows, pb.ap, st); }));
+    pb.ap.blist2 = pb.bb.blist2;
+    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(pb.bb, P, plan.chunk_rows, pb.ap, st, true); }));
     return TGA_OK;
   }
   if (pb.buckets) {
@@ -866,7 +872,8 @@ static tga_status partition_rows(tga_context *ctx, const QuerySpec &q, const voi
                                  nullptr, grid, st, plan.scatter_wc);
     }));
     pb.ap.blist = bb.blist;
-    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st); }));
+    pb.ap.blist2 = bb.blist2;
+    TGA_HIP(ctx, timed(ctx, "bucket_lists", [&] { return launch_bucket_lists(bb, P, plan.chunk_rows, pb.ap, st, true); }));
     return TGA_OK;
   }
   const uint32_t ntiles = (uint32_t)((m + plan.tile_rows - 1) / plan.tile_rows);

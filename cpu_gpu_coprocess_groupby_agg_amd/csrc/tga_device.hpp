@@ -526,170 +526,56 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_lds(uint32_t *arr, uint
 // Contiguous mode: rows [b, e), taken by the waves in blocks of kBucketRows
 // rows (wave w takes blocks w, w+nw, ...).  Bucket mode (ents != null): b..e
 // index a list of bucket entries, rows << 32 | id; bucket `id` holds `rows`
-// (1..kBucketRows) rows at [id*kBucketRows, ...) of the partitioned copy, and
-// one bucket is one wave block.
+// (1..kBucketRows) rows at [id*kBucketRows, ...) of the partitioned copy.
+// One entry is one wave block: the bucket of ents[k], followed by the bucket
+// of ents2[k] when that is not 0 (two partly filled buckets of at most
+// kBucketRows rows together).  An entry of 0 rows ends the list.
 // ---------------------------------------------------------------------------
 
 struct BucketList {
-  const uint64_t *ents;  // null: contiguous mode
+  const uint64_t *ents;   // null: contiguous mode
+  const uint64_t *ents2;  // with ents: the second bucket of each block
+};
+
+struct ListEnt {
+  uint64_t a, b;
 };
 
 // List entry of block i of wave `wave` (0 in contiguous mode).  Past the end
-// of the list it is the last entry: the load is unconditional, so the
-// register it fills is never also set by another path while it is in flight
+// of the list it is the last entry: the loads are unconditional, so the
+// registers they fill are never also set by another path while in flight
 // (which would make the compiler wait for every load in flight first).
-__device__ __forceinline__ uint64_t list_entry(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
-                                               uint32_t nw) {
-  if (!bl.ents || e <= b) return 0;
+__device__ __forceinline__ ListEnt list_entry(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i,
+                                              uint32_t wave, uint32_t nw) {
+  if (!bl.ents || e <= b) return ListEnt{0, 0};
   const uint64_t k = b + wave + i * nw;
-  return bl.ents[k < e ? k : e - 1];
+  const uint64_t kk = k < e ? k : e - 1;
+  return ListEnt{bl.ents[kk], bl.ents2[kk]};
 }
 
-// Block i of wave `wave` (wave-uniform), given its list entry: rows
-// [bb, be); false past the end.
-__device__ __forceinline__ bool wave_block(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
-                                           uint32_t nw, uint64_t ent, uint64_t &bb, uint64_t &be) {
-  if (bl.ents) {
-    if (b + wave + i * nw >= e || (ent >> 32) == 0) return false;
-    bb = (ent & 0xffffffffULL) << kBucketShift;
-    be = bb + (ent >> 32);
-    return true;
-  }
-  bb = b + (wave + i * nw) * (uint64_t)kBucketRows;
-  be = e;
-  return bb < e;
-}
-
-// The same as a wave-uniform first row and row count (1..kBucketRows), so
-// the row loop addresses a block from a scalar base with 32-bit lane
-// offsets; false past the end.
+// Block i of wave `wave` (wave-uniform), given its list entry: its first
+// row, row count (1..kBucketRows) and, for a block of two buckets, the first
+// row of the second one and the row count of the first (ca == cnt: one
+// bucket, or contiguous rows).  The row loop addresses a block from a scalar
+// base with 32-bit lane offsets; false past the end.
 __device__ __forceinline__ bool wave_block_n(const BucketList &bl, uint64_t b, uint64_t e, uint64_t i, uint32_t wave,
-                                             uint32_t nw, uint64_t ent, uint64_t &bb, uint32_t &cnt) {
+                                             uint32_t nw, ListEnt ent, uint64_t &bb, uint32_t &cnt, uint64_t &bb2,
+                                             uint32_t &ca) {
   if (bl.ents) {
-    ent = wave_uniform64(ent);
-    if (b + wave + i * nw >= e || (ent >> 32) == 0) return false;
-    bb = (ent & 0xffffffffULL) << kBucketShift;
-    cnt = (uint32_t)(ent >> 32);
+    const uint64_t x = wave_uniform64(ent.a), y = wave_uniform64(ent.b);
+    if (b + wave + i * nw >= e || (x >> 32) == 0) return false;
+    bb = (x & 0xffffffffULL) << kBucketShift;
+    ca = (uint32_t)(x >> 32);
+    bb2 = (y & 0xffffffffULL) << kBucketShift;
+    cnt = ca + (uint32_t)(y >> 32);
     return true;
   }
   bb = b + (wave + i * nw) * (uint64_t)kBucketRows;
   if (bb >= e) return false;
   cnt = (uint32_t)min<uint64_t>(kBucketRows, e - bb);
+  bb2 = bb;
+  ca = cnt;
   return true;
-}
-
-// Aggregates the item's rows (see wave_block) into the LDS table.  Rows whose
-// key finds no free slot within max_probe go straight to the global table.
-// FX: the schema has exact fp64 SUM accumulators (their limb vectors cost
-// registers, so the common integer case is compiled without them).
-template <typename K, bool FX, int NC = kMaxCols>
-__device__ __forceinline__ void agg_rows(const K *__restrict__ keys, const Schema &s, uint64_t b0, uint64_t e0,
-                         const LdsTable<K> &t, const GTable &g, int nbits, int max_probe,
-                         const BucketList bl = BucketList{nullptr}) {
-  constexpr int U = 4;
-  constexpr int NF = FX ? kMaxFx : 1;
-  const int lane = (int)__lane_id();
-  const uint32_t wave = wave_uniform(threadIdx.x >> 6);
-  const uint32_t nw = blockDim.x >> 6;
-  const int ncol = s.ncols, nacc = s.nacc;
-  uint64_t base = 0, e = 0;
-  for (uint64_t it = 0; wave_block(bl, b0, e0, it, wave, nw, list_entry(bl, b0, e0, it, wave, nw), base, e); ++it) {
-    K kk[U];
-    uint64_t cell[U][kMaxCols];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t row = base + (uint64_t)u * kWave + lane;
-      const bool act = row < e;
-      kk[u] = act ? keys[row] : KeyTraits<K>::kEmpty;
-#pragma unroll
-      for (int c = 0; c < kMaxCols; ++c) cell[u][c] = (c < NC && act && c < ncol) ? load_cell(s, c, row) : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t row = base + (uint64_t)u * kWave + lane;
-      const bool act = row < e;
-      if (!__ballot(act)) break;
-      const K key = kk[u];
-      const uint64_t h = key_hash((uint64_t)key);
-      const int slot = act ? lds_find_or_insert(t, key, h, max_probe) : -1;
-      uint64_t ops[kMaxAccs];
-      int64_t lv[NF][kLimbs];
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-#pragma unroll
-        for (int l = 0; l < kLimbs; ++l) lv[f][l] = 0;
-#pragma unroll
-      for (int a = 0; a < kMaxAccs; ++a) {
-        ops[a] = 0;
-        if (a < nacc && act) {
-          ops[a] = acc_operand(s, a, cell[u]);
-          if constexpr (FX) {
-            if (s.acc[a].kind == ACC_SUM_F) {
-              if (s.fx_slot[a] == 0) fx_expand(ops[a], lv[0], g.flags);
-              else fx_expand(ops[a], lv[NF - 1], g.flags);
-            }
-          }
-        }
-      }
-      const bool spill = act && slot < 0;
-      if (__ballot(spill)) {
-        if (spill) {
-          const int64_t gs = g_find_or_insert<K>(g, key, h);
-          if (gs >= 0) {
-#pragma unroll
-            for (int a = 0; a < kMaxAccs; ++a) {
-              if (a >= nacc) break;
-              const int kind = s.acc[a].kind;
-              if (kind == ACC_SUM_F) {
-                if constexpr (FX) g_apply_limbs(g, a, gs, s.fx_slot[a] == 0 ? lv[0] : lv[NF - 1]);
-              } else {
-                g_apply(g, a, kind, gs, ops[a]);
-              }
-            }
-          }
-        }
-      }
-      const bool on = act && slot >= 0;
-      const uint64_t m = match_mask((uint32_t)slot, nbits, on);
-      const bool leader = on && ((m & lanemask_lt()) == 0);
-      uint64_t rest = leader ? (m & (m - 1)) : 0ULL;
-      // fold duplicate keys of this wave into their leader lane
-      while (__ballot(rest != 0)) {
-        const int src = rest ? (int)__builtin_ctzll(rest) : lane;
-#pragma unroll
-        for (int a = 0; a < kMaxAccs; ++a) {
-          if (a >= nacc) break;
-          const int kind = s.acc[a].kind;
-          if (kind != ACC_SUM_F) {
-            const uint64_t o = shfl64(ops[a], src);
-            if (rest) ops[a] = acc_fold(kind, ops[a], o);
-          }
-        }
-        if constexpr (FX) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f)
-#pragma unroll
-            for (int l = 0; l < kLimbs; ++l) {
-              const int64_t o = (int64_t)shfl64((uint64_t)lv[f][l], src);
-              if (rest) lv[f][l] += o;
-            }
-        }
-        if (rest) rest &= rest - 1;
-      }
-      if (leader) {
-#pragma unroll
-        for (int a = 0; a < kMaxAccs; ++a) {
-          if (a >= nacc) break;
-          const int kind = s.acc[a].kind;
-          if (kind == ACC_SUM_F) {
-            if constexpr (FX) lds_apply_limbs(t, a, slot, s.fx_slot[a] == 0 ? lv[0] : lv[NF - 1]);
-          } else {
-            lds_apply(t, a, kind, slot, ops[a]);
-          }
-        }
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -876,28 +762,35 @@ using CellOf = typename std::conditional<(VW == 4 || vw_is_rec(VW)), uint32_t, u
 
 // Rows [base, base + cnt) of a block (base and cnt wave-uniform, cnt >= 1):
 // lane l takes rows u * 64 + l.
-template <typename K, int NC, int VW, int U>
+template <typename K, int NC, int VW, int U, bool PAIR = false>
 __device__ __forceinline__ void load_block(const K *__restrict__ keys, const void *const *cols, uint32_t ncols,
                                            uint32_t coldt, uint64_t base, uint32_t cnt, int lane, K *kk,
-                                           CellOf<VW> (*raw)[NC]) {
+                                           CellOf<VW> (*raw)[NC], uint64_t base2 = 0, uint32_t ca = 0) {
   // rows past the end re-read the block's last row (in bounds, no branch),
   // so all U key and value loads are in flight together
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const uint32_t j = min((uint32_t)(u * kWave + lane), cnt - 1);
-    const uint64_t r = base + j;
+    uint32_t j = min((uint32_t)(u * kWave + lane), cnt - 1);
+    uint64_t blk = base;
+    if constexpr (PAIR) {
+      // rows ca.. of the block are the second bucket's (a per-lane base)
+      const bool first = j < ca;
+      blk = first ? base : base2;
+      j = first ? j : j - ca;
+    }
+    const uint64_t r = blk + j;
     if constexpr (VW == kVwRec4) {  // records (key, 4-byte value) at `keys`
       static_assert(NC == 1, "records carry one value column");
       uint32_t v;
       // a scalar block base plus a 32-bit lane offset
-      Rec4<K>::load_at((const char *)keys + base * (uint64_t)Rec4<K>::kBytes, j * (uint32_t)Rec4<K>::kBytes, kk[u], v);
+      Rec4<K>::load_at((const char *)keys + blk * (uint64_t)Rec4<K>::kBytes, j * (uint32_t)Rec4<K>::kBytes, kk[u], v);
       raw[u][0] = v;
     } else if constexpr (vw_is_rec(VW)) {  // records of VD value words at `keys`
       constexpr int VD = vw_rec_vd(VW);
       static_assert(VD <= NC, "one cell per value word");
       using R = RecW<K, VD>;
       uint32_t v[VD];
-      R::load_at((const char *)keys + base * (uint64_t)R::kBytes, j * (uint32_t)R::kBytes, kk[u], v);
+      R::load_at((const char *)keys + blk * (uint64_t)R::kBytes, j * (uint32_t)R::kBytes, kk[u], v);
 #pragma unroll
       for (int d = 0; d < NC; ++d) raw[u][d] = d < VD ? v[d < VD ? d : 0] : 0u;
     } else if constexpr (VW == 4) {
@@ -1015,13 +908,16 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   uint32_t consumed = 0;
   // bucket mode: the list entry of a block is read two blocks ahead, so no
   // wave waits on it before issuing that block's row loads
-  uint64_t base = 0;
-  uint32_t cnt = 0;
-  bool have = wave_block_n(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, cnt);
-  uint64_t ent_next = list_entry(bl, b0, e0, 1, wave, nw);
+  uint64_t base = 0, base2 = 0;
+  uint32_t cnt = 0, ca = 0;
+  bool have = wave_block_n(bl, b0, e0, 0, wave, nw, list_entry(bl, b0, e0, 0, wave, nw), base, cnt, base2, ca);
+  ListEnt ent_next = list_entry(bl, b0, e0, 1, wave, nw);
   K kk[U];
   CellOf<VW> cel[U][NC];
-  if (have) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel);
+  if (have) {
+    if (ca != cnt) load_block<K, NC, VW, U, true>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel, base2, ca);
+    else load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel);
+  }
   // The first block is waited for here, so the loop is entered with no load
   // in flight, as it is re-entered: otherwise the compiler, merging the two
   // states, waits inside every iteration for the loads it has just issued
@@ -1034,12 +930,17 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     if (giveup && wave_uniform(stop)) break;
     if (giveup) stop = lds_read_volatile(giveup);
     consumed += kWave * U;
-    uint64_t nbase = 0;
-    uint32_t ncnt = 0;
-    const bool next = wave_block_n(bl, b0, e0, it, wave, nw, ent_next, nbase, ncnt);
+    uint64_t nbase = 0, nbase2 = 0;
+    uint32_t ncnt = 0, nca = 0;
+    const bool next = wave_block_n(bl, b0, e0, it, wave, nw, ent_next, nbase, ncnt, nbase2, nca);
     K nkk[U];
     CellOf<VW> ncel[U][NC];
-    if (next) load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel);
+    if (next) {
+      if (nca != ncnt)
+        load_block<K, NC, VW, U, true>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel, nbase2, nca);
+      else
+        load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel);
+    }
     ent_next = list_entry(bl, b0, e0, it + 1, wave, nw);
     uint32_t hh[U];
     int slot[U];

@@ -12,7 +12,10 @@ namespace tga {
 // cut into cstart[p+1]-cstart[p] slices; ticket hands out slice ids.  With a
 // bucket list (blist != null) pstart indexes blist instead: segment p is the
 // buckets blist[pstart[p] .. pstart[p+1]), each entry rows << 32 | id for the
-// bucket holding `rows` rows at rows [id * kBucketRows, ...).
+// bucket holding `rows` rows at rows [id * kBucketRows, ...).  blist2 (set
+// with blist for k_agg) holds a second bucket of the same block, entry 0 for
+// none: two partly filled buckets whose rows fit one block are one block.
+// An entry of 0 rows ends the segment (paired lists end in such entries).
 struct AggPlan {
   uint64_t *pstart;  // [P+1]
   uint32_t *cstart;  // [P+1]
@@ -20,6 +23,7 @@ struct AggPlan {
   uint32_t *ticket;
   uint64_t *split_rows;  // rows of partitions cut into more than one item
   const uint64_t *blist;
+  const uint64_t *blist2;
   // Global-table sizing by the plan kernel (gslots null: none).  Only the
   // partitions cut into several items aggregate through the global table,
   // so it takes the power of two >= min(2 * their rows, gs_per_split * their
@@ -73,12 +77,13 @@ hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
                            hipStream_t st);
 size_t scatter_lds_bytes(const Schema &s, int key_bytes, int pbits, uint32_t tile_rows);
 
-// Buckets of the count-free partitioning: bpart/bcnt [cap] u32, blist [cap]
-// u64 entries; pool (the allocation counter), pbk [P] u32, prow [P] u64 and
-// bcur [P] u32 start zeroed.
+// Buckets of the count-free partitioning: bpart/bcnt [cap] u32, blist and
+// blist2 [cap] u64 entries; pool (the allocation counter), pbk [P] u32, prow
+// [P] u64 and bcur [P] u32 start zeroed; bend [P] u32 (the back cursor of
+// the partly filled buckets) is set by the list plan.
 struct BucketBufs {
-  uint32_t *bpart, *bcnt, *pool, *pbk, *bcur;
-  uint64_t *blist, *prow;
+  uint32_t *bpart, *bcnt, *pool, *pbk, *bcur, *bend;
+  uint64_t *blist, *blist2, *prow;
   uint32_t cap;
 };
 // persistent workgroups of the scatter for ntiles tiles (at most max_grid)
@@ -128,9 +133,13 @@ hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint
 hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uint64_t chmax, const AggPlan &plan,
                               hipStream_t st);
 // bucket headers -> per-partition bucket lists and the k_agg work plan
-// (items of at most about chmax rows); plan.blist/bcnt must point at bb's
+// (items of at most about chmax rows); plan.blist/bcnt must point at bb's.
+// pair: the lists feed k_agg -- partly filled buckets of a partition are
+// paired into blocks of at most kBucketRows rows (plan.blist2 = bb.blist2).
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
-                               hipStream_t st);
+                               hipStream_t st, bool pair);
+// The pairing alone, over lists launch_bucket_lists made with pair = false.
+hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 // adds the group count of g (occupied slots + the empty-key slot) to *out

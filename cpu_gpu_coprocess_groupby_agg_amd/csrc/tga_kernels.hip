@@ -1040,7 +1040,7 @@ __global__ void __launch_bounds__(1024) k_bucket_count(const uint32_t *__restric
 // [pstart[p], pstart[p+1]) and is cut into ceil(rows / chmax) work items;
 // bcur[p] = pstart[p] (the fill cursor of k_bucket_list).  One workgroup.
 __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const uint64_t *prow, uint32_t P,
-                                                      uint64_t chmax, uint32_t *bcur, AggPlan plan) {
+                                                      uint64_t chmax, uint32_t *bcur, uint32_t *bend, AggPlan plan) {
   extern __shared__ uint32_t cnt[];  // [2P]
   __shared__ uint32_t scratch[17];
   __shared__ unsigned long long s_srows;
@@ -1063,6 +1063,7 @@ __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
     plan.pstart[p] = nbk[p];
     bcur[p] = nbk[p];
+    bend[p] = p + 1 < P ? nbk[p + 1] : buckets;
     plan.cstart[p] = nit[p];
   }
   if (threadIdx.x == 0) {
@@ -1074,12 +1075,16 @@ __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const
 }
 
 // Buckets grouped by partition: blist[pstart[p] ..] = the entries
-// rows << 32 | id of the buckets of p (any order).  Each workgroup counts its
-// range of ids per partition in LDS, reserves its share of every list with
-// one atomic, then places its entries.
+// rows << 32 | id of the buckets of p, the full ones first (any order), then
+// the partly filled ones (one per scatter workgroup that saw the partition).
+// Each workgroup counts its range of full buckets per partition in LDS,
+// reserves its share of every list with one atomic, then places its entries;
+// a partly filled bucket takes its place from the back with an atomic of its
+// own (bend[p] starts at pstart[p+1]).
 __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict__ bpart,
                                                       const uint32_t *__restrict__ bcnt, const uint32_t *pool,
-                                                      uint32_t cap, uint32_t P, uint32_t *bcur, uint64_t *blist) {
+                                                      uint32_t cap, uint32_t P, uint32_t *bcur, uint32_t *bend,
+                                                      uint64_t *blist) {
   extern __shared__ uint32_t cnt[];  // [2P]
   uint32_t *lc = cnt, *lb = cnt + P;
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) lc[p] = 0;
@@ -1089,7 +1094,7 @@ __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict
   const uint32_t lo = blockIdx.x * per, hi = min(nb, lo + per);
   for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
     const uint32_t p = bpart[id];
-    if (p < P) atomicAdd(&lc[p], 1u);
+    if (p < P && bcnt[id] == kBucketRows) atomicAdd(&lc[p], 1u);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
@@ -1100,7 +1105,78 @@ __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict
   for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
     const uint32_t p = bpart[id];
     if (p >= P) continue;
-    blist[lb[p] + atomicAdd(&lc[p], 1u)] = ((uint64_t)bcnt[id] << 32) | id;
+    const uint32_t rows = bcnt[id];
+    const uint32_t at = rows == kBucketRows ? lb[p] + atomicAdd(&lc[p], 1u) : atomicAdd(&bend[p], ~0u) - 1u;
+    blist[at] = ((uint64_t)rows << 32) | id;
+  }
+}
+
+// Pairs the partly filled buckets of partition blockIdx.x (list entries
+// [bend[p], pstart[p+1])) into blocks of at most kBucketRows rows, so k_agg,
+// whose wave block is one bucket, does not spend a whole block on each.
+// Sorted by rows a[0] <= ... <= a[m-1], the largest k for which every pair
+// (a[i], a[2k-1-i]), i < k, fits is the most pairs any pairing reaches
+// (if the 2k smallest can be paired at all, they can be paired this way, and
+// the largest feasible k is found by bisection: feasibility only shrinks as
+// k grows).  Written: the k pairs (larger bucket in blist, smaller in
+// blist2), then the m - 2k unpaired buckets, then k entries of 0 rows, which
+// end the segment.  blist2 of the full buckets is 0.
+constexpr uint32_t kPairMax = 1024;
+__global__ void __launch_bounds__(256) k_bucket_pair(const uint64_t *__restrict__ pstart,
+                                                     const uint32_t *__restrict__ bend, uint64_t *blist,
+                                                     uint64_t *blist2) {
+  __shared__ uint64_t a[kPairMax];
+  __shared__ uint32_t s_bad;
+  const uint32_t p = blockIdx.x;
+  const uint64_t b = pstart[p], end = pstart[p + 1], lo = bend[p];
+  for (uint64_t k = b + threadIdx.x; k < lo; k += blockDim.x) blist2[k] = 0;
+  const uint32_t m = (uint32_t)(end - lo);
+  if (m < 2 || m > kPairMax) {
+    for (uint64_t k = lo + threadIdx.x; k < end; k += blockDim.x) blist2[k] = 0;
+    return;
+  }
+  uint32_t m2 = 2;
+  while (m2 < m) m2 <<= 1;
+  // entries sort by rows, then id; padding sorts last
+  for (uint32_t i = threadIdx.x; i < m2; i += blockDim.x) a[i] = i < m ? blist[lo + i] : ~0ULL;
+  __syncthreads();
+  for (uint32_t size = 2; size <= m2; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m2 / 2; i += blockDim.x) {
+        const uint32_t x = 2 * i - (i & (stride - 1)), y = x + stride;
+        const bool up = (x & size) == 0;
+        const uint64_t u = a[x], v = a[y];
+        if ((u > v) == up) {
+          a[x] = v;
+          a[y] = u;
+        }
+      }
+      __syncthreads();
+    }
+  uint32_t klo = 0, khi = m / 2;
+  while (klo < khi) {
+    const uint32_t k = (klo + khi + 1) / 2;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x)
+      if ((a[i] >> 32) + (a[2 * k - 1 - i] >> 32) > kBucketRows) s_bad = 1;
+    __syncthreads();
+    const bool bad = s_bad != 0;
+    __syncthreads();
+    if (bad) khi = k - 1;
+    else klo = k;
+  }
+  const uint32_t k = klo;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    uint64_t x = 0, y = 0;
+    if (i < k) {
+      x = a[2 * k - 1 - i];
+      y = a[i];
+    } else if (i < m - k) {
+      x = a[k + i];
+    }
+    blist[lo + i] = x;
+    blist2[lo + i] = y;
   }
 }
 
@@ -1910,7 +1986,7 @@ hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint
 static hipError_t lds_attr(const void *fn, size_t bytes) { return raise_lds_limit(fn, bytes); }
 
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
-                               hipStream_t st) {
+                               hipStream_t st, bool pair) {
   const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(256, bb.cap / 4096));
   size_t lds = (size_t)P * 8;
   hipError_t e = lds_attr((const void *)&k_bucket_count, lds);
@@ -1920,11 +1996,17 @@ hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax,
   if ((e = hipGetLastError()) != hipSuccess) return e;
   lds = (size_t)P * 8;
   if ((e = lds_attr((const void *)&k_bucket_plan, lds)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bucket_plan, dim3(1), dim3(1024), lds, st, bb.pbk, bb.prow, P, chmax, bb.bcur, plan);
+  hipLaunchKernelGGL(k_bucket_plan, dim3(1), dim3(1024), lds, st, bb.pbk, bb.prow, P, chmax, bb.bcur, bb.bend, plan);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = lds_attr((const void *)&k_bucket_list, lds)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_bucket_list, dim3(grid), dim3(1024), lds, st, bb.bpart, bb.bcnt, bb.pool, bb.cap, P, bb.bcur,
-                     bb.blist);
+                     bb.bend, bb.blist);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return pair ? launch_bucket_pairs(bb, P, plan, st) : hipSuccess;
+}
+
+hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st) {
+  hipLaunchKernelGGL(k_bucket_pair, dim3(P), dim3(256), 0, st, plan.pstart, bb.bend, bb.blist, bb.blist2);
   return hipGetLastError();
 }
 

@@ -278,7 +278,7 @@ bool tga_server::launch_one() {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     hipSetDevice(ctx->device);
     st = batch_prepare(ctx, qs.data(), (int32_t)qs.size(), *f.ex);
-    if (!st) st = batch_build(*f.ex, qs.data());
+    if (!st) st = batch_build(*f.ex, qs.data(), streams[slot]);
     if (!st) st = batch_enqueue(*f.ex, streams[slot]);
     if (!st && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess) st = TGA_ERR_HIP;
     if (!st && hipEventRecord(f.ev, streams[slot]) != hipSuccess) st = TGA_ERR_HIP;
