@@ -1114,7 +1114,7 @@ __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict
 // Pairs the partly filled buckets of partition blockIdx.x (list entries
 // [bend[p], pstart[p+1])) into blocks of at most kBucketRows rows, so k_agg,
 // whose wave block is one bucket, does not spend a whole block on each.
-// Sorted by rows a[0] <= ... <= a[m-1], the largest k for which every pair
+// Sorted by rows a[0] <= ... <= a[m-1] (a counting sort), the largest k for which every pair
 // (a[i], a[2k-1-i]), i < k, fits is the most pairs any pairing reaches
 // (if the 2k smallest can be paired at all, they can be paired this way, and
 // the largest feasible k is found by bisection: feasibility only shrinks as
@@ -1126,6 +1126,8 @@ __global__ void __launch_bounds__(256) k_bucket_pair(const uint64_t *__restrict_
                                                      const uint32_t *__restrict__ bend, uint64_t *blist,
                                                      uint64_t *blist2) {
   __shared__ uint64_t a[kPairMax];
+  __shared__ uint32_t hist[kBucketRows];
+  __shared__ uint32_t scratch[17];
   __shared__ uint32_t s_bad;
   const uint32_t p = blockIdx.x;
   const uint64_t b = pstart[p], end = pstart[p + 1], lo = bend[p];
@@ -1135,24 +1137,27 @@ __global__ void __launch_bounds__(256) k_bucket_pair(const uint64_t *__restrict_
     for (uint64_t k = lo + threadIdx.x; k < end; k += blockDim.x) blist2[k] = 0;
     return;
   }
-  uint32_t m2 = 2;
-  while (m2 < m) m2 <<= 1;
-  // entries sort by rows, then id; padding sorts last
-  for (uint32_t i = threadIdx.x; i < m2; i += blockDim.x) a[i] = i < m ? blist[lo + i] : ~0ULL;
+  // counting sort by rows (1..kBucketRows-1): rank inside the row count's
+  // bin from the LDS histogram, bin offsets from its scan
+  for (uint32_t r = threadIdx.x; r < kBucketRows; r += blockDim.x) hist[r] = 0;
   __syncthreads();
-  for (uint32_t size = 2; size <= m2; size <<= 1)
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = threadIdx.x; i < m2 / 2; i += blockDim.x) {
-        const uint32_t x = 2 * i - (i & (stride - 1)), y = x + stride;
-        const bool up = (x & size) == 0;
-        const uint64_t u = a[x], v = a[y];
-        if ((u > v) == up) {
-          a[x] = v;
-          a[y] = u;
-        }
-      }
-      __syncthreads();
-    }
+  uint64_t ent[kPairMax / 256];
+  uint32_t rank[kPairMax / 256];
+#pragma unroll
+  for (uint32_t j = 0; j < kPairMax / 256; ++j) {
+    const uint32_t i = threadIdx.x + j * 256;
+    ent[j] = i < m ? blist[lo + i] : 0;
+    rank[j] = i < m ? atomicAdd(&hist[(uint32_t)(ent[j] >> 32) & (kBucketRows - 1)], 1u) : 0;
+  }
+  __syncthreads();
+  block_exclusive_scan_lds(hist, kBucketRows, scratch);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kPairMax / 256; ++j) {
+    const uint32_t i = threadIdx.x + j * 256;
+    if (i < m) a[hist[(uint32_t)(ent[j] >> 32) & (kBucketRows - 1)] + rank[j]] = ent[j];
+  }
+  __syncthreads();
   uint32_t klo = 0, khi = m / 2;
   while (klo < khi) {
     const uint32_t k = (klo + khi + 1) / 2;
