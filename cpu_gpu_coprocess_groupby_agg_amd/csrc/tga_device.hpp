@@ -914,8 +914,12 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
   ListEnt ent_next = list_entry(bl, b0, e0, 1, wave, nw);
   K kk[U];
   CellOf<VW> cel[U][NC];
+  // Blocks of two buckets come only with 8-byte keys (the host pairs no
+  // others): the kernels of 4-byte keys keep the one block load, whose
+  // registers the second form would add to.
+  constexpr bool kPairs = sizeof(K) == 8;
   if (have) {
-    if (ca != cnt) load_block<K, NC, VW, U, true>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel, base2, ca);
+    if (kPairs && ca != cnt) load_block<K, NC, VW, U, true>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel, base2, ca);
     else load_block<K, NC, VW, U>(keys, cols, ncols, coldt, base, cnt, lane, kk, cel);
   }
   // The first block is waited for here, so the loop is entered with no load
@@ -936,7 +940,7 @@ __device__ __forceinline__ void agg_rows_int(const K *__restrict__ keys, const v
     K nkk[U];
     CellOf<VW> ncel[U][NC];
     if (next) {
-      if (nca != ncnt)
+      if (kPairs && nca != ncnt)
         load_block<K, NC, VW, U, true>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel, nbase2, nca);
       else
         load_block<K, NC, VW, U>(keys, cols, ncols, coldt, nbase, ncnt, lane, nkk, ncel);

@@ -134,12 +134,13 @@ hipError_t launch_plan_chunks(const uint32_t *pbase, uint32_t P, uint64_t n, uin
                               hipStream_t st);
 // bucket headers -> per-partition bucket lists and the k_agg work plan
 // (items of at most about chmax rows); plan.blist/bcnt must point at bb's.
-// pair: the lists feed k_agg -- partly filled buckets of a partition are
-// paired into blocks of at most kBucketRows rows (plan.blist2 = bb.blist2).
+// for_agg: the lists feed k_agg (plan.blist2 = bb.blist2 is written); with
+// pair, partly filled buckets of a partition are paired into blocks of at
+// most kBucketRows rows (k_agg reads such blocks for 8-byte keys only).
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
-                               hipStream_t st, bool pair);
-// The pairing alone, over lists launch_bucket_lists made with pair = false.
-hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st);
+                               hipStream_t st, bool for_agg, bool pair);
+// The k_agg part alone, over lists launch_bucket_lists made for another use.
+hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st, bool pair);
 uint64_t compact_tiles(uint64_t slots_plus_one);
 // Output rows start at *base when base is non-null (after directly emitted groups).
 // adds the group count of g (occupied slots + the empty-key slot) to *out

@@ -1120,11 +1120,12 @@ __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict
 // the largest feasible k is found by bisection: feasibility only shrinks as
 // k grows).  Written: the k pairs (larger bucket in blist, smaller in
 // blist2), then the m - 2k unpaired buckets, then k entries of 0 rows, which
-// end the segment.  blist2 of the full buckets is 0.
+// end the segment.  blist2 of the full buckets is 0, and of all buckets
+// when pair is 0.
 constexpr uint32_t kPairMax = 1024;
 __global__ void __launch_bounds__(256) k_bucket_pair(const uint64_t *__restrict__ pstart,
                                                      const uint32_t *__restrict__ bend, uint64_t *blist,
-                                                     uint64_t *blist2) {
+                                                     uint64_t *blist2, uint32_t pair) {
   __shared__ uint64_t a[kPairMax];
   __shared__ uint32_t hist[kBucketRows];
   __shared__ uint32_t scratch[17];
@@ -1133,7 +1134,7 @@ __global__ void __launch_bounds__(256) k_bucket_pair(const uint64_t *__restrict_
   const uint64_t b = pstart[p], end = pstart[p + 1], lo = bend[p];
   for (uint64_t k = b + threadIdx.x; k < lo; k += blockDim.x) blist2[k] = 0;
   const uint32_t m = (uint32_t)(end - lo);
-  if (m < 2 || m > kPairMax) {
+  if (!pair || m < 2 || m > kPairMax) {
     for (uint64_t k = lo + threadIdx.x; k < end; k += blockDim.x) blist2[k] = 0;
     return;
   }
@@ -1991,7 +1992,7 @@ hipError_t launch_colscan(const uint32_t *cnt, uint32_t ntiles, uint32_t P, uint
 static hipError_t lds_attr(const void *fn, size_t bytes) { return raise_lds_limit(fn, bytes); }
 
 hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax, const AggPlan &plan,
-                               hipStream_t st, bool pair) {
+                               hipStream_t st, bool for_agg, bool pair) {
   const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(256, bb.cap / 4096));
   size_t lds = (size_t)P * 8;
   hipError_t e = lds_attr((const void *)&k_bucket_count, lds);
@@ -2007,11 +2008,12 @@ hipError_t launch_bucket_lists(const BucketBufs &bb, uint32_t P, uint64_t chmax,
   hipLaunchKernelGGL(k_bucket_list, dim3(grid), dim3(1024), lds, st, bb.bpart, bb.bcnt, bb.pool, bb.cap, P, bb.bcur,
                      bb.bend, bb.blist);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  return pair ? launch_bucket_pairs(bb, P, plan, st) : hipSuccess;
+  return for_agg ? launch_bucket_pairs(bb, P, plan, st, pair) : hipSuccess;
 }
 
-hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket_pair, dim3(P), dim3(256), 0, st, plan.pstart, bb.bend, bb.blist, bb.blist2);
+hipError_t launch_bucket_pairs(const BucketBufs &bb, uint32_t P, const AggPlan &plan, hipStream_t st, bool pair) {
+  hipLaunchKernelGGL(k_bucket_pair, dim3(P), dim3(256), 0, st, plan.pstart, bb.bend, bb.blist, bb.blist2,
+                     (uint32_t)pair);
   return hipGetLastError();
 }
 
