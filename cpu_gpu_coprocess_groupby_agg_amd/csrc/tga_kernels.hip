@@ -1077,10 +1077,11 @@ __global__ void __launch_bounds__(1024) k_bucket_plan(const uint32_t *pbk, const
 // Buckets grouped by partition: blist[pstart[p] ..] = the entries
 // rows << 32 | id of the buckets of p, the full ones first (any order), then
 // the partly filled ones (one per scatter workgroup that saw the partition).
-// Each workgroup counts its range of full buckets per partition in LDS,
-// reserves its share of every list with one atomic, then places its entries;
-// a partly filled bucket takes its place from the back with an atomic of its
-// own (bend[p] starts at pstart[p+1]).
+// Each workgroup counts its range of buckets per partition in LDS (full ones
+// in the low, partly filled ones in the high half of a word), reserves its
+// share of every list with one atomic from the front (bcur[p]) and one from
+// the back (bend[p], which starts at pstart[p+1]), then places its entries,
+// the full ones first.
 __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict__ bpart,
                                                       const uint32_t *__restrict__ bcnt, const uint32_t *pool,
                                                       uint32_t cap, uint32_t P, uint32_t *bcur, uint32_t *bend,
@@ -1092,21 +1093,36 @@ __global__ void __launch_bounds__(1024) k_bucket_list(const uint32_t *__restrict
   const uint32_t nb = min(*pool, cap);
   const uint32_t per = (nb + gridDim.x - 1) / gridDim.x;
   const uint32_t lo = blockIdx.x * per, hi = min(nb, lo + per);
+  // per < 2^16: a workgroup's counts of one partition fit half a word
+  const bool packed = per < (1u << 16);
   for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
     const uint32_t p = bpart[id];
-    if (p < P && bcnt[id] == kBucketRows) atomicAdd(&lc[p], 1u);
+    if (p < P && (bcnt[id] == kBucketRows || packed)) atomicAdd(&lc[p], bcnt[id] == kBucketRows ? 1u : 1u << 16);
   }
   __syncthreads();
   for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
-    lb[p] = lc[p] ? atomicAdd(&bcur[p], lc[p]) : 0u;
-    lc[p] = 0;
+    const uint32_t full = packed ? lc[p] & 0xffffu : lc[p];
+    lb[p] = full ? atomicAdd(&bcur[p], full) : 0u;
+    // the back share: its lowest slot, kept in lc's high half's place
+    lc[p] = packed ? lc[p] >> 16 : 0u;
   }
   __syncthreads();
+  if (packed) {
+    for (uint32_t p = threadIdx.x; p < P; p += blockDim.x) {
+      const uint32_t part = lc[p];
+      lc[p] = part ? atomicAdd(&bend[p], 0u - part) - part : 0u;
+    }
+  }
+  __syncthreads();
+  // full buckets: lb[p] + rank; partly filled: lc[p] + rank (packed), or a
+  // slot of their own from the back; ranks count up from the bases
   for (uint32_t id = lo + threadIdx.x; id < hi; id += blockDim.x) {
     const uint32_t p = bpart[id];
     if (p >= P) continue;
     const uint32_t rows = bcnt[id];
-    const uint32_t at = rows == kBucketRows ? lb[p] + atomicAdd(&lc[p], 1u) : atomicAdd(&bend[p], ~0u) - 1u;
+    const uint32_t at = rows == kBucketRows ? atomicAdd(&lb[p], 1u)
+                        : packed           ? atomicAdd(&lc[p], 1u)
+                                           : atomicAdd(&bend[p], ~0u) - 1u;
     blist[at] = ((uint64_t)rows << 32) | id;
   }
 }
